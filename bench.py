@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Committed txn/s of the batched CC + execute path on one MI355X.
+
+Workload (the plain-batch key-value configuration): a table of 10,000,000
+rows x 10 fields of 100 bytes (key = row id), filled from a seed in HBM; one
+batch of 2^20 transactions x 16 ops, keys Zipf(0.99) over the rows and
+distinct within a transaction, 50% READ of one field / 50% UPDATE of one
+field with a seeded 100-byte payload.  The batch is uploaded before the
+timed region; a step is one execution of the whole batch (CC + execute) on
+the device-resident table.  The headline mode is the ordered mode (every
+txn commits, each cell sees its ops in TID order); the lowest-TID-wins mode
+is reported beside it.
+
+Prints one JSON line (see the field list at the end of main()).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "committed txn/s + p50/p99 commit latency, served vs plain batch, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+TABLE_SEED = 0x4C54504D47000001
+BATCH_SEED = 0x4C54504D47000002
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_counts(batch, width: int) -> dict:
+    """Counts the algorithmic-byte formulas need (host side, numpy)."""
+    kind = (batch.meta & 3).astype(np.int64)
+    col = ((batch.meta >> 2) & 63).astype(np.int64)
+    cell = batch.rows.astype(np.int64) * 64 + col
+    n = len(kind)
+    order = np.argsort(cell, kind="stable")
+    sc, sk = cell[order], kind[order]
+    head = np.ones(n, bool)
+    head[1:] = sc[1:] != sc[:-1]
+    seg = np.cumsum(head) - 1
+    is_upd = (sk != 0).astype(np.int64)
+    # inclusive running "an update happened in this segment" (segment ids increase)
+    cm = np.maximum.accumulate((seg << 1) | is_upd) & 1
+    reads = sk == 0
+    snapshot_reads = int(np.count_nonzero(reads & (cm == 0)))
+    written_cells = int(np.unique(sc[sk != 0]).size)
+    return {"n_ops": n, "n_txns": batch.n_txns, "reads": int(reads.sum()),
+            "writes": int((sk != 0).sum()), "snapshot_reads": snapshot_reads,
+            "written_cells": written_cells, "width": width}
+
+
+def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | None:
+    """Bytes a kernel must move per launch (not counting re-reads or padding)."""
+    n, t, w = c["n_ops"], c["n_txns"], c["width"]
+    per = {
+        "mo_keys": 14 * n,                     # rows 4 + meta 2 in, key 4 + val 4 out
+        "rs_hist": 4 * n,                      # keys in
+        "rs_scatter": 16 * n,                  # key+val in and out
+        "mo_seg_scan": 8 * n,                  # sorted key+val in
+        # sorted key+val; per READ its 100-B result, plus the snapshot cell or
+        # the 8-B payload seed of the in-batch UPDATE it sees; 12-B final-write entry per cell
+        "mo_apply_reads": 8 * n + c["reads"] * w + c["snapshot_reads"] * w
+                          + 8 * (c["reads"] - c["snapshot_reads"]) + 12 * c["written_cells"],
+        # per written cell: entry 12 + payload seed 8 + cell write
+        "mo_final_writes": c["written_cells"] * (12 + 8 + w),
+    }
+    if commit is not None and batch is not None:
+        kind = batch.meta & 3
+        op_t = np.repeat(np.arange(batch.n_txns), np.diff(batch.txn_off.astype(np.int64)))
+        cm = commit[op_t] != 0
+        cr = int(np.count_nonzero(cm & (kind == 0)))
+        cw = int(np.count_nonzero(cm & (kind != 0)))
+        wr = int(np.count_nonzero(kind != 0))
+        per.update({
+            # meta 2 every op, row 4 + lock RMW 16 per write op, txn_off 4 per txn
+            "cc_reserve": 2 * n + 20 * wr + 4 * t,
+            # meta 2 + row 4 + lock 8 per op, txn_off 4 + commit flag 1 per txn,
+            # committed READ: cell in + result out
+            "cc_validate_read": 14 * n + 5 * t + cr * 2 * w,
+            # meta 2 + row 4 per op, commit flag 1 + txn_off 4 per txn,
+            # committed write: payload seed 8 + cell write
+            "cc_write": 6 * n + 5 * t + cw * (8 + w),
+        })
+    return per.get(kernel)
+
+
+def run_mode(e, batch, mode: int, steps: int, warmup: int) -> dict:
+    for _ in range(warmup):
+        e.execute(mode)
+        e.wait()
+    e.wait()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        e.execute(mode)
+        e.wait()
+        lat.append(time.perf_counter() - s0)
+    t1 = time.perf_counter()
+    st = e.stats()
+    return {"wall_s": t1 - t0, "lat": lat, "committed": st["committed"], "aborted": st["aborted"],
+            "exec_ms_last": st["exec_ms"], "upload_ms": st["upload_ms"]}
+
+
+def profile_mode(e, mode: int) -> dict:
+    """One extra step with per-kernel HIP events; stats() sums repeated stages."""
+    e.execute(mode, profile=True)
+    e.wait()
+    return e.stats()["kernels"]
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def cpu_baseline(batch, rows: int, width: int, n_cols: int, sample_txns: int, threads: int) -> dict:
+    import oracle_ffi as orc
+    from ltpmg_amd.engine import Batch
+    s = min(sample_txns, batch.n_txns)
+    n_ops = int(batch.txn_off[s])
+    sb = Batch(batch.txn_off[: s + 1].copy(), batch.rows[:n_ops].copy(), batch.meta[:n_ops].copy(),
+               batch.args[:n_ops].copy())
+    im = orc.Image({0: (rows, [width] * n_cols)})  # untouched pages stay unmapped
+    col = (sb.meta >> 2) & 63
+    for c in range(n_cols):
+        r = np.unique(sb.rows[col == c])
+        im.fill_cells(TABLE_SEED, 0, c, r)
+    times = []
+    for _ in range(3):
+        _, _, secs = orc.cpu_pipeline(im, sb, orc.MODE_O, 128, n_threads=threads)
+        times.append(secs)
+    secs = statistics.median(times)
+    return {"value": s / secs, "unit": "txn/s", "cores": threads, "kind": "port",
+            "sample": f"first {s} txns ({n_ops} ops) of the same batch, ordered mode, "
+                      f"C++ restatement of the reference pipeline (histogram, scan, scatter, "
+                      f"per-row TID sort, per-row apply), median of 3, {platform.processor() or 'host'}",
+            "seconds": secs}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=10_000_000)
+    ap.add_argument("--txns", type=int, default=1 << 20)
+    ap.add_argument("--ops", type=int, default=16)
+    ap.add_argument("--theta", type=float, default=0.99)
+    ap.add_argument("--cpu-sample-txns", type=int, default=1 << 18)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the batched transaction path on a "
+              "single MI355X (multi-GPU sharding is design-only); use --gpus 1.", file=sys.stderr)
+        return 2
+
+    from ltpmg_amd import MODE_A, MODE_O, OP_UPDATE, Engine, gen_kv
+
+    n_cols, width = 10, 100
+    log(f"generating batch: {args.txns} txns x {args.ops} ops, zipf {args.theta}, {args.rows} rows")
+    batch = gen_kv(args.txns, args.rows, n_cols, ops=args.ops, zipf_theta=args.theta, read_pct=50,
+                   write_kind=OP_UPDATE, seed=BATCH_SEED)
+    counts = batch_counts(batch, width)
+    log(f"batch: {batch.n_ops} ops, {counts['written_cells']} written cells")
+    e = Engine()
+    e.create_table(0, args.rows, [width] * n_cols)
+    e.fill_table(0, TABLE_SEED)
+    log("table filled")
+    t0 = time.perf_counter()
+    e.submit(batch)
+    e.wait()
+    upload_wall = time.perf_counter() - t0
+
+    res = {}
+    for name, mode in (("ordered", MODE_O), ("lowest_tid_wins", MODE_A)):
+        steps = args.steps if mode == MODE_O else max(1, args.steps // 2)
+        warm = args.warmup if mode == MODE_O else 1
+        r = run_mode(e, batch, mode, steps, warm)
+        log(f"{name}: {steps} steps in {r['wall_s']:.3f} s, committed {r['committed']}")
+        r["kernels_ms"] = profile_mode(e, mode)
+        commit, _ = e.fetch(results=False)
+        r["commit"] = commit
+        res[name] = r
+
+    head = res["ordered"]
+    value = head["committed"] * args.steps / head["wall_s"]
+    ms_per_step = 1000.0 * head["wall_s"] / args.steps
+
+    # roofline of the dominant kernel of the headline mode
+    ker = head["kernels_ms"]
+    passes = (max(1, int(np.ceil(np.log2(args.rows)))) + 4 + 7) // 8
+    launches = {k: (passes if k.startswith("rs_") else 1) for k in ker}
+    avg = {k: ker[k] / launches[k] for k in ker}
+    dom = max(avg, key=lambda k: avg[k])
+    alg = algorithmic_bytes(dom, counts)
+    achieved = alg / (avg[dom] * 1e-3) / 1e9 if alg else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                "traffic": None, "avg_launch_ms": avg[dom], "algorithmic_bytes": alg}
+    per_kernel = {}
+    for name, r in res.items():
+        for k, ms in r["kernels_ms"].items():
+            a = algorithmic_bytes(k, counts, r["commit"], batch)
+            l = passes if k.startswith("rs_") else 1
+            per_kernel[f"{name}/{k}"] = {
+                "ms": ms, "launches": l,
+                "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(batch, args.rows, width, n_cols, args.cpu_sample_txns, threads)
+
+    modes = {}
+    for name, r in res.items():
+        steps = len(r["lat"])
+        modes[name] = {
+            "committed_txn_per_s": r["committed"] * steps / r["wall_s"],
+            "committed": r["committed"], "aborted": r["aborted"],
+            "device_ms_per_batch": r["exec_ms_last"],
+            "p50_commit_latency_ms": 1000 * pct(r["lat"], 50),
+            "p99_commit_latency_ms": 1000 * pct(r["lat"], 99),
+        }
+    out = {
+        "metric": METRIC, "value": value, "unit": "txn/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded table fill and Zipf batch; no dataset)",
+        "config": {"workload": f"plain batch, ordered mode: KV table {args.rows} rows x {n_cols} "
+                               f"x {width} B fields, {args.txns} txns x {args.ops} ops, "
+                               f"zipf {args.theta}, 50% READ / 50% UPDATE",
+                   "rows": args.rows, "fields": n_cols, "field_bytes": width,
+                   "batch_txns": args.txns, "ops_per_txn": args.ops, "zipf_theta": args.theta,
+                   "mode": "ordered", "parallelism": "single GPU"},
+        "p50_commit_latency_ms": modes["ordered"]["p50_commit_latency_ms"],
+        "p99_commit_latency_ms": modes["ordered"]["p99_commit_latency_ms"],
+        "modes": modes,
+        "upload_ms": head["upload_ms"], "upload_wall_s": upload_wall,
+        "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu,
+        "batch_counts": counts,
+    }
+    print(json.dumps(out), flush=True)
+    e.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

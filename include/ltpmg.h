@@ -1,0 +1,169 @@
+/*
+ * ltpmg.h -- C ABI of libltpmg_hip.so, the MI355X batched transaction engine.
+ *
+ * The reference exposes its GPU path as eight C++ functions taking
+ * std::shared_ptr<Param>, std::vector<std::any> and table PODs
+ * (include/database.h:66-70, include/query.h:63-70, include/scheduler.h:33-37,
+ * defined in cuda/src/gpuscheduler.cu:3-47).  Only POD C types cross this
+ * ABI; the C++ shim in ltpmg_amd/host/gpu_shim.cpp keeps those eight
+ * signatures and calls the functions below.  Every function returns an int
+ * status (LTPMG_OK or a negative LTPMG_E*), with a message available from
+ * ltpmg_last_error().  Nothing here falls back to the CPU: without a usable
+ * GPU, ltpmg_open() fails with LTPMG_EDEVICE.
+ *
+ * Batch model.  A batch is n_txns transactions in CSR form: txn t owns ops
+ * [txn_off[t], txn_off[t+1]), 1..LTPMG_MAX_TXN_OPS of them, and its TID is t
+ * (its position in the batch, as src/query.cpp:284-292 assigns it).  An op
+ * is (row u32, meta u16, arg u64), meta = kind | col<<2 | table<<8, the SoA
+ * replacement of the reference's 52-byte Global_Txn record
+ * (cuda/include/txn_structure.cuh:21-41).
+ *
+ * Value semantics (the reference's row operators are no-ops,
+ * cuda/src/gpulauncher.cu:8-165, so these are this engine's definitions):
+ *   stream(s, j) = mix64(s + (j+1)*0x9E3779B97F4A7C15), mix64 = SplitMix64's
+ *   finaliser.  A cell's initial bytes from ltpmg_fill_table(seed) are the
+ *   little-endian bytes of stream(cellseed, 0..) with
+ *   cellseed = mix64(seed + 0x9E3779B97F4A7C15 * (1 + (row<<10 | table<<6 | col))).
+ *   READ      returns the cell (result slot bytes past the width are 0).
+ *   UPDATE    sets the cell to the bytes of stream(arg, 0..).
+ *   RMW_ADD   adds (uint32_t)arg to every little-endian u32 word of the cell.
+ * Within one txn a cell that is written must not be accessed by another op.
+ *
+ * Modes (both equal a serial replay in ascending TID of the committed txns):
+ *   LTPMG_MODE_A  lowest TID wins: W[row] = min TID writing the row; a txn
+ *                 commits iff no lower TID writes any row it touches (the
+ *                 commented atomicMin rule of gpulauncher.cu:414, 2130-2142).
+ *   LTPMG_MODE_O  ordered: every txn commits; each cell sees its ops in TID
+ *                 order (what filter_commit + commit intend,
+ *                 gpulauncher.cu:2439-2700).
+ */
+#ifndef LTPMG_H
+#define LTPMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LTPMG_MAX_TABLES 16
+#define LTPMG_MAX_COLS 64
+#define LTPMG_MAX_COL_BYTES 256
+#define LTPMG_MAX_TXN_OPS 256
+#define LTPMG_MAX_SLOTS 8
+
+#define LTPMG_OK 0
+#define LTPMG_EINVAL (-1)   /* bad argument or malformed batch */
+#define LTPMG_ENOMEM (-2)   /* device or pinned allocation failed */
+#define LTPMG_EBUDGET (-3)  /* request exceeds the configured HBM budget */
+#define LTPMG_EHIP (-4)     /* HIP runtime error */
+#define LTPMG_ESTATE (-5)   /* call out of order (no table / no batch) */
+#define LTPMG_EDEVICE (-6)  /* no usable GPU */
+
+#define LTPMG_OP_READ 0u
+#define LTPMG_OP_UPDATE 1u
+#define LTPMG_OP_RMW_ADD 2u
+#define LTPMG_META(kind, table, col) \
+    ((uint16_t)((kind) | ((uint32_t)(col) << 2) | ((uint32_t)(table) << 8)))
+
+#define LTPMG_MODE_A 0u
+#define LTPMG_MODE_O 1u
+#define LTPMG_EXEC_PROFILE 0x100u /* or-ed into the mode: record per-kernel HIP events */
+
+#define LTPMG_CFG_PROFILE 1u /* record per-kernel HIP events on every execute */
+
+typedef struct ltpmg_engine ltpmg_engine;
+
+typedef struct {
+    int32_t device;      /* HIP device ordinal */
+    uint32_t n_slots;    /* batch slots, 1..LTPMG_MAX_SLOTS */
+    uint32_t max_txns;   /* per-slot capacity */
+    uint32_t max_ops;    /* per-slot capacity */
+    uint64_t hbm_budget; /* bytes the engine may allocate; 0 = 90% of free HBM */
+    uint32_t flags;      /* LTPMG_CFG_* */
+} ltpmg_cfg;
+
+typedef struct {
+    uint32_t n_rows;
+    uint32_t n_cols;
+    uint32_t col_bytes[LTPMG_MAX_COLS]; /* multiple of 4, 4..LTPMG_MAX_COL_BYTES */
+} ltpmg_table_desc;
+
+typedef struct {
+    uint32_t n_txns;
+    uint32_t n_ops;
+    const uint32_t *txn_off; /* n_txns + 1 entries, txn_off[0] = 0 */
+    const uint32_t *rows;    /* n_ops */
+    const uint16_t *meta;    /* n_ops */
+    const uint64_t *args;    /* n_ops */
+} ltpmg_batch;
+
+#define LTPMG_MAX_KSTATS 24
+typedef struct {
+    uint32_t n_txns;
+    uint32_t n_ops;
+    uint32_t committed;
+    uint32_t aborted;
+    uint32_t mode;
+    float exec_ms;   /* device time from the first CC kernel to the end of execute */
+    float upload_ms; /* device time of the host-to-device batch copies */
+    uint32_t n_kernels;
+    char kernel_name[LTPMG_MAX_KSTATS][32];
+    float kernel_ms[LTPMG_MAX_KSTATS]; /* with LTPMG_CFG_PROFILE or LTPMG_EXEC_PROFILE */
+    uint64_t hbm_allocated;
+} ltpmg_batch_stats;
+
+/* Engine lifecycle.  Replaces GPUScheduler's global singleton
+ * (cuda/include/gpuscheduler.cuh:54) with an explicit handle. */
+int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out);
+int ltpmg_close(ltpmg_engine *e);
+const char *ltpmg_last_error(const ltpmg_engine *e);
+
+/* Tables in HBM: columnar, each column a 256-byte-aligned array of slots of
+ * col_bytes rounded up (4, 8, 16, then multiples of 32; a 100-byte field
+ * takes a 128-byte slot).  Replaces GPUdatabase::malloc_global_row /
+ * copy_to_global_row (cuda/src/gpudatabase.cu:9-238), whose row-major typed
+ * arrays (int/double/8xu32 strings) map to 4-, 8- and 32-byte columns. */
+int ltpmg_create_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *desc);
+int ltpmg_fill_table(ltpmg_engine *e, uint32_t table_id, uint64_t seed);
+int ltpmg_load_column(ltpmg_engine *e, uint32_t table_id, uint32_t col, uint32_t row0,
+                      uint32_t n_rows, const void *src); /* packed, col_bytes per row */
+int ltpmg_read_column(ltpmg_engine *e, uint32_t table_id, uint32_t col, uint32_t row0,
+                      uint32_t n_rows, void *dst);
+/* Order-independent digest of a table: sum over cells of a hash of
+ * (table, row, col, cell bytes).  Replaces nothing in the reference (it has
+ * no readback, gpudatabase.cu:240-339 frees without copying back). */
+int ltpmg_table_digest(ltpmg_engine *e, uint32_t table_id, uint64_t *out);
+/* Bytes per op in the result buffer: the largest slot over all tables. */
+uint32_t ltpmg_result_stride(const ltpmg_engine *e);
+
+/* Batches.  submit copies the SoA batch through pinned staging into the
+ * slot's device buffers (replaces GPUquery::copy_global_txn + query_parse,
+ * cuda/src/gpuquery.cu:593-2149); execute enqueues the CC + execute kernels
+ * on the slot's stream (replaces GPUlauncher::txn_kernel_launcher,
+ * cuda/src/gpulauncher.cu:755-1116); wait blocks until the slot is idle;
+ * fetch copies the commit flags (n_txns bytes) and the read results
+ * (n_ops * ltpmg_result_stride() bytes, slot i holds op i's READ) back.  The
+ * reference has no results path (Global_Txn_Result is never written). */
+int ltpmg_submit_batch(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *b);
+int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode);
+int ltpmg_wait(ltpmg_engine *e, uint32_t slot);
+int ltpmg_fetch_results(ltpmg_engine *e, uint32_t slot, uint8_t *commit, void *results,
+                        uint64_t results_bytes);
+int ltpmg_stats(ltpmg_engine *e, uint32_t slot, ltpmg_batch_stats *out);
+
+/* Building-block self-tests, the counterparts of the reference's in-binary
+ * LTPMG_GPULAUNCHER_TEST_MERGESORT and _PREFIXSUM checks
+ * (cuda/src/gpulauncher.cu:1237-1321).  sort: stable sort of (key, val)
+ * pairs by the low key_bits of key, in place on host arrays.  segscan:
+ * segmented exclusive sum of u32 values, a segment starting wherever
+ * head[i] != 0 (and at i = 0). */
+int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_t n,
+                        uint32_t key_bits);
+int ltpmg_selftest_segscan(ltpmg_engine *e, const uint32_t *vals, const uint8_t *head,
+                           uint32_t *out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

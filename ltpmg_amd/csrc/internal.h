@@ -1,0 +1,185 @@
+// internal.h -- shared device-side definitions of the MI355X engine.
+//
+// Layout in HBM (one GPU):
+//   column c of table t: an array of n_rows slots, slot = col_bytes rounded up
+//   (4, 8, 16, then a multiple of 32), base 256-byte aligned; the bytes of a
+//   slot past the column width are kept zero so whole slots can be moved with
+//   16-byte vector accesses.  A 100-byte field uses a 128-byte slot: exactly
+//   one 128-byte line per access.
+//   lock word of row r of table t: u64 {epoch:32 | ~tid:32}; atomicMax keeps
+//   the lowest TID of the current epoch, and words of older epochs read as
+//   "no writer", so the array is never cleared between batches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ltpmg.h"
+
+namespace ltpmg {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+constexpr uint32_t kBlock = 256;       // threads per workgroup (4 waves of 64)
+constexpr uint32_t kTileOps = 2048;    // nominal ops per CC tile
+constexpr uint32_t kTileOpsMax = kTileOps + LTPMG_MAX_TXN_OPS;
+constexpr uint32_t kSortItems = 16;    // keys per thread in a radix-sort tile
+constexpr uint32_t kSortTile = kBlock * kSortItems;
+constexpr uint32_t kNoSet = 0xFFFFFFFFu;     // ordered-mode state: no write seen
+constexpr uint32_t kRmwOnly = 0xFFFFFFFEu;   // ordered-mode state: only RMW_ADDs seen
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// j-th output of a SplitMix64 generator whose state starts at s.
+__host__ __device__ inline uint64_t stream64(uint64_t s, uint32_t j) {
+    return mix64(s + (uint64_t)(j + 1) * kGolden);
+}
+__host__ __device__ inline uint64_t cell_id(uint32_t table, uint32_t row, uint32_t col) {
+    return ((uint64_t)row << 10) | ((uint64_t)table << 6) | (uint64_t)col;
+}
+__host__ __device__ inline uint64_t cellseed(uint64_t seed, uint32_t table, uint32_t row,
+                                             uint32_t col) {
+    return mix64(seed + kGolden * (cell_id(table, row, col) + 1));
+}
+// u32 word w of the byte string stream(s, 0), stream(s, 1), ... (little endian)
+__host__ __device__ inline uint32_t stream_word(uint64_t s, uint32_t w) {
+    return (uint32_t)(stream64(s, w >> 1) >> ((w & 1u) * 32));
+}
+
+struct ColDesc {
+    uint8_t *base;
+    uint32_t slot;   // bytes between consecutive rows
+    uint32_t width;  // meaningful bytes of a cell
+};
+
+struct TabDesc {
+    unsigned long long *lock;
+    uint32_t n_rows;
+    uint32_t n_cols;
+    uint32_t row_base;  // first global row of this table in ordered-mode cell keys
+    uint32_t live;
+};
+
+struct OpView {
+    const uint32_t *txn_off;
+    const uint32_t *rows;
+    const uint16_t *meta;
+    const uint64_t *args;
+    uint32_t n_txns;
+    uint32_t n_ops;
+};
+
+__device__ inline uint32_t op_kind(uint16_t m) { return m & 3u; }
+__device__ inline uint32_t op_col(uint16_t m) { return (m >> 2) & 63u; }
+__device__ inline uint32_t op_tab(uint16_t m) { return (m >> 8) & 15u; }
+
+// popcount of the bits of m below this lane (wave64)
+__device__ inline uint32_t lanemask_lt_popc(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Vector helpers: 16-byte cell chunks.
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+__device__ inline u32x4 ld16(const uint8_t *p) {
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return {v.x, v.y, v.z, v.w};
+}
+__device__ inline void st16(uint8_t *p, u32x4 v) {
+    *reinterpret_cast<uint4 *>(p) = make_uint4(v.x, v.y, v.z, v.w);
+}
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+// streaming store for write-once outputs (read results)
+__device__ inline void st16_nt(uint8_t *p, u32x4 v) {
+    v4u32 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<v4u32 *>(p));
+}
+// 16-byte chunk starting at byte b of the UPDATE payload of width w
+__device__ inline u32x4 payload16(uint64_t arg, uint32_t b, uint32_t w) {
+    uint64_t lo = stream64(arg, b >> 3), hi = stream64(arg, (b >> 3) + 1);
+    u32x4 v{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+    if (b + 16 > w) {
+        if (b + 4 > w) v.x = 0;
+        if (b + 8 > w) v.y = 0;
+        if (b + 12 > w) v.z = 0;
+        v.w = 0;
+    }
+    return v;
+}
+__device__ inline u32x4 add16(u32x4 v, uint32_t d, uint32_t b, uint32_t w) {
+    if (b < w) v.x += d;
+    if (b + 4 < w) v.y += d;
+    if (b + 8 < w) v.z += d;
+    if (b + 12 < w) v.w += d;
+    return v;
+}
+
+// ---- launchers (defined in the kernel translation units) ----
+struct DevTables {
+    const ColDesc *cols;  // [LTPMG_MAX_TABLES * LTPMG_MAX_COLS]
+    const TabDesc *tabs;  // [LTPMG_MAX_TABLES]
+};
+
+void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
+                        uint32_t n_rows, uint64_t seed);
+void launch_digest_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
+                          uint32_t n_rows, unsigned long long *acc);
+
+void launch_tile_bounds(hipStream_t s, const uint32_t *txn_off, uint32_t n_txns,
+                        uint32_t n_tiles, uint32_t *tile_t0);
+void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
+void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
+                       uint32_t n_tiles, uint32_t epoch);
+void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
+                             uint32_t n_tiles, uint32_t epoch, uint8_t *commit,
+                             uint8_t *results, uint32_t res_stride, uint32_t *counters);
+void launch_cc_write(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
+                     uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot);
+
+struct SortBufs {
+    uint32_t *keys[2];
+    uint32_t *vals[2];
+    uint32_t *hist;      // 256 * n_tiles
+    uint32_t *partials;  // scan partials
+};
+// Sorts (keys[0], vals[0]) by the low key_bits bits, stably; returns which
+// buffer index holds the result.
+int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits);
+// one LSD pass, split so each kernel can be timed: hist, scan, scatter
+void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
+void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n);
+void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
+void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
+                           uint32_t *partials);
+size_t scan_partials_needed(uint32_t n);
+
+struct OrderBufs {
+    SortBufs sort;
+    uint32_t *tile_flag;    // per seg-scan tile: has a segment head
+    uint32_t *tile_set;     // per tile aggregate state
+    uint32_t *tile_delta;
+    uint32_t *carry_set;    // per tile carry-in
+    uint32_t *carry_delta;
+    uint32_t *fin;          // final-write list: 3 u32 per entry {key, set, delta}
+    uint32_t *counters;     // [0] unused, [1] final-write list length
+};
+void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, uint32_t *keys,
+                    uint32_t *vals);
+// ordered-mode stages, run in this order after launch_mo_keys + launch_radix_sort
+void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted);
+void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
+                           uint32_t col_bits, uint32_t n_tabs, uint8_t *results,
+                           uint32_t res_stride);
+void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
+                            uint32_t col_bits, uint32_t n_tabs, uint32_t res_stride);
+uint32_t seg_tiles(uint32_t n);
+void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
+void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
+                             uint32_t *out, uint32_t n, uint32_t *tile_scratch);
+
+}  // namespace ltpmg

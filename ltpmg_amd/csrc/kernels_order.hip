@@ -1,0 +1,562 @@
+// kernels_order.hip -- ordered mode: every txn commits and each cell sees its
+// ops in TID order.
+//
+// The reference builds per-row op lists (histogram, cnt_TID atomics,
+// gpulauncher.cu:413-417; segment offsets, 1558-1589; scatter, 2106-2437),
+// merge-sorts each row's list by TID with one warp per row (2439-2523) and
+// applies the list serially (2525-2700).  Under zipf keys the hottest row's
+// serial list is the tail.  Here the order is made data-parallel instead:
+//
+//   mo_keys       key = (global row << col_bits) | col, val = op | kind<<30
+//   radix sort    stable LSD sort on key, 8-bit digits (ops are already in
+//                 TID order, so a stable sort yields per-cell TID order)
+//   seg reduce /  segmented scan of the per-cell state monoid
+//   carry / apply   state = (last UPDATE op or none, sum of RMW deltas since),
+//                 combine(a, b) = b has an UPDATE ? b : (a.set, a.d + b.d);
+//                 a READ's value = (set ? payload(args[set]) : snapshot) + d,
+//                 so no cell is ever walked serially
+//   final writes  one write per touched cell (the segment's inclusive state),
+//                 in a later kernel so every snapshot read has finished.
+#include "internal.h"
+
+namespace ltpmg {
+
+namespace {
+
+constexpr uint32_t kSegItems = 8;
+constexpr uint32_t kSegTile = kBlock * kSegItems;
+
+struct St {
+    uint32_t set, d;
+};
+struct Seg {
+    uint32_t f, set, d;
+};
+
+__device__ inline St combine(St a, St b) {
+    if (b.set < kRmwOnly) return b;
+    uint32_t s = a.set < kRmwOnly ? a.set : (a.set < b.set ? a.set : b.set);
+    return {s, a.d + b.d};
+}
+__device__ inline Seg seg_op(Seg a, Seg b) {
+    if (b.f) return b;
+    St c = combine({a.set, a.d}, {b.set, b.d});
+    return {a.f, c.set, c.d};
+}
+__device__ inline St elem_state(uint32_t val, const uint64_t *args) {
+    uint32_t kind = val >> 30, o = val & 0x3FFFFFFFu;
+    if (kind == LTPMG_OP_UPDATE) return {o, 0u};
+    if (kind == LTPMG_OP_RMW_ADD) return {kRmwOnly, (uint32_t)args[o]};
+    return {kNoSet, 0u};
+}
+
+// Exclusive segmented scan over the block's 256 thread aggregates.
+__device__ inline Seg block_seg_exclusive(Seg x, Seg *total) {
+    __shared__ Seg ws[kBlock / 64];
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    Seg inc = x;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        Seg y{(uint32_t)__shfl_up((int)inc.f, off), (uint32_t)__shfl_up((int)inc.set, off),
+              (uint32_t)__shfl_up((int)inc.d, off)};
+        if (lane >= off) inc = seg_op(y, inc);
+    }
+    Seg ex{(uint32_t)__shfl_up((int)inc.f, 1), (uint32_t)__shfl_up((int)inc.set, 1),
+           (uint32_t)__shfl_up((int)inc.d, 1)};
+    if (lane == 0) ex = Seg{0u, kNoSet, 0u};
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    Seg pre{0u, kNoSet, 0u};
+    for (uint32_t i = 0; i < w; ++i) pre = seg_op(pre, ws[i]);
+    Seg tot{0u, kNoSet, 0u};
+    for (uint32_t i = 0; i < kBlock / 64; ++i) tot = seg_op(tot, ws[i]);
+    __syncthreads();
+    *total = tot;
+    return seg_op(pre, ex);
+}
+
+// Element sources for the segmented scan.
+struct OrderSrc {
+    const uint32_t *keys, *vals;
+    const uint64_t *args;
+    uint32_t n;
+    __device__ bool head(uint32_t i) const { return i == 0 || keys[i] != keys[i - 1]; }
+    __device__ St state(uint32_t i) const { return elem_state(vals[i], args); }
+};
+struct SelftestSrc {
+    const uint32_t *vals;
+    const uint8_t *heads;
+    uint32_t n;
+    __device__ bool head(uint32_t i) const { return i == 0 || heads[i] != 0; }
+    __device__ St state(uint32_t i) const { return {kRmwOnly, vals[i]}; }
+};
+
+template <class Src>
+__device__ inline Seg thread_aggregate(const Src &src, uint32_t first) {
+    Seg a{0u, kNoSet, 0u};
+    for (uint32_t j = 0; j < kSegItems; ++j) {
+        uint32_t i = first + j;
+        if (i >= src.n) break;
+        St s = src.state(i);
+        a = seg_op(a, Seg{src.head(i) ? 1u : 0u, s.set, s.d});
+    }
+    return a;
+}
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_seg_reduce(Src src, uint32_t *tflag, uint32_t *tset,
+                                                       uint32_t *tdelta) {
+    uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
+    Seg tot;
+    block_seg_exclusive(thread_aggregate(src, first), &tot);
+    if (threadIdx.x == 0) {
+        tflag[blockIdx.x] = tot.f;
+        tset[blockIdx.x] = tot.set;
+        tdelta[blockIdx.x] = tot.d;
+    }
+}
+
+// One block: carry[k] = segmented fold of tile aggregates 0..k-1.
+__global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
+                                                      const uint32_t *tdelta, uint32_t n_tiles,
+                                                      uint32_t *cset, uint32_t *cdelta) {
+    Seg run{0u, kNoSet, 0u};
+    for (uint32_t base = 0; base < n_tiles; base += kBlock) {
+        uint32_t k = base + threadIdx.x;
+        Seg x = k < n_tiles ? Seg{tflag[k], tset[k], tdelta[k]} : Seg{0u, kNoSet, 0u};
+        Seg tot;
+        Seg ex = block_seg_exclusive(x, &tot);
+        Seg c = seg_op(run, ex);
+        if (k < n_tiles) {
+            cset[k] = c.set;
+            cdelta[k] = c.d;
+        }
+        run = seg_op(run, tot);
+    }
+}
+
+// Per-thread walk with the exclusive prefix of the thread's first element.
+template <class Src>
+__device__ inline St thread_prefix(const Src &src, const uint32_t *cset, const uint32_t *cdelta,
+                                   uint32_t first) {
+    Seg tot;
+    Seg ex = block_seg_exclusive(thread_aggregate(src, first), &tot);
+    Seg p = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, ex);
+    return {p.set, p.d};
+}
+
+__global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, const uint32_t *cset,
+                                                           const uint32_t *cdelta, uint32_t *out) {
+    uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
+    St run = thread_prefix(src, cset, cdelta, first);
+    for (uint32_t j = 0; j < kSegItems; ++j) {
+        uint32_t i = first + j;
+        if (i >= src.n) break;
+        St s = src.state(i);
+        bool h = src.head(i);
+        St ex = h ? St{kNoSet, 0u} : run;
+        out[i] = ex.d;
+        run = h ? s : combine(run, s);
+    }
+}
+
+struct KeyDecode {
+    const TabDesc *tabs;
+    uint32_t col_bits, n_tabs;
+    // global row -> (table, row); returns false when out of range
+    __device__ bool operator()(uint32_t key, uint32_t &tab, uint32_t &row, uint32_t &col) const {
+        uint32_t g = key >> col_bits;
+        col = key & ((1u << col_bits) - 1u);
+        for (uint32_t t = 0; t < n_tabs; ++t) {
+            const TabDesc &td = tabs[t];
+            if (td.live && g >= td.row_base && g - td.row_base < td.n_rows) {
+                tab = t;
+                row = g - td.row_base;
+                return col < td.n_cols;
+            }
+        }
+        return false;
+    }
+};
+
+// Value of a cell after state st: (st has an UPDATE ? payload : cell) + d,
+// 16 bytes at byte b (VEC16) or one word.
+template <int G, bool VEC16>
+__device__ inline void emit_value(const ColDesc &cd, const uint8_t *cell, St st,
+                                  const uint64_t *args, uint8_t *dst, uint32_t span, uint32_t lane,
+                                  bool nontemporal) {
+    const bool has_set = st.set < kRmwOnly;
+    const uint64_t arg = has_set ? args[st.set] : 0;
+    if (VEC16 && cd.slot >= 16) {
+        for (uint32_t b = lane * 16; b < span; b += G * 16) {
+            u32x4 v{0, 0, 0, 0};
+            if (b < cd.slot) {
+                v = has_set ? payload16(arg, b, cd.width) : ld16(cell + b);
+                v = add16(v, st.d, b, cd.width);
+            }
+            if (nontemporal) st16_nt(dst + b, v);
+            else st16(dst + b, v);
+        }
+    } else {
+        for (uint32_t b = lane * 4; b < span; b += G * 4) {
+            uint32_t v = 0;
+            if (b < cd.width) {
+                v = has_set ? stream_word(arg, b >> 2) : *reinterpret_cast<const uint32_t *>(cell + b);
+                v += st.d;
+            }
+            *reinterpret_cast<uint32_t *>(dst + b) = v;
+        }
+    }
+}
+
+template <int G, bool VEC16>
+__global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *cset,
+                                                           const uint32_t *cdelta, DevTables dt,
+                                                           KeyDecode dec, uint8_t *results,
+                                                           uint32_t res_stride, uint32_t *fin,
+                                                           uint32_t *fin_count) {
+    __shared__ uint4 list[kSegTile];  // {op, set, d, key}
+    __shared__ uint32_t n_list;
+    if (threadIdx.x == 0) n_list = 0;
+    const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
+    St run = thread_prefix(src, cset, cdelta, first);  // contains __syncthreads
+    for (uint32_t j = 0; j < kSegItems; ++j) {
+        uint32_t i = first + j;
+        bool rd = false, wr = false;
+        uint32_t key = 0, op = 0;
+        St ex{kNoSet, 0u};
+        if (i < src.n) {
+            key = src.keys[i];
+            uint32_t val = src.vals[i];
+            op = val & 0x3FFFFFFFu;
+            St s = elem_state(val, src.args);
+            bool h = i == 0 || src.keys[i - 1] != key;
+            ex = h ? St{kNoSet, 0u} : run;
+            run = h ? s : combine(run, s);
+            rd = (val >> 30) == LTPMG_OP_READ;
+            bool last = i + 1 == src.n || src.keys[i + 1] != key;
+            wr = last && run.set != kNoSet;
+        }
+        // READ -> LDS list (one LDS atomic per wave)
+        uint64_t m = __ballot(rd);
+        if (m) {
+            uint32_t leader = __builtin_ctzll(m), base = 0;
+            if (__lane_id() == leader) base = atomicAdd(&n_list, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader);
+            if (rd) list[base + lanemask_lt_popc(m)] = make_uint4(op, ex.set, ex.d, key);
+        }
+        // segment end with a write -> global final-write list
+        uint64_t mw = __ballot(wr);
+        if (mw) {
+            uint32_t leader = __builtin_ctzll(mw), base = 0;
+            if (__lane_id() == leader) base = atomicAdd(fin_count, (uint32_t)__popcll(mw));
+            base = __shfl(base, (int)leader);
+            if (wr) {
+                uint32_t p = (base + lanemask_lt_popc(mw)) * 3;
+                fin[p] = key;
+                fin[p + 1] = run.set;
+                fin[p + 2] = run.d;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t n = n_list;
+    const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G;
+    for (uint32_t k = g; k < n; k += kBlock / G) {
+        uint4 e = list[k];
+        uint32_t tab, row, col;
+        if (!dec(e.w, tab, row, col)) continue;
+        ColDesc cd = dt.cols[tab * LTPMG_MAX_COLS + col];
+        const uint8_t *cell = cd.base + (uint64_t)row * cd.slot;
+        emit_value<G, VEC16>(cd, cell, St{e.y, e.z}, src.args,
+                             results + (uint64_t)e.x * res_stride, res_stride, lane, true);
+    }
+}
+
+template <int G, bool VEC16>
+__global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint32_t *fin,
+                                                            const uint32_t *fin_count,
+                                                            DevTables dt, KeyDecode dec,
+                                                            const uint64_t *args) {
+    const uint32_t n = *fin_count;
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    const uint32_t groups = gridDim.x * blockDim.x / G;
+    for (uint32_t k = g; k < n; k += groups) {
+        uint32_t key = fin[3 * k], set = fin[3 * k + 1], d = fin[3 * k + 2];
+        uint32_t tab, row, col;
+        if (!dec(key, tab, row, col)) continue;
+        ColDesc cd = dt.cols[tab * LTPMG_MAX_COLS + col];
+        uint8_t *cell = cd.base + (uint64_t)row * cd.slot;
+        emit_value<G, VEC16>(cd, cell, St{set, d}, args, cell, cd.slot, lane, false);
+    }
+}
+
+// ---------------- radix sort ----------------
+
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *keys, uint32_t n,
+                                                    uint32_t shift, uint32_t n_tiles,
+                                                    uint32_t *hist) {
+    __shared__ uint32_t h[kBlock / 64][256];
+    for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (uint32_t it = 0; it < kSortItems; ++it) {
+        uint32_t idx = base + it * kBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[w][(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t d = threadIdx.x;  // kBlock == 256 digits
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < kBlock / 64; ++i) c += h[i][d];
+    hist[(uint64_t)d * n_tiles + blockIdx.x] = c;
+}
+
+// Stable scatter: wave w of tile k ranks its 16 chunks of 64 keys in order;
+// lanes holding the same digit are found with 8 ballots (wave64 multisplit).
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin,
+                                                       uint32_t *kout, uint32_t *vout,
+                                                       uint32_t n, uint32_t shift,
+                                                       uint32_t n_tiles, const uint32_t *offs) {
+    __shared__ uint32_t cnt[kBlock / 64][256];
+    __shared__ uint32_t gbase[256];
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
+    gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+    __syncthreads();
+    const uint32_t wbase = blockIdx.x * kSortTile + w * (kSortTile / (kBlock / 64));
+    uint32_t k[kSortItems], v[kSortItems], r[kSortItems];
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        uint32_t idx = wbase + c * 64 + lane;
+        bool valid = idx < n;
+        uint32_t key = valid ? kin[idx] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
+        uint32_t prior = 0;
+        if (valid && lane == leader) {
+            prior = cnt[w][d];
+            cnt[w][d] = prior + (uint32_t)__popcll(peers);
+        }
+        prior = __shfl(prior, (int)leader);
+        r[c] = prior + lanemask_lt_popc(peers);
+        k[c] = key;
+        v[c] = valid ? vin[idx] : 0u;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {  // exclusive offsets of each wave within the tile, per digit
+        uint32_t d = threadIdx.x, run = 0;
+        for (uint32_t i = 0; i < kBlock / 64; ++i) {
+            uint32_t t = cnt[i][d];
+            cnt[i][d] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        uint32_t idx = wbase + c * 64 + lane;
+        if (idx < n) {
+            uint32_t d = (k[c] >> shift) & 255u;
+            uint32_t pos = gbase[d] + cnt[w][d] + r[c];
+            kout[pos] = k[c];
+            vout[pos] = v[c];
+        }
+    }
+}
+
+// ---------------- exclusive scan of u32 ----------------
+
+constexpr uint32_t kScanItems = 16;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+
+__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = x;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t i = 0; i < kBlock / 64; ++i) {
+        if (i < w) pre += ws[i];
+        tot += ws[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *in, uint32_t n,
+                                                        uint32_t *partials) {
+    uint32_t first = blockIdx.x * kScanTile + threadIdx.x * kScanItems, s = 0;
+    for (uint32_t j = 0; j < kScanItems; ++j)
+        if (first + j < n) s += in[first + j];
+    uint32_t tot;
+    block_exclusive_sum(s, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t *partials, uint32_t np) {
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < np; base += kBlock) {
+        uint32_t k = base + threadIdx.x;
+        uint32_t x = k < np ? partials[k] : 0u, tot;
+        uint32_t ex = block_exclusive_sum(x, &tot);
+        if (k < np) partials[k] = run + ex;
+        run += tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32_t *out,
+                                                      uint32_t n, const uint32_t *partials) {
+    uint32_t first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], s = 0;
+    for (uint32_t j = 0; j < kScanItems; ++j) {
+        v[j] = first + j < n ? in[first + j] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_sum(s, &tot) + partials[blockIdx.x];
+    for (uint32_t j = 0; j < kScanItems; ++j) {
+        if (first + j < n) out[first + j] = run;
+        run += v[j];
+    }
+}
+
+// ---------------- mode O keys ----------------
+
+__global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uint32_t col_bits,
+                                                    uint32_t *keys, uint32_t *vals) {
+    for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < ov.n_ops;
+         o += gridDim.x * blockDim.x) {
+        uint16_t m = ov.meta[o];
+        const TabDesc &td = dt.tabs[op_tab(m)];
+        keys[o] = ((td.row_base + ov.rows[o]) << col_bits) | op_col(m);
+        vals[o] = o | (op_kind(m) << 30);
+    }
+}
+
+}  // namespace
+
+size_t scan_partials_needed(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
+                           uint32_t *partials) {
+    if (n == 0) return;
+    uint32_t nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n, partials);
+}
+
+void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
+    const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(k_rs_hist, dim3(n_tiles), dim3(kBlock), 0, s, b.keys[cur], n, shift,
+                       n_tiles, b.hist);
+}
+
+void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n) {
+    const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
+    launch_scan_exclusive(s, b.hist, b.hist, 256u * n_tiles, b.partials);
+}
+
+void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
+    const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(n_tiles), dim3(kBlock), 0, s, b.keys[cur], b.vals[cur],
+                       b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, n_tiles, b.hist);
+}
+
+int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits) {
+    int cur = 0;
+    if (n <= 1) return cur;
+    for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+        launch_rs_hist(s, b, cur, n, shift);
+        launch_rs_scan(s, b, n);
+        launch_rs_scatter(s, b, cur, n, shift);
+        cur ^= 1;
+    }
+    return cur;
+}
+
+void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, uint32_t *keys,
+                    uint32_t *vals) {
+    uint32_t blocks = (ov.n_ops + kBlock - 1) / kBlock;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, col_bits, keys, vals);
+}
+
+#define LTPMG_MO_DISPATCH(KERNEL, GRID, ...)                                               \
+    do {                                                                                   \
+        if (res_stride >= 128)                                                             \
+            hipLaunchKernelGGL((KERNEL<8, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else if (res_stride >= 64)                                                         \
+            hipLaunchKernelGGL((KERNEL<4, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else if (res_stride >= 32)                                                         \
+            hipLaunchKernelGGL((KERNEL<2, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else if (res_stride >= 16)                                                         \
+            hipLaunchKernelGGL((KERNEL<1, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else                                                                               \
+            hipLaunchKernelGGL((KERNEL<1, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+    } while (0)
+
+void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
+    const uint32_t n = ov.n_ops;
+    launch_zero_u32(s, ob.counters, 2);
+    if (n == 0) return;
+    const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
+    OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
+    hipLaunchKernelGGL(k_seg_reduce<OrderSrc>, dim3(n_tiles), dim3(kBlock), 0, s, src,
+                       ob.tile_flag, ob.tile_set, ob.tile_delta);
+    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, ob.tile_flag, ob.tile_set,
+                       ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta);
+}
+
+void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
+                           uint32_t col_bits, uint32_t n_tabs, uint8_t *results,
+                           uint32_t res_stride) {
+    const uint32_t n = ov.n_ops;
+    if (n == 0) return;
+    const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
+    OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
+    KeyDecode dec{dt.tabs, col_bits, n_tabs};
+    LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.carry_set, ob.carry_delta, dt, dec,
+                      results, res_stride, ob.fin, ob.counters + 1);
+}
+
+void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
+                            uint32_t col_bits, uint32_t n_tabs, uint32_t res_stride) {
+    const uint32_t n = ov.n_ops;
+    if (n == 0) return;
+    KeyDecode dec{dt.tabs, col_bits, n_tabs};
+    uint32_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 8192) blocks = 8192;
+    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.fin, ob.counters + 1, dt, dec, ov.args);
+}
+
+void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
+                             uint32_t *out, uint32_t n, uint32_t *scratch) {
+    if (n == 0) return;
+    const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
+    uint32_t *tf = scratch, *ts = scratch + n_tiles, *td = scratch + 2 * n_tiles;
+    uint32_t *cs = scratch + 3 * n_tiles, *cd = scratch + 4 * n_tiles;
+    SelftestSrc src{vals, head, n};
+    hipLaunchKernelGGL(k_seg_reduce<SelftestSrc>, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts,
+                       td);
+    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, tf, ts, td, n_tiles, cs, cd);
+    hipLaunchKernelGGL(k_selftest_apply, dim3(n_tiles), dim3(kBlock), 0, s, src, cs, cd, out);
+}
+
+uint32_t seg_tiles(uint32_t n) { return (n + kSegTile - 1) / kSegTile; }
+
+}  // namespace ltpmg

@@ -1,0 +1,328 @@
+"""ctypes binding of the engine's C ABI (include/ltpmg.h) and of the workload
+generator (include/ltpmg_workload.h).
+
+The engine runs only on the GPU: there is no CPU fallback, and loading fails
+with a RuntimeError when libltpmg_hip.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+MAX_TABLES = 16
+MAX_COLS = 64
+MAX_TXN_OPS = 256
+MAX_SLOTS = 8
+OP_READ, OP_UPDATE, OP_RMW_ADD = 0, 1, 2
+MODE_A, MODE_O = 0, 1
+EXEC_PROFILE = 0x100
+CFG_PROFILE = 1
+
+STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EBUDGET", -4: "EHIP", -5: "ESTATE",
+          -6: "EDEVICE"}
+
+
+def meta(kind: int, table: int, col: int) -> int:
+    return kind | (col << 2) | (table << 8)
+
+
+class LtpmgError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} -> {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Cfg(C.Structure):
+    _fields_ = [("device", C.c_int32), ("n_slots", C.c_uint32), ("max_txns", C.c_uint32),
+                ("max_ops", C.c_uint32), ("hbm_budget", C.c_uint64), ("flags", C.c_uint32)]
+
+
+class _TableDesc(C.Structure):
+    _fields_ = [("n_rows", C.c_uint32), ("n_cols", C.c_uint32),
+                ("col_bytes", C.c_uint32 * MAX_COLS)]
+
+
+class _Batch(C.Structure):
+    _fields_ = [("n_txns", C.c_uint32), ("n_ops", C.c_uint32),
+                ("txn_off", C.POINTER(C.c_uint32)), ("rows", C.POINTER(C.c_uint32)),
+                ("meta", C.POINTER(C.c_uint16)), ("args", C.POINTER(C.c_uint64))]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("n_txns", C.c_uint32), ("n_ops", C.c_uint32), ("committed", C.c_uint32),
+                ("aborted", C.c_uint32), ("mode", C.c_uint32), ("exec_ms", C.c_float),
+                ("upload_ms", C.c_float), ("n_kernels", C.c_uint32),
+                ("kernel_name", (C.c_char * 32) * 24), ("kernel_ms", C.c_float * 24),
+                ("hbm_allocated", C.c_uint64)]
+
+
+class _KvSpec(C.Structure):
+    _fields_ = [("n_txns", C.c_uint32), ("ops_min", C.c_uint32), ("ops_max", C.c_uint32),
+                ("n_rows", C.c_uint32), ("zipf_theta", C.c_double), ("n_cols", C.c_uint32),
+                ("read_pct", C.c_uint32), ("write_kind", C.c_uint32),
+                ("read_all_cols", C.c_uint32), ("table_id", C.c_uint32),
+                ("rmw_delta", C.c_int32), ("seed", C.c_uint64)]
+
+
+# Every symbol include/ltpmg.h declares, with its ctypes signature.
+ENGINE_API = {
+    "ltpmg_open": (C.c_int, [C.POINTER(_Cfg), C.POINTER(C.c_void_p)]),
+    "ltpmg_close": (C.c_int, [C.c_void_p]),
+    "ltpmg_last_error": (C.c_char_p, [C.c_void_p]),
+    "ltpmg_create_table": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_TableDesc)]),
+    "ltpmg_fill_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "ltpmg_load_column": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_void_p]),
+    "ltpmg_read_column": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_void_p]),
+    "ltpmg_table_digest": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "ltpmg_result_stride": (C.c_uint32, [C.c_void_p]),
+    "ltpmg_submit_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Batch)]),
+    "ltpmg_execute": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "ltpmg_wait": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "ltpmg_fetch_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.c_uint64]),
+    "ltpmg_stats": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Stats)]),
+    "ltpmg_selftest_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_uint32]),
+    "ltpmg_selftest_segscan": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_uint32]),
+}
+
+WORKLOAD_API = {
+    "ltpmg_gen_kv_max_ops": (C.c_uint64, [C.POINTER(_KvSpec)]),
+    "ltpmg_gen_kv": (C.c_int, [C.POINTER(_KvSpec), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32]),
+    "ltpmg_zeta": (C.c_double, [C.c_uint64, C.c_double]),
+}
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def _load(name: str, api: dict) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = LIB_DIR / name
+    if not path.exists():
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() "
+                           "(python -m ltpmg_amd.build) first")
+    lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+    for fn, (res, args) in api.items():
+        f = getattr(lib, fn)
+        f.restype = res
+        f.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def hip_lib() -> C.CDLL:
+    return _load("libltpmg_hip.so", ENGINE_API)
+
+
+def host_lib() -> C.CDLL:
+    hip_lib()  # the host library links against it
+    return _load("libltpmg_host.so", WORKLOAD_API)
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class Batch:
+    """SoA batch: txn t owns ops [txn_off[t], txn_off[t+1]); its TID is t."""
+    txn_off: np.ndarray  # uint32 [n_txns + 1]
+    rows: np.ndarray     # uint32 [n_ops]
+    meta: np.ndarray     # uint16 [n_ops]
+    args: np.ndarray     # uint64 [n_ops]
+
+    @property
+    def n_txns(self) -> int:
+        return len(self.txn_off) - 1
+
+    @property
+    def n_ops(self) -> int:
+        return len(self.rows)
+
+    def normalized(self) -> "Batch":
+        return Batch(np.ascontiguousarray(self.txn_off, np.uint32),
+                     np.ascontiguousarray(self.rows, np.uint32),
+                     np.ascontiguousarray(self.meta, np.uint16),
+                     np.ascontiguousarray(self.args, np.uint64))
+
+    def c_struct(self) -> _Batch:
+        return _Batch(self.n_txns, self.n_ops,
+                      self.txn_off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                      self.rows.ctypes.data_as(C.POINTER(C.c_uint32)),
+                      self.meta.ctypes.data_as(C.POINTER(C.c_uint16)),
+                      self.args.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    @staticmethod
+    def from_txns(txns: list[list[tuple[int, int, int, int, int]]]) -> "Batch":
+        """txns: per txn a list of (kind, table, col, row, arg)."""
+        off = [0]
+        rows, metas, args = [], [], []
+        for ops in txns:
+            for kind, table, col, row, arg in ops:
+                rows.append(row)
+                metas.append(meta(kind, table, col))
+                args.append(arg & 0xFFFFFFFFFFFFFFFF)
+            off.append(len(rows))
+        return Batch(np.array(off, np.uint32), np.array(rows, np.uint32),
+                     np.array(metas, np.uint16), np.array(args, np.uint64))
+
+
+def gen_kv(n_txns: int, n_rows: int, n_cols: int, *, ops: int | tuple[int, int] = 16,
+           zipf_theta: float = 0.0, read_pct: int = 50, write_kind: int = OP_UPDATE,
+           read_all_cols: bool = False, table_id: int = 0, rmw_delta: int = 1,
+           seed: int = 1, n_threads: int = 0) -> Batch:
+    lo, hi = (ops, ops) if isinstance(ops, int) else ops
+    spec = _KvSpec(n_txns, lo, hi, n_rows, zipf_theta, n_cols, read_pct, write_kind,
+                   int(read_all_cols), table_id, rmw_delta, seed & 0xFFFFFFFFFFFFFFFF)
+    lib = host_lib()
+    cap = int(lib.ltpmg_gen_kv_max_ops(C.byref(spec)))
+    off = np.zeros(n_txns + 1, np.uint32)
+    rows = np.empty(cap, np.uint32)
+    metas = np.empty(cap, np.uint16)
+    args = np.empty(cap, np.uint64)
+    n = C.c_uint32(0)
+    rc = lib.ltpmg_gen_kv(C.byref(spec), _ptr(off), _ptr(rows), _ptr(metas), _ptr(args), cap,
+                          C.byref(n), n_threads)
+    if rc != 0:
+        raise LtpmgError("ltpmg_gen_kv", rc, "invalid workload spec")
+    k = n.value
+    return Batch(off, rows[:k].copy(), metas[:k].copy(), args[:k].copy())
+
+
+def zeta(n: int, theta: float) -> float:
+    return float(host_lib().ltpmg_zeta(n, theta))
+
+
+class Engine:
+    """One MI355X engine (one GPU).  Mirrors the C ABI one call per method."""
+
+    def __init__(self, device: int = 0, n_slots: int = 1, max_txns: int = 0, max_ops: int = 0,
+                 hbm_budget: int = 0, profile: bool = False):
+        self._lib = hip_lib()
+        cfg = _Cfg(device, n_slots, max_txns, max_ops, hbm_budget, CFG_PROFILE if profile else 0)
+        h = C.c_void_p()
+        rc = self._lib.ltpmg_open(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise LtpmgError("ltpmg_open", rc, "cannot open the GPU engine")
+        self._h = h
+        self._batches: dict[int, Batch] = {}
+        self.tables: dict[int, tuple[int, list[int]]] = {}
+
+    def _check(self, fn: str, rc: int) -> None:
+        if rc != 0:
+            msg = self._lib.ltpmg_last_error(self._h)
+            raise LtpmgError(fn, rc, msg.decode() if msg else "")
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.ltpmg_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def create_table(self, table_id: int, n_rows: int, col_bytes: list[int]) -> None:
+        d = _TableDesc(n_rows, len(col_bytes))
+        for i, w in enumerate(col_bytes):
+            d.col_bytes[i] = w
+        self._check("ltpmg_create_table", self._lib.ltpmg_create_table(self._h, table_id, C.byref(d)))
+        self.tables[table_id] = (n_rows, list(col_bytes))
+
+    def fill_table(self, table_id: int, seed: int) -> None:
+        self._check("ltpmg_fill_table", self._lib.ltpmg_fill_table(self._h, table_id, seed))
+
+    def load_column(self, table_id: int, col: int, data: np.ndarray, row0: int = 0) -> None:
+        w = self.tables[table_id][1][col]
+        a = np.ascontiguousarray(data, np.uint8).reshape(-1, w)
+        self._check("ltpmg_load_column",
+                    self._lib.ltpmg_load_column(self._h, table_id, col, row0, a.shape[0], _ptr(a)))
+
+    def read_column(self, table_id: int, col: int, row0: int = 0, n_rows: int | None = None) -> np.ndarray:
+        rows, widths = self.tables[table_id]
+        n = rows - row0 if n_rows is None else n_rows
+        out = np.empty((n, widths[col]), np.uint8)
+        self._check("ltpmg_read_column",
+                    self._lib.ltpmg_read_column(self._h, table_id, col, row0, n, _ptr(out)))
+        return out
+
+    def digest(self, table_id: int) -> int:
+        v = C.c_uint64(0)
+        self._check("ltpmg_table_digest", self._lib.ltpmg_table_digest(self._h, table_id, C.byref(v)))
+        return v.value
+
+    @property
+    def result_stride(self) -> int:
+        return int(self._lib.ltpmg_result_stride(self._h))
+
+    def submit(self, batch: Batch, slot: int = 0) -> None:
+        b = batch.normalized()
+        self._check("ltpmg_submit_batch", self._lib.ltpmg_submit_batch(self._h, slot, C.byref(b.c_struct())))
+        self._batches[slot] = b
+
+    def execute(self, mode: int, slot: int = 0, profile: bool = False) -> None:
+        m = mode | (EXEC_PROFILE if profile else 0)
+        self._check("ltpmg_execute", self._lib.ltpmg_execute(self._h, slot, m))
+
+    def wait(self, slot: int = 0) -> None:
+        self._check("ltpmg_wait", self._lib.ltpmg_wait(self._h, slot))
+
+    def fetch(self, slot: int = 0, results: bool = True) -> tuple[np.ndarray, np.ndarray | None]:
+        b = self._batches[slot]
+        commit = np.empty(b.n_txns, np.uint8)
+        res = np.empty((b.n_ops, self.result_stride), np.uint8) if results else None
+        self._check("ltpmg_fetch_results",
+                    self._lib.ltpmg_fetch_results(self._h, slot, _ptr(commit),
+                                                  _ptr(res) if res is not None else None,
+                                                  res.nbytes if res is not None else 0))
+        return commit, res
+
+    def stats(self, slot: int = 0) -> dict:
+        s = _Stats()
+        self._check("ltpmg_stats", self._lib.ltpmg_stats(self._h, slot, C.byref(s)))
+        kernels = {}
+        for i in range(s.n_kernels):
+            name = bytes(s.kernel_name[i]).split(b"\0", 1)[0].decode()
+            kernels[name] = kernels.get(name, 0.0) + float(s.kernel_ms[i])
+        return {"n_txns": s.n_txns, "n_ops": s.n_ops, "committed": s.committed,
+                "aborted": s.aborted, "mode": s.mode, "exec_ms": float(s.exec_ms),
+                "upload_ms": float(s.upload_ms), "kernels": kernels,
+                "hbm_allocated": int(s.hbm_allocated)}
+
+    def run(self, batch: Batch, mode: int, slot: int = 0, results: bool = True):
+        self.submit(batch, slot)
+        self.execute(mode, slot)
+        return self.fetch(slot, results)
+
+    def selftest_sort(self, keys: np.ndarray, vals: np.ndarray, key_bits: int):
+        k = np.ascontiguousarray(keys, np.uint32).copy()
+        v = np.ascontiguousarray(vals, np.uint32).copy()
+        self._check("ltpmg_selftest_sort",
+                    self._lib.ltpmg_selftest_sort(self._h, _ptr(k), _ptr(v), len(k), key_bits))
+        return k, v
+
+    def selftest_segscan(self, vals: np.ndarray, heads: np.ndarray) -> np.ndarray:
+        v = np.ascontiguousarray(vals, np.uint32)
+        h = np.ascontiguousarray(heads, np.uint8)
+        out = np.empty_like(v)
+        self._check("ltpmg_selftest_segscan",
+                    self._lib.ltpmg_selftest_segscan(self._h, _ptr(v), _ptr(h), _ptr(out), len(v)))
+        return out

@@ -1,0 +1,289 @@
+"""GPU parity: the HIP engine against the oracle, bit-exact.
+
+Every test opens a fresh engine on cuda:0, fills tables from a seed on the
+device, runs a batch through the C ABI and compares commit flags, every
+defined read result and the whole final table image with the oracle's serial
+replay of the same batch.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_ffi as orc
+from batches import OP_READ, OP_RMW_ADD, OP_UPDATE, concat, random_batch
+from ltpmg_amd import MODE_A, MODE_O, Batch, Engine, LtpmgError, gen_kv
+
+pytestmark = pytest.mark.gpu
+
+BOTH = [MODE_A, MODE_O]
+
+
+def _setup(e: Engine, schema: dict, seed: int) -> orc.Image:
+    for t, (n, widths) in sorted(schema.items()):
+        e.create_table(t, n, widths)
+        e.fill_table(t, seed)
+    return orc.Image(schema).fill(seed)
+
+
+def _compare(e: Engine, im: orc.Image, batch: Batch, mode: int, check_image=True):
+    rs = e.result_stride
+    commit, res = e.run(batch, mode)
+    oc, ores = orc.replay(im, batch, mode, rs)
+    assert commit.shape == oc.shape
+    bad = np.nonzero(commit != oc)[0]
+    assert bad.size == 0, f"commit flags differ at txns {bad[:10]}"
+    m = orc.read_mask(batch, oc)
+    if not (res[m] == ores[m]).all():
+        idx = np.nonzero(m)[0]
+        wrong = idx[np.any(res[m] != ores[m], axis=1)]
+        pytest.fail(f"{wrong.size} read results differ, first ops {wrong[:8]}")
+    if check_image:
+        for t, (n, widths) in im.schema.items():
+            for c in range(len(widths)):
+                got = e.read_column(t, c)
+                if not (got == im.cols[t][c]).all():
+                    rows = np.nonzero(np.any(got != im.cols[t][c], axis=1))[0]
+                    pytest.fail(f"table {t} col {c}: {rows.size} rows differ, first {rows[:8]}")
+            assert e.digest(t) == orc.digest_columns(t, im.cols[t])
+    st = e.stats()
+    assert st["committed"] == int(oc.sum()) and st["aborted"] == batch.n_txns - int(oc.sum())
+    return commit, res, st
+
+
+# ---------------- tables ----------------
+def test_fill_and_digest_match_oracle():
+    schema = {0: (1000, [4, 8, 12, 16, 32, 100, 256]), 3: (77, [100])}
+    with Engine() as e:
+        im = _setup(e, schema, 0x4C54504D47000001)
+        for t, (n, widths) in schema.items():
+            for c in range(len(widths)):
+                assert (e.read_column(t, c) == im.cols[t][c]).all(), (t, c)
+            assert e.digest(t) == orc.digest_columns(t, im.cols[t])
+        assert e.result_stride == 256
+
+
+def test_load_and_read_column_roundtrip():
+    with Engine() as e:
+        e.create_table(0, 500, [4, 100])
+        data = np.random.default_rng(1).integers(0, 256, (500, 100), dtype=np.uint8)
+        e.load_column(0, 1, data)
+        assert (e.read_column(0, 1) == data).all()
+        part = e.read_column(0, 1, row0=123, n_rows=10)
+        assert (part == data[123:133]).all()
+        with pytest.raises(LtpmgError):
+            e.read_column(0, 1, row0=495, n_rows=10)
+
+
+# ---------------- building blocks (reference self-test KATs) ----------------
+def test_sort_reference_kat_descending_million():
+    # gpulauncher.cu:1237-1264: 1,000,000 values in descending order sort to i+1
+    n = 1_000_000
+    keys = np.arange(n, 0, -1, dtype=np.uint32)
+    with Engine() as e:
+        k, v = e.selftest_sort(keys, np.arange(n, dtype=np.uint32), 32)
+    assert (k == np.arange(1, n + 1, dtype=np.uint32)).all()
+    assert (v == np.arange(n - 1, -1, -1, dtype=np.uint32)).all()
+
+
+@pytest.mark.parametrize("n,bits", [(1, 8), (100, 7), (4097, 8), (70000, 13), (300001, 24),
+                                    (1 << 20, 28), (123457, 32)])
+def test_sort_is_stable_and_matches_numpy(n, bits):
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    if bits < 32:
+        keys &= np.uint32((1 << bits) - 1)
+    if n > 1000:  # heavy duplicates, as zipf keys have
+        keys[: n // 3] = keys[0]
+    vals = np.arange(n, dtype=np.uint32)
+    with Engine() as e:
+        k, v = e.selftest_sort(keys, vals, bits)
+    order = np.argsort(keys, kind="stable")
+    assert (k == keys[order]).all()
+    assert (v == vals[order]).all()
+
+
+def test_segscan_reference_kat():
+    # gpulauncher.cu:1267-1321: 128 values i % 64 in two segments of 64
+    vals = (np.arange(128) % 64).astype(np.uint32)
+    heads = np.zeros(128, np.uint8)
+    heads[64] = 1
+    with Engine() as e:
+        out = e.selftest_segscan(vals, heads)
+    seg = np.concatenate([np.cumsum(vals[:64]) - vals[:64], np.cumsum(vals[64:]) - vals[64:]])
+    assert (out == seg.astype(np.uint32)).all()
+
+
+@pytest.mark.parametrize("n,p", [(5000, 0.01), (200003, 0.0001), (100000, 0.5), (9000, 0.0)])
+def test_segscan_random(n, p):
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, 1000, n).astype(np.uint32)
+    heads = (rng.random(n) < p).astype(np.uint8)
+    with Engine() as e:
+        out = e.selftest_segscan(vals, heads)
+    expect = np.empty(n, np.uint64)
+    run = 0
+    for i in range(n):
+        if i == 0 or heads[i]:
+            run = 0
+        expect[i] = run
+        run += int(vals[i])
+    assert (out == (expect & 0xFFFFFFFF).astype(np.uint32)).all()
+
+
+# ---------------- closed forms on the GPU ----------------
+def test_gpu_mode_a_closed_forms():
+    schema = {0: (64, [4, 4])}
+    with Engine() as e:
+        im = _setup(e, schema, 21)
+        distinct = Batch.from_txns([[(OP_UPDATE, 0, 0, 2 * t, t + 1), (OP_READ, 0, 1, 2 * t + 1, 0)]
+                                    for t in range(20)])
+        c, _, _ = _compare(e, im, distinct, MODE_A)
+        assert c.all()
+        one_key = Batch.from_txns([[(OP_READ, 0, 0, 40 + t % 20, 0), (OP_RMW_ADD, 0, 1, 5, 1)]
+                                   for t in range(16)])
+        c, _, _ = _compare(e, im, one_key, MODE_A)
+        assert c.tolist() == [1] + [0] * 15
+        readers = [[(OP_READ, 0, 0, 7, 0)] for _ in range(12)]
+        readers[4] = [(OP_UPDATE, 0, 0, 7, 99)]
+        c, _, _ = _compare(e, im, Batch.from_txns(readers), MODE_A)
+        assert c.tolist() == [1] * 5 + [0] * 7
+
+
+# ---------------- random batches vs the oracle ----------------
+@pytest.mark.parametrize("mode", BOTH)
+@pytest.mark.parametrize("seed", range(4))
+def test_random_batches(mode, seed):
+    rng = np.random.default_rng(1000 + seed)
+    schema = {0: (3000, [4, 8, 100, 32])}
+    b = random_batch(rng, n_txns=5000, n_rows=3000, n_cols=4, max_len=12,
+                     zipf=0.9 if seed % 2 else None)
+    with Engine() as e:
+        im = _setup(e, schema, seed)
+        _compare(e, im, b, mode)
+
+
+@pytest.mark.parametrize("mode", BOTH)
+def test_multi_table(mode):
+    rng = np.random.default_rng(5)
+    schema = {0: (2000, [4, 4, 4]), 2: (1, [8]), 5: (700, [256, 16, 100])}
+    b = concat(random_batch(rng, 3000, 2000, 3, max_len=6, table=0),
+               random_batch(rng, 200, 1, 1, max_len=1, table=2),
+               random_batch(rng, 3000, 700, 3, max_len=6, table=5))
+    perm = rng.permutation(b.n_txns)  # interleave the tables' txns
+    txns = [[(int(b.meta[i] & 3), int(b.meta[i] >> 8), int((b.meta[i] >> 2) & 63), int(b.rows[i]),
+              int(b.args[i])) for i in range(b.txn_off[t], b.txn_off[t + 1])] for t in perm]
+    with Engine() as e:
+        im = _setup(e, schema, 77)
+        _compare(e, im, Batch.from_txns(txns), mode)
+
+
+@pytest.mark.parametrize("n_ops", [1, 2047, 2048, 2049, 4096 + 255])
+@pytest.mark.parametrize("mode", BOTH)
+def test_tile_boundaries(n_ops, mode):
+    # one-op txns; when there is room, a 255-op txn starts at op 1900 and
+    # straddles the first 2048-op tile boundary
+    rng = np.random.default_rng(n_ops)
+
+    def single():
+        return [(int(rng.integers(0, 3)), 0, 0, int(rng.integers(0, 5000)), 3)]
+
+    if n_ops > 1900 + 255:
+        rows = rng.choice(5000, 255, replace=False)
+        big = [(int(rng.integers(0, 3)), 0, 0, int(r), int(rng.integers(0, 2**40))) for r in rows]
+        txns = [single() for _ in range(1900)] + [big] + [single() for _ in range(n_ops - 2155)]
+    else:
+        txns = [single() for _ in range(n_ops)]
+    with Engine() as e:
+        im = _setup(e, {0: (5000, [16])}, 3)
+        _compare(e, im, Batch.from_txns(txns), mode)
+
+
+def test_max_length_txns():
+    rng = np.random.default_rng(9)
+    txns = []
+    for _ in range(40):
+        rows = rng.choice(20000, 256, replace=False)
+        txns.append([(int(rng.integers(0, 3)), 0, 1, int(r), int(rng.integers(0, 2**60))) for r in rows])
+    b = Batch.from_txns(txns)
+    for mode in BOTH:
+        with Engine() as e:
+            im = _setup(e, {0: (20000, [4, 100])}, 4)
+            _compare(e, im, b, mode)
+
+
+# ---------------- BASELINE workloads ----------------
+@pytest.mark.parametrize("mode", BOTH)
+def test_c1_kv_int32_rmw(mode):
+    # 100k rows x 10 int32; 8192 txns x 10 ops on distinct uniform keys,
+    # 50% READ (whole row) / 50% RMW(+1) on one column
+    b = gen_kv(8192, 100_000, 10, ops=10, read_pct=50, write_kind=OP_RMW_ADD, read_all_cols=True,
+               rmw_delta=1, seed=0x4C54504D47000002)
+    with Engine() as e:
+        im = _setup(e, {0: (100_000, [4] * 10)}, 0x4C54504D47000001)
+        c, _, st = _compare(e, im, b, mode)
+    if mode == MODE_A:
+        assert 0 < c.sum() < len(c)
+
+
+@pytest.mark.parametrize("mode", BOTH)
+def test_c2_shape_reduced(mode):
+    # C2's shape at 1/10 of the rows and 1/16 of the txns: 100-byte fields,
+    # 16 ops per txn, zipf 0.99, 50/50 READ/UPDATE
+    b = gen_kv(1 << 16, 1_000_000, 10, ops=16, zipf_theta=0.99, read_pct=50, write_kind=OP_UPDATE,
+               seed=0x4C54504D47000003)
+    with Engine() as e:
+        im = _setup(e, {0: (1_000_000, [100] * 10)}, 0x4C54504D47000004)
+        _compare(e, im, b, mode)
+
+
+def test_batch_sequence_and_determinism():
+    schema = {0: (4000, [4, 100])}
+    rng = np.random.default_rng(33)
+    batches = [random_batch(rng, 3000, 4000, 2, max_len=10, zipf=0.8) for _ in range(5)]
+    modes = [MODE_A, MODE_O, MODE_A, MODE_A, MODE_O]
+    outs = []
+    for _ in range(2):
+        with Engine() as e:
+            im = _setup(e, schema, 8)
+            run = []
+            for b, m in zip(batches, modes):
+                c, r, _ = _compare(e, im, b, m, check_image=False)
+                run.append((c, r[orc.read_mask(b, c)]))
+            for c in range(2):
+                assert (e.read_column(0, c) == im.cols[0][c]).all()
+            outs.append((run, e.digest(0)))
+    for (c1, r1), (c2, r2) in zip(outs[0][0], outs[1][0]):
+        assert (c1 == c2).all() and (r1 == r2).all()
+    assert outs[0][1] == outs[1][1]
+
+
+# ---------------- error paths ----------------
+def test_invalid_batches_are_rejected():
+    with Engine() as e:
+        im = _setup(e, {0: (100, [4])}, 1)
+        bad = [
+            Batch.from_txns([[(OP_READ, 0, 0, 100, 0)]]),        # row out of range
+            Batch.from_txns([[(OP_READ, 0, 1, 5, 0)]]),          # column out of range
+            Batch.from_txns([[(OP_READ, 4, 0, 5, 0)]]),          # no such table
+            Batch(np.array([0, 1, 1], np.uint32), np.array([1], np.uint32),
+                  np.array([0], np.uint16), np.array([0], np.uint64)),  # empty txn
+            Batch(np.array([0, 1], np.uint32), np.array([1], np.uint32),
+                  np.array([3], np.uint16), np.array([0], np.uint64)),  # kind 3
+        ]
+        for b in bad:
+            with pytest.raises(LtpmgError) as ei:
+                e.submit(b)
+            assert ei.value.code == -1
+        with pytest.raises(LtpmgError):
+            e.execute(5)
+        # the engine is still usable
+        _compare(e, im, Batch.from_txns([[(OP_UPDATE, 0, 0, 3, 5)], [(OP_READ, 0, 0, 3, 0)]]), MODE_O)
+
+
+def test_hbm_budget_is_enforced():
+    with Engine(hbm_budget=64 << 20) as e:
+        with pytest.raises(LtpmgError) as ei:
+            e.create_table(0, 1_000_000, [100])
+        assert ei.value.code == -3
+        e.create_table(1, 1000, [4])
