@@ -1,0 +1,141 @@
+// param.h -- run parameters of the host API (the reference's include/param.h:
+// same class, fields and methods, built from the flags in common.h).
+//
+// Benchmarks: YCSB_A..E (the reference's key-value mixes over its 10-column
+// YCSB table), plus this build's KV_C1 (int32 columns, READ whole row /
+// RMW +1) and KV (fields of kv_field_bytes, READ / UPDATE one field).
+// TEST and TPCC_* sizes are kept so existing callers compile; the
+// order-entry adapter that would execute TPCC_* is not built.
+#pragma once
+#include "common.h"
+#include "db_structure.h"
+#include "define.h"
+#include "txn_structure.h"
+
+struct Result {
+    double cost;          // device seconds of the last batch's CC + execute
+    uint32_t epoch;
+    uint32_t batch_size;
+    uint32_t committed = 0;  // added: txns committed by the last batch
+    uint32_t aborted = 0;
+};
+
+class Param {
+public:
+    std::string benchmark;
+    uint32_t table_size;
+    uint32_t batch_size;
+    uint32_t epoch_tp;
+    uint32_t epoch_sync;
+    std::string deviceIDs;
+    uint32_t neworder_percent;
+    std::string filepath;
+    uint32_t table_cnt;
+    uint32_t device_cnt = 0;
+    uint32_t *device_IDs = nullptr;
+    std::string data_distribution;
+    double zipf_config;
+
+    uint32_t test_1_size = 0;
+    uint32_t test_2_size = 0;
+
+    uint32_t warehouse_size = 0;
+    uint32_t district_size = 0;
+    uint32_t customer_size = 0;
+    uint32_t neworder_size = 0;
+    uint32_t history_size = 0;
+    uint32_t order_size = 0;
+    uint32_t orderline_size = 0;
+    uint32_t stock_size = 0;
+    uint32_t item_size = 100000;
+
+    uint32_t ycsb_size = 0;
+    uint32_t bitmap_row_cnt = 0;
+    uint32_t bitmap_table_cnt = 0;
+
+    uint32_t test_query_batch_size = 0;
+    uint32_t test_query_2_batch_size = 0;
+    uint32_t neworder_query_batch_size = 0;
+    uint32_t payment_query_batch_size = 0;
+    uint32_t orderstatus_query_batch_size = 0;
+    uint32_t delivery_query_batch_size = 0;
+    uint32_t stocklevel_query_batch_size = 0;
+    uint32_t ycsb_a_query_batch_size = 0;
+    uint32_t ycsb_b_query_batch_size = 0;
+    uint32_t ycsb_c_query_batch_size = 0;
+    uint32_t ycsb_d_query_batch_size = 0;
+    uint32_t ycsb_e_query_batch_size = 0;
+
+    uint32_t test_query_subtxn_cnt = 0;
+    uint32_t test_query_2_subtxn_cnt = 0;
+    uint32_t neworder_query_subtxn_cnt = 0;
+    uint32_t payment_query_subtxn_cnt = 0;
+    uint32_t orderstatus_query_subtxn_cnt = 0;
+    uint32_t delivery_query_subtxn_cnt = 0;
+    uint32_t stocklevel_query_subtxn_cnt = 0;
+    uint32_t ycsb_a_query_subtxn_cnt = 0;
+    uint32_t ycsb_b_query_subtxn_cnt = 0;
+    uint32_t ycsb_c_query_subtxn_cnt = 0;
+    uint32_t ycsb_d_query_subtxn_cnt = 0;
+    uint32_t ycsb_e_query_subtxn_cnt = 0;
+
+    uint32_t test_sub_txn_size = 0;
+    uint32_t tpcc_all_sub_txn_size = 0;
+    uint32_t tpcc_part_sub_txn_size = 0;
+    uint32_t ycsb_a_sub_txn_size = 0;
+    uint32_t ycsb_b_sub_txn_size = 0;
+    uint32_t ycsb_c_sub_txn_size = 0;
+    uint32_t ycsb_d_sub_txn_size = 0;
+    uint32_t ycsb_e_sub_txn_size = 0;
+
+    uint32_t test_query_subtxn_kinds = 0;
+    uint32_t tpcc_all_query_subtxn_kinds = 0;
+    uint32_t tpcc_part_query_subtxn_kinds = 0;
+    uint32_t ycsb_a_query_subtxn_kinds = 0;
+    uint32_t ycsb_b_query_subtxn_kinds = 0;
+    uint32_t ycsb_c_query_subtxn_kinds = 0;
+    uint32_t ycsb_d_query_subtxn_kinds = 0;
+    uint32_t ycsb_e_query_subtxn_kinds = 0;
+
+    uint32_t select_batch_size = 0;
+    uint32_t insert_batch_size = 0;
+    uint32_t update_batch_size = 0;
+    uint32_t scan_batch_size = 0;
+    uint32_t delete_batch_size = 0;
+
+    // added by this build
+    uint32_t cc_mode = 1;      // LTPMG_MODE_O (1) or LTPMG_MODE_A (0)
+    uint64_t seed = 0;
+    uint32_t kv_size = 0;      // rows of the KV / KV_C1 table
+    uint32_t kv_fields = 10;
+    uint32_t kv_field_bytes = 100;
+    uint32_t kv_ops = 16;
+    uint32_t read_percent = 50;
+    uint32_t kv_sub_txn_size = 0;  // upper bound on device ops per batch
+
+    Result result;
+
+    Param();
+
+    ~Param();
+
+    void print();
+
+    void analyse_deviceIDs();
+
+    void free_deviceIDs();
+
+    const std::string &get_filepath() { return filepath; }
+    void set_table_cnt(uint32_t table_cnt) { this->table_cnt = table_cnt; }
+
+    uint32_t get_subtxn_kinds();
+
+    uint32_t get_sub_txn_size();
+
+    uint32_t get_subtxn_cnt(const std::type_info &query_kind);
+
+    uint32_t get_txn_batch_size(const std::type_info &query_kind);
+
+    bool is_ycsb() const { return benchmark.rfind("YCSB_", 0) == 0; }
+    bool is_kv() const { return benchmark == "KV" || benchmark == "KV_C1"; }
+};

@@ -1,0 +1,62 @@
+// query.cpp -- the batch of the host API (the reference's src/query.cpp).
+// generate_txn runs once; TIDs are positions in the batch (query.cpp:284-292
+// of the reference, which shuffles with a time-seeded engine first; the
+// queries here are already i.i.d. draws, so no shuffle is needed).
+#include "query.h"
+
+namespace {
+// The batch is handed to the extern functions without a copy; the reference
+// copies the whole vector on every call (src/query.cpp:503-519).
+std::shared_ptr<std::vector<std::any>> alias(std::vector<std::any> &v) {
+    return std::shared_ptr<std::vector<std::any>>(&v, [](std::vector<std::any> *) {});
+}
+
+template <typename Q>
+void append(std::shared_ptr<Txn_Generator<Q>> &gen, std::shared_ptr<Param> param, size_t n,
+            std::vector<std::any> &batch) {
+    gen = std::make_shared<Txn_Generator<Q>>();
+    gen->initialize(param);
+    gen->generate_txn(param, n);
+    batch.reserve(batch.size() + gen->get_txn().size());
+    for (auto &q : gen->get_txn()) batch.emplace_back(q);
+    gen.reset();
+}
+}  // namespace
+
+Query::Query() {}
+
+Query::~Query() {}
+
+void Query::generate_txn(std::shared_ptr<Param> param) {
+    transactions_batch.clear();
+    const std::string &b = param->benchmark;
+    if (b == "YCSB_A") append(txn_generator_ycsb_a_query, param, param->ycsb_a_query_batch_size, transactions_batch);
+    else if (b == "YCSB_B") append(txn_generator_ycsb_b_query, param, param->ycsb_b_query_batch_size, transactions_batch);
+    else if (b == "YCSB_C") append(txn_generator_ycsb_c_query, param, param->ycsb_c_query_batch_size, transactions_batch);
+    else if (b == "YCSB_D") append(txn_generator_ycsb_d_query, param, param->ycsb_d_query_batch_size, transactions_batch);
+    else if (b == "YCSB_E") append(txn_generator_ycsb_e_query, param, param->ycsb_e_query_batch_size, transactions_batch);
+    else if (param->is_kv()) {
+        Txn_Generator<KV_Batch> gen;
+        gen.initialize(param);
+        gen.generate_txn(param, param->batch_size);
+        transactions_batch.emplace_back(std::move(gen.get_txn().front()));
+    } else {
+        LOG(FATAL) << "benchmark " << b << ": the order-entry and TEST adapters are not built in this engine";
+    }
+}
+
+void Query::malloc_global_txn(std::shared_ptr<Param> param) { global_txn_info_size = param->get_subtxn_kinds(); }
+
+void Query::free_global_txn() {}
+
+void Query::initial_on_GPU(std::shared_ptr<Param> param) {
+    initial_query_on_GPU(param, alias(transactions_batch), global_txn_info);
+}
+
+void Query::transfer_to_GPU(std::shared_ptr<Param> param) {
+    transfer_query_to_GPU(param, alias(transactions_batch), global_txn_info);
+}
+
+void Query::transfer_to_CPU(std::shared_ptr<Param> param) {
+    transfer_query_to_CPU(param, alias(transactions_batch), global_txn_info);
+}
