@@ -121,6 +121,36 @@ def profile_mode(e, mode: int) -> dict:
     return e.stats()["kernels"]
 
 
+def serve_compare(e, pool, args) -> dict:
+    """Open-loop Poisson arrivals of single transactions drawn from the batch,
+    served two ways on the same device-resident table, ordered mode:
+      plain:  batches of exactly --plain-batch txns, one at a time;
+      served: continuous batching, cut at --serve-max-batch txns or when the
+              oldest waited --serve-wait-us, 2 batches in flight."""
+    from ltpmg_amd import MODE_O
+    out = {"arrivals": args.serve_arrivals, "plain_batch": args.plain_batch,
+           "served_max_batch": args.serve_max_batch, "served_max_wait_us": args.serve_wait_us,
+           "rates": []}
+    keys = ("committed_per_s", "p50_ms", "p99_ms", "p999_ms", "mean_batch_txns", "batches",
+            "host_pack_s", "device_ms", "wall_s")
+    for rate in args.serve_rates:
+        row = {"arrival_rate": rate}
+        for form in ("plain", "served"):
+            if form == "plain":
+                r = e.serve(pool, arrival_rate=rate, n_arrivals=args.serve_arrivals,
+                            max_batch=args.plain_batch, min_batch=args.plain_batch, n_inflight=1,
+                            mode=MODE_O, seed=7)
+            else:
+                r = e.serve(pool, arrival_rate=rate, n_arrivals=args.serve_arrivals,
+                            max_batch=args.serve_max_batch, max_wait_us=args.serve_wait_us,
+                            n_inflight=2, mode=MODE_O, seed=7)
+            row[form] = {k: r[k] for k in keys}
+            log(f"serve {form} @ {rate:.3g}/s: {r['committed_per_s']:.3g} txn/s, "
+                f"p50 {r['p50_ms']:.3f} ms, p99 {r['p99_ms']:.3f} ms, {r['batches']} batches")
+        out["rates"].append(row)
+    return out
+
+
 def pct(xs, q):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
@@ -162,6 +192,12 @@ def main() -> int:
     ap.add_argument("--theta", type=float, default=0.99)
     ap.add_argument("--cpu-sample-txns", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8],
+                    help="arrival rates (txn/s) for the served-vs-plain comparison; none to skip")
+    ap.add_argument("--serve-arrivals", type=int, default=1 << 20)
+    ap.add_argument("--plain-batch", type=int, default=1 << 18)
+    ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
+    ap.add_argument("--serve-wait-us", type=float, default=2000.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the batched transaction path on a "
@@ -176,7 +212,7 @@ def main() -> int:
                    write_kind=OP_UPDATE, seed=BATCH_SEED)
     counts = batch_counts(batch, width)
     log(f"batch: {batch.n_ops} ops, {counts['written_cells']} written cells")
-    e = Engine()
+    e = Engine(n_slots=2)
     e.create_table(0, args.rows, [width] * n_cols)
     e.fill_table(0, TABLE_SEED)
     log("table filled")
@@ -220,6 +256,10 @@ def main() -> int:
                 "ms": ms, "launches": l,
                 "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None}
 
+    serving = None
+    if args.serve_rates:
+        serving = serve_compare(e, batch, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -251,7 +291,7 @@ def main() -> int:
         "p99_commit_latency_ms": modes["ordered"]["p99_commit_latency_ms"],
         "modes": modes,
         "upload_ms": head["upload_ms"], "upload_wall_s": upload_wall,
-        "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu,
+        "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
         "batch_counts": counts,
     }
     print(json.dumps(out), flush=True)

@@ -146,7 +146,33 @@ uint32_t ltpmg_result_stride(const ltpmg_engine *e);
  * (n_ops * ltpmg_result_stride() bytes, slot i holds op i's READ) back.  The
  * reference has no results path (Global_Txn_Result is never written). */
 int ltpmg_submit_batch(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *b);
+/* The checks submit applies, without submitting (for a pool of requests). */
+int ltpmg_check_batch(ltpmg_engine *e, const ltpmg_batch *b);
+/* Zero-copy submit: stage returns pointers into the slot's pinned buffer
+ * (waiting for the slot to be idle), the caller writes the batch there, and
+ * submit_staged uploads it.  With LTPMG_SUBMIT_TRUSTED the host-side checks
+ * are skipped (for batches cut from an already checked pool); the kernels
+ * still never touch memory outside the tables, but a malformed op's result
+ * is then undefined. */
+typedef struct {
+    uint32_t *txn_off; /* n_txns + 1 */
+    uint32_t *rows;
+    uint16_t *meta;
+    uint64_t *args;
+} ltpmg_staging;
+#define LTPMG_SUBMIT_TRUSTED 1u
+int ltpmg_stage_batch(ltpmg_engine *e, uint32_t slot, uint32_t n_txns, uint32_t n_ops,
+                      ltpmg_staging *out);
+int ltpmg_submit_staged(ltpmg_engine *e, uint32_t slot, uint32_t flags);
+/* Batches take effect on the tables in the order of ltpmg_execute calls,
+ * across slots: a batch's reads and writes of cells and lock words wait for
+ * the previously executed batch.  In the ordered mode the key build, sort
+ * and scan of a batch only touch its slot and overlap the previous batch.
+ * Replaces the reference's one-batch-at-a-time epoch loop
+ * (src/scheduler.cpp:44-59) for the serving form. */
 int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode);
+/* Non-blocking: LTPMG_OK when the slot is idle, 1 while work is queued. */
+int ltpmg_query(ltpmg_engine *e, uint32_t slot);
 int ltpmg_wait(ltpmg_engine *e, uint32_t slot);
 int ltpmg_fetch_results(ltpmg_engine *e, uint32_t slot, uint8_t *commit, void *results,
                         uint64_t results_bytes);

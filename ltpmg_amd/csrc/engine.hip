@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -54,8 +55,10 @@ struct Slot {
     uint8_t *h_stage = nullptr;
     size_t stage_bytes = 0;
     uint32_t n_txns = 0, n_ops = 0, mode = 0;
+    uint32_t staged_txns = UINT32_MAX, staged_ops = 0;  // set by ltpmg_stage_batch
     bool has_batch = false, executed = false;
     hipEvent_t up0 = nullptr, up1 = nullptr, x0 = nullptr, x1 = nullptr;
+    hipEvent_t done = nullptr;  // recorded after the batch's last table access
     std::vector<Stage> stages;
     uint32_t n_stages = 0;
 };
@@ -76,6 +79,7 @@ struct ltpmg_engine {
     uint64_t budget = 0, allocated = 0;
     uint32_t default_txns = 0, default_ops = 0;
     uint32_t epoch = 0;
+    int tail_slot = -1;  // slot of the most recently executed batch
     Table tables[LTPMG_MAX_TABLES];
     ColDesc *d_cols = nullptr;
     TabDesc *d_tabs = nullptr;
@@ -248,7 +252,7 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
 int validate_batch(ltpmg_engine *e, const ltpmg_batch *b) {
     if (!b || !b->txn_off || (b->n_ops && (!b->rows || !b->meta || !b->args)))
         return fail(e, LTPMG_EINVAL, "null batch array");
-    if (b->n_ops >= (1u << 30)) return fail(e, LTPMG_EINVAL, "n_ops must be < 2^30");
+    if (b->n_ops >= (1u << 29)) return fail(e, LTPMG_EINVAL, "n_ops must be < 2^29");
     if (b->txn_off[0] != 0) return fail(e, LTPMG_EINVAL, "txn_off[0] must be 0");
     for (uint32_t t = 0; t < b->n_txns; ++t) {
         uint32_t len = b->txn_off[t + 1] - b->txn_off[t];
@@ -258,16 +262,41 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b) {
     }
     if (b->txn_off[b->n_txns] != b->n_ops)
         return fail(e, LTPMG_EINVAL, "txn_off[n_txns]=%u != n_ops=%u", b->txn_off[b->n_txns], b->n_ops);
-    for (uint32_t i = 0; i < b->n_ops; ++i) {
+    // per-op checks, split over host threads for large batches; the first
+    // bad op (lowest index) is reported
+    auto bad_op = [&](uint32_t i) -> bool {
+        uint16_t m = b->meta[i];
+        uint32_t kind = m & 3u, col = (m >> 2) & 63u;
+        const Table &t = e->tables[(m >> 8) & 15u];
+        return (m >> 12) || kind > LTPMG_OP_RMW_ADD || !t.live || col >= t.n_cols || b->rows[i] >= t.n_rows;
+    };
+    const uint32_t n = b->n_ops;
+    const uint32_t n_threads = n < (1u << 20) ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<uint32_t> first(n_threads, UINT32_MAX);
+    auto scan = [&](uint32_t w) {
+        uint32_t a = (uint32_t)((uint64_t)n * w / n_threads), z = (uint32_t)((uint64_t)n * (w + 1) / n_threads);
+        for (uint32_t i = a; i < z; ++i)
+            if (bad_op(i)) {
+                first[w] = i;
+                return;
+            }
+    };
+    if (n_threads == 1) {
+        scan(0);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t w = 0; w < n_threads; ++w) th.emplace_back(scan, w);
+        for (auto &t : th) t.join();
+    }
+    uint32_t i = *std::min_element(first.begin(), first.end());
+    if (i != UINT32_MAX) {
         uint16_t m = b->meta[i];
         uint32_t kind = m & 3u, col = (m >> 2) & 63u, tab = (m >> 8) & 15u;
-        if ((m >> 12) || kind > LTPMG_OP_RMW_ADD)
-            return fail(e, LTPMG_EINVAL, "op %u: bad meta 0x%04x", i, m);
         const Table &t = e->tables[tab];
+        if ((m >> 12) || kind > LTPMG_OP_RMW_ADD) return fail(e, LTPMG_EINVAL, "op %u: bad meta 0x%04x", i, m);
         if (!t.live) return fail(e, LTPMG_EINVAL, "op %u: table %u does not exist", i, tab);
         if (col >= t.n_cols) return fail(e, LTPMG_EINVAL, "op %u: column %u >= %u", i, col, t.n_cols);
-        if (b->rows[i] >= t.n_rows)
-            return fail(e, LTPMG_EINVAL, "op %u: row %u >= %u rows of table %u", i, b->rows[i], t.n_rows, tab);
+        return fail(e, LTPMG_EINVAL, "op %u: row %u >= %u rows of table %u", i, b->rows[i], t.n_rows, tab);
     }
     return LTPMG_OK;
 }
@@ -332,7 +361,8 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
         if (rc != LTPMG_OK) break;
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreate(&s.up0) != hipSuccess || hipEventCreate(&s.up1) != hipSuccess ||
-            hipEventCreate(&s.x0) != hipSuccess || hipEventCreate(&s.x1) != hipSuccess)
+            hipEventCreate(&s.x0) != hipSuccess || hipEventCreate(&s.x1) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
             rc = LTPMG_EHIP;
     }
     if (rc != LTPMG_OK) {
@@ -353,7 +383,7 @@ int ltpmg_close(ltpmg_engine *e) {
             (void)hipEventDestroy(st.a);
             (void)hipEventDestroy(st.b);
         }
-        for (hipEvent_t ev : {s.up0, s.up1, s.x0, s.x1})
+        for (hipEvent_t ev : {s.up0, s.up1, s.x0, s.x1, s.done})
             if (ev) (void)hipEventDestroy(ev);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
@@ -479,43 +509,101 @@ int ltpmg_table_digest(ltpmg_engine *e, uint32_t table_id, uint64_t *out) {
     return LTPMG_OK;
 }
 
+// Pinned staging layout of a slot: offsets | pad to 8 | args | rows | meta
+// (keeps args 8-byte aligned).
+static ltpmg_staging stage_ptrs(Slot &s, uint32_t n_txns, uint32_t n_ops) {
+    const size_t off_b = ((size_t)n_txns + 1) * 4;
+    uint8_t *p_args = s.h_stage + ((off_b + 7) & ~(size_t)7);
+    uint8_t *p_rows = p_args + (size_t)n_ops * 8;
+    uint8_t *p_meta = p_rows + (size_t)n_ops * 4;
+    return ltpmg_staging{reinterpret_cast<uint32_t *>(s.h_stage), reinterpret_cast<uint32_t *>(p_rows),
+                         reinterpret_cast<uint16_t *>(p_meta), reinterpret_cast<uint64_t *>(p_args)};
+}
+
+static int stage_locked(ltpmg_engine *e, Slot &s, uint32_t n_txns, uint32_t n_ops) {
+    if (n_ops >= (1u << 29)) return fail(e, LTPMG_EINVAL, "n_ops must be < 2^29");
+    int rc = ensure_slot_capacity(e, s, n_txns, n_ops, max_slot_all(e));
+    if (rc != LTPMG_OK) return rc;
+    // the staging buffer may still feed the slot's previous upload
+    HIP_TRY(e, hipStreamSynchronize(s.stream));
+    s.staged_txns = n_txns;
+    s.staged_ops = n_ops;
+    s.has_batch = s.executed = false;
+    return LTPMG_OK;
+}
+
+static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
+    const uint32_t n_txns = s.staged_txns, n_ops = s.staged_ops;
+    ltpmg_staging st = stage_ptrs(s, n_txns, n_ops);
+    if (!trusted) {
+        ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};
+        int rc = validate_batch(e, &b);
+        if (rc != LTPMG_OK) return rc;
+    }
+    HIP_TRY(e, hipEventRecord(s.up0, s.stream));
+    HIP_TRY(e, hipMemcpyAsync(s.d_txn_off, st.txn_off, ((size_t)n_txns + 1) * 4, hipMemcpyHostToDevice, s.stream));
+    if (n_ops) {
+        HIP_TRY(e, hipMemcpyAsync(s.d_args, st.args, (size_t)n_ops * 8, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(e, hipMemcpyAsync(s.d_rows, st.rows, (size_t)n_ops * 4, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(e, hipMemcpyAsync(s.d_meta, st.meta, (size_t)n_ops * 2, hipMemcpyHostToDevice, s.stream));
+    }
+    HIP_TRY(e, hipEventRecord(s.up1, s.stream));
+    s.n_txns = n_txns;
+    s.n_ops = n_ops;
+    s.has_batch = true;
+    s.executed = false;
+    return LTPMG_OK;
+}
+
 int ltpmg_submit_batch(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *b) {
-    if (!e) return LTPMG_EINVAL;
+    if (!e || !b) return LTPMG_EINVAL;
     std::lock_guard<std::mutex> g(e->mu);
     HIP_TRY(e, hipSetDevice(e->device));
     Slot *s = get_slot(e, slot);
     if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
     int rc = validate_batch(e, b);
     if (rc != LTPMG_OK) return rc;
-    rc = ensure_slot_capacity(e, *s, b->n_txns, b->n_ops, max_slot_all(e));
+    rc = stage_locked(e, *s, b->n_txns, b->n_ops);
     if (rc != LTPMG_OK) return rc;
-    // the staging buffer may still feed the previous upload
-    HIP_TRY(e, hipStreamSynchronize(s->stream));
-    uint8_t *p = s->h_stage;
-    const size_t off_b = ((size_t)b->n_txns + 1) * 4, rows_b = (size_t)b->n_ops * 4,
-                 meta_b = (size_t)b->n_ops * 2, args_b = (size_t)b->n_ops * 8;
-    // layout: offsets | pad to 8 | args | rows | meta (keeps args 8-byte aligned)
-    uint8_t *p_off = p, *p_args = p + ((off_b + 7) & ~(size_t)7);
-    uint8_t *p_rows = p_args + args_b, *p_meta = p_rows + rows_b;
-    std::memcpy(p_off, b->txn_off, off_b);
+    ltpmg_staging st = stage_ptrs(*s, b->n_txns, b->n_ops);
+    std::memcpy(st.txn_off, b->txn_off, ((size_t)b->n_txns + 1) * 4);
     if (b->n_ops) {
-        std::memcpy(p_args, b->args, args_b);
-        std::memcpy(p_rows, b->rows, rows_b);
-        std::memcpy(p_meta, b->meta, meta_b);
+        std::memcpy(st.args, b->args, (size_t)b->n_ops * 8);
+        std::memcpy(st.rows, b->rows, (size_t)b->n_ops * 4);
+        std::memcpy(st.meta, b->meta, (size_t)b->n_ops * 2);
     }
-    HIP_TRY(e, hipEventRecord(s->up0, s->stream));
-    HIP_TRY(e, hipMemcpyAsync(s->d_txn_off, p_off, off_b, hipMemcpyHostToDevice, s->stream));
-    if (b->n_ops) {
-        HIP_TRY(e, hipMemcpyAsync(s->d_args, p_args, args_b, hipMemcpyHostToDevice, s->stream));
-        HIP_TRY(e, hipMemcpyAsync(s->d_rows, p_rows, rows_b, hipMemcpyHostToDevice, s->stream));
-        HIP_TRY(e, hipMemcpyAsync(s->d_meta, p_meta, meta_b, hipMemcpyHostToDevice, s->stream));
-    }
-    HIP_TRY(e, hipEventRecord(s->up1, s->stream));
-    s->n_txns = b->n_txns;
-    s->n_ops = b->n_ops;
-    s->has_batch = true;
-    s->executed = false;
+    return upload_locked(e, *s, true);
+}
+
+int ltpmg_check_batch(ltpmg_engine *e, const ltpmg_batch *b) {
+    if (!e || !b) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    return validate_batch(e, b);
+}
+
+int ltpmg_stage_batch(ltpmg_engine *e, uint32_t slot, uint32_t n_txns, uint32_t n_ops,
+                      ltpmg_staging *out) {
+    if (!e || !out) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    Slot *s = get_slot(e, slot);
+    if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
+    int rc = stage_locked(e, *s, n_txns, n_ops);
+    if (rc != LTPMG_OK) return rc;
+    *out = stage_ptrs(*s, n_txns, n_ops);
     return LTPMG_OK;
+}
+
+int ltpmg_submit_staged(ltpmg_engine *e, uint32_t slot, uint32_t flags) {
+    if (!e) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    Slot *s = get_slot(e, slot);
+    if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
+    if (s->staged_txns == UINT32_MAX) return fail(e, LTPMG_ESTATE, "nothing staged in slot %u", slot);
+    int rc = upload_locked(e, *s, (flags & LTPMG_SUBMIT_TRUSTED) != 0);
+    s->staged_txns = UINT32_MAX;
+    return rc;
 }
 
 int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
@@ -541,6 +629,9 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
         const uint32_t epoch = ++e->epoch;
         if (epoch == 0) return fail(e, LTPMG_ESTATE, "epoch counter wrapped");
         const uint32_t n_tiles = std::max(1u, (s.n_ops + kTileOps - 1) / kTileOps);
+        // lock words and table rows: after the previous batch, whatever its slot
+        if (e->tail_slot >= 0 && e->tail_slot != (int)slot)
+            HIP_TRY(e, hipStreamWaitEvent(s.stream, e->slots[e->tail_slot].done, 0));
         rec.begin("tile_bounds");
         launch_zero_u32(s.stream, s.d_counters, 4);
         launch_tile_bounds(s.stream, s.d_txn_off, s.n_txns, n_tiles, s.d_tile_t0);
@@ -588,6 +679,10 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
         rec.begin("mo_seg_scan");
         launch_mo_seg_scan(s.stream, ov, s.ob, sorted);
         rec.end();
+        // keys, sort and scan only touch this slot's buffers, so they overlap
+        // the previous batch; reading and writing cells waits for it
+        if (e->tail_slot >= 0 && e->tail_slot != (int)slot)
+            HIP_TRY(e, hipStreamWaitEvent(s.stream, e->slots[e->tail_slot].done, 0));
         rec.begin("mo_apply_reads");
         launch_mo_apply_reads(s.stream, dt, ov, s.ob, sorted, col_bits, n_tabs, s.d_results, rs);
         rec.end();
@@ -597,9 +692,21 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
         rec.end();
     }
     HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipEventRecord(s.done, s.stream));
+    e->tail_slot = (int)slot;
     HIP_TRY(e, hipEventRecord(s.x1, s.stream));
     s.executed = true;
     return LTPMG_OK;
+}
+
+int ltpmg_query(ltpmg_engine *e, uint32_t slot) {
+    if (!e) return LTPMG_EINVAL;
+    Slot *s = get_slot(e, slot);
+    if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
+    hipError_t st = hipStreamQuery(s->stream);
+    if (st == hipSuccess) return LTPMG_OK;
+    if (st == hipErrorNotReady) return 1;
+    return fail(e, LTPMG_EHIP, "hipStreamQuery failed: %s", hipGetErrorString(st));
 }
 
 int ltpmg_wait(ltpmg_engine *e, uint32_t slot) {

@@ -76,6 +76,15 @@ __device__ inline uint32_t op_kind(uint16_t m) { return m & 3u; }
 __device__ inline uint32_t op_col(uint16_t m) { return (m >> 2) & 63u; }
 __device__ inline uint32_t op_tab(uint16_t m) { return (m >> 8) & 15u; }
 
+// An op the kernels may act on: reserved meta bits clear, known kind, and
+// row and column inside a live table.  The host checks every submitted op;
+// this keeps an unchecked (trusted) submit from touching memory outside the
+// tables.
+__device__ inline bool op_valid(const TabDesc *tabs, uint16_t m, uint32_t row) {
+    const TabDesc &td = tabs[op_tab(m)];
+    return (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows && op_col(m) < td.n_cols;
+}
+
 // popcount of the bits of m below this lane (wave64)
 __device__ inline uint32_t lanemask_lt_popc(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),

@@ -74,8 +74,8 @@ __global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
         uint16_t m = ov.meta[o];
         if (op_kind(m) == LTPMG_OP_READ) continue;
         uint32_t row = ov.rows[o];
+        if (!op_valid(dt.tabs, m, row)) continue;  // never touch memory outside the tables
         const TabDesc &td = dt.tabs[op_tab(m)];
-        if (row >= td.n_rows) continue;  // rejected on the host; never touch memory out of range
         unsigned long long tag = ep | (uint32_t)~(t.t0 + owner[i]);
         unsigned long long *p = td.lock + row;
         unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -104,11 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
         uint32_t o = t.op0 + i;
         uint16_t m = ov.meta[o];
         uint32_t row = ov.rows[o];
-        const TabDesc &td = dt.tabs[op_tab(m)];
-        if (row >= td.n_rows) {
+        if (!op_valid(dt.tabs, m, row)) {  // an invalid op aborts its txn
             ok[owner[i]] = 0;
             continue;
         }
+        const TabDesc &td = dt.tabs[op_tab(m)];
         unsigned long long lk = td.lock[row];
         uint32_t tid = t.t0 + owner[i];
         if ((uint32_t)(lk >> 32) == epoch && (uint32_t)~(uint32_t)lk < tid) ok[owner[i]] = 0;
@@ -172,8 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
         bool take = false;
         if (i < t.nops && ok[owner[i]]) {
             uint32_t o = t.op0 + i;
-            take = op_kind(ov.meta[o]) != LTPMG_OP_READ &&
-                   ov.rows[o] < dt.tabs[op_tab(ov.meta[o])].n_rows;
+            take = op_kind(ov.meta[o]) != LTPMG_OP_READ && op_valid(dt.tabs, ov.meta[o], ov.rows[o]);
         }
         wave_append(take, t.op0 + i, list, &n_list);
     }

@@ -43,8 +43,13 @@ __device__ inline Seg seg_op(Seg a, Seg b) {
     St c = combine({a.set, a.d}, {b.set, b.d});
     return {a.f, c.set, c.d};
 }
+// val = op index | invalid << 29 | kind << 30; an invalid op (only possible
+// with an unchecked submit) is an identity element and emits nothing
+constexpr uint32_t kInvalid = 1u << 29;
+constexpr uint32_t kOpMask = kInvalid - 1u;
 __device__ inline St elem_state(uint32_t val, const uint64_t *args) {
-    uint32_t kind = val >> 30, o = val & 0x3FFFFFFFu;
+    uint32_t kind = val >> 30, o = val & kOpMask;
+    if (val & kInvalid) return {kNoSet, 0u};
     if (kind == LTPMG_OP_UPDATE) return {o, 0u};
     if (kind == LTPMG_OP_RMW_ADD) return {kRmwOnly, (uint32_t)args[o]};
     return {kNoSet, 0u};
@@ -227,12 +232,12 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         if (i < src.n) {
             key = src.keys[i];
             uint32_t val = src.vals[i];
-            op = val & 0x3FFFFFFFu;
+            op = val & kOpMask;
             St s = elem_state(val, src.args);
             bool h = i == 0 || src.keys[i - 1] != key;
             ex = h ? St{kNoSet, 0u} : run;
             run = h ? s : combine(run, s);
-            rd = (val >> 30) == LTPMG_OP_READ;
+            rd = (val >> 30) == LTPMG_OP_READ && !(val & kInvalid);
             bool last = i + 1 == src.n || src.keys[i + 1] != key;
             wr = last && run.set != kNoSet;
         }
@@ -441,8 +446,11 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
          o += gridDim.x * blockDim.x) {
         uint16_t m = ov.meta[o];
         const TabDesc &td = dt.tabs[op_tab(m)];
-        keys[o] = ((td.row_base + ov.rows[o]) << col_bits) | op_col(m);
-        vals[o] = o | (op_kind(m) << 30);
+        uint32_t row = ov.rows[o];
+        bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
+                  op_col(m) < td.n_cols;
+        keys[o] = ok ? ((td.row_base + row) << col_bits) | op_col(m) : 0u;
+        vals[o] = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 }
 

@@ -61,6 +61,11 @@ class _Stats(C.Structure):
                 ("hbm_allocated", C.c_uint64)]
 
 
+class _Staging(C.Structure):
+    _fields_ = [("txn_off", C.POINTER(C.c_uint32)), ("rows", C.POINTER(C.c_uint32)),
+                ("meta", C.POINTER(C.c_uint16)), ("args", C.POINTER(C.c_uint64))]
+
+
 class _KvSpec(C.Structure):
     _fields_ = [("n_txns", C.c_uint32), ("ops_min", C.c_uint32), ("ops_max", C.c_uint32),
                 ("n_rows", C.c_uint32), ("zipf_theta", C.c_double), ("n_cols", C.c_uint32),
@@ -83,7 +88,12 @@ ENGINE_API = {
     "ltpmg_table_digest": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "ltpmg_result_stride": (C.c_uint32, [C.c_void_p]),
     "ltpmg_submit_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Batch)]),
+    "ltpmg_check_batch": (C.c_int, [C.c_void_p, C.POINTER(_Batch)]),
+    "ltpmg_stage_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.POINTER(_Staging)]),
+    "ltpmg_submit_staged": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "ltpmg_execute": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "ltpmg_query": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ltpmg_wait": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ltpmg_fetch_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                       C.c_uint64]),
@@ -94,11 +104,36 @@ ENGINE_API = {
                                          C.c_uint32]),
 }
 
+class _ServeCfg(C.Structure):
+    _fields_ = [("arrival_rate", C.c_double), ("n_arrivals", C.c_uint32), ("max_batch", C.c_uint32),
+                ("min_batch", C.c_uint32), ("max_wait_us", C.c_double), ("n_inflight", C.c_uint32),
+                ("mode", C.c_uint32), ("seed", C.c_uint64), ("pack_threads", C.c_uint32)]
+
+
+class _ServeStats(C.Structure):
+    _fields_ = [("offered", C.c_uint64), ("committed", C.c_uint64), ("aborted_attempts", C.c_uint64),
+                ("batches", C.c_uint64), ("wall_s", C.c_double), ("committed_per_s", C.c_double),
+                ("p50_ms", C.c_double), ("p90_ms", C.c_double), ("p99_ms", C.c_double),
+                ("p999_ms", C.c_double), ("max_ms", C.c_double), ("mean_batch_txns", C.c_double),
+                ("host_pack_s", C.c_double), ("device_ms", C.c_double)]
+
+
+class _ServeRecord(C.Structure):
+    _fields_ = [("batch_off", C.c_void_p), ("batch_txns", C.c_void_p), ("batch_commit", C.c_void_p),
+                ("cap_batches", C.c_uint32), ("cap_entries", C.c_uint32), ("n_batches", C.c_uint32),
+                ("n_entries", C.c_uint32), ("results", C.c_void_p)]
+
+
 WORKLOAD_API = {
     "ltpmg_gen_kv_max_ops": (C.c_uint64, [C.POINTER(_KvSpec)]),
     "ltpmg_gen_kv": (C.c_int, [C.POINTER(_KvSpec), C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32]),
     "ltpmg_zeta": (C.c_double, [C.c_uint64, C.c_double]),
+}
+
+SERVE_API = {
+    "ltpmg_serve_run": (C.c_int, [C.c_void_p, C.POINTER(_ServeCfg), C.POINTER(_Batch),
+                                  C.POINTER(_ServeStats), C.POINTER(_ServeRecord)]),
 }
 
 _libs: dict[str, C.CDLL] = {}
@@ -126,7 +161,7 @@ def hip_lib() -> C.CDLL:
 
 def host_lib() -> C.CDLL:
     hip_lib()  # the host library links against it
-    return _load("libltpmg_host.so", WORKLOAD_API)
+    return _load("libltpmg_host.so", {**WORKLOAD_API, **SERVE_API})
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -278,9 +313,30 @@ class Engine:
         self._check("ltpmg_submit_batch", self._lib.ltpmg_submit_batch(self._h, slot, C.byref(b.c_struct())))
         self._batches[slot] = b
 
+    def submit_staged(self, batch: Batch, slot: int = 0, trusted: bool = False) -> None:
+        """Writes the batch straight into the slot's pinned staging, then uploads."""
+        b = batch.normalized()
+        st = _Staging()
+        self._check("ltpmg_stage_batch",
+                    self._lib.ltpmg_stage_batch(self._h, slot, b.n_txns, b.n_ops, C.byref(st)))
+        C.memmove(st.txn_off, b.txn_off.ctypes.data, b.txn_off.nbytes)
+        if b.n_ops:
+            C.memmove(st.rows, b.rows.ctypes.data, b.rows.nbytes)
+            C.memmove(st.meta, b.meta.ctypes.data, b.meta.nbytes)
+            C.memmove(st.args, b.args.ctypes.data, b.args.nbytes)
+        self._check("ltpmg_submit_staged",
+                    self._lib.ltpmg_submit_staged(self._h, slot, 1 if trusted else 0))
+        self._batches[slot] = b
+
     def execute(self, mode: int, slot: int = 0, profile: bool = False) -> None:
         m = mode | (EXEC_PROFILE if profile else 0)
         self._check("ltpmg_execute", self._lib.ltpmg_execute(self._h, slot, m))
+
+    def idle(self, slot: int = 0) -> bool:
+        rc = self._lib.ltpmg_query(self._h, slot)
+        if rc not in (0, 1):
+            self._check("ltpmg_query", rc)
+        return rc == 0
 
     def wait(self, slot: int = 0) -> None:
         self._check("ltpmg_wait", self._lib.ltpmg_wait(self._h, slot))
@@ -326,3 +382,49 @@ class Engine:
         self._check("ltpmg_selftest_segscan",
                     self._lib.ltpmg_selftest_segscan(self._h, _ptr(v), _ptr(h), _ptr(out), len(v)))
         return out
+
+    def serve(self, pool: Batch, *, arrival_rate: float, n_arrivals: int, max_batch: int,
+              min_batch: int = 0, max_wait_us: float = 200.0, n_inflight: int = 2,
+              mode: int = MODE_O, seed: int = 1, pack_threads: int = 8, record: bool = False,
+              record_results: bool = False) -> dict:
+        """Open-loop serving run (include/ltpmg_serve.h).  Returns the stats,
+        plus the batch record when asked for."""
+        p = pool.normalized()
+        cfg = _ServeCfg(arrival_rate, n_arrivals, max_batch, min_batch, max_wait_us, n_inflight,
+                        mode, seed & 0xFFFFFFFFFFFFFFFF, pack_threads)
+        st = _ServeStats()
+        rec = None
+        keep = {}
+        if record:
+            cap_e = n_arrivals * (64 if mode == MODE_A else 1) + 16
+            keep["off"] = np.zeros(cap_e + 1, np.uint32)
+            keep["txns"] = np.zeros(cap_e, np.uint32)
+            keep["commit"] = np.zeros(cap_e, np.uint8)
+            if record_results:
+                keep["results"] = np.zeros((p.n_ops, self.result_stride), np.uint8)
+            rec = _ServeRecord(keep["off"].ctypes.data, keep["txns"].ctypes.data,
+                               keep["commit"].ctypes.data, cap_e, cap_e, 0, 0,
+                               keep["results"].ctypes.data if record_results else None)
+        rc = host_lib().ltpmg_serve_run(self._h, C.byref(cfg), C.byref(p.c_struct()), C.byref(st),
+                                        C.byref(rec) if rec is not None else None)
+        self._check("ltpmg_serve_run", rc)
+        out = {f: getattr(st, f) for f, _ in _ServeStats._fields_}
+        if rec is not None:
+            nb, ne = rec.n_batches, rec.n_entries
+            out["batch_off"] = keep["off"][:nb + 1].copy()
+            out["batch_txns"] = keep["txns"][:ne].copy()
+            out["batch_commit"] = keep["commit"][:ne].copy()
+            if record_results:
+                out["results"] = keep["results"]
+        return out
+
+
+def take_txns(pool: Batch, ids: np.ndarray) -> Batch:
+    """The batch made of pool transactions `ids`, in that order (TID = position)."""
+    ids = np.asarray(ids, np.int64)
+    lens = np.diff(pool.txn_off.astype(np.int64))[ids]
+    off = np.zeros(len(ids) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    starts = pool.txn_off.astype(np.int64)[ids]
+    idx = np.repeat(starts - off[:-1], lens) + np.arange(off[-1])
+    return Batch(off.astype(np.uint32), pool.rows[idx], pool.meta[idx], pool.args[idx])

@@ -1,0 +1,219 @@
+// server.cpp -- open-loop serving over the engine (include/ltpmg_serve.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "ltpmg.h"
+#include "ltpmg_serve.h"
+
+namespace {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+using Clock = std::chrono::steady_clock;
+
+struct Flight {
+    uint32_t slot;
+    size_t first;  // index into `entries`
+    uint32_t n;
+    uint32_t n_ops;
+};
+
+double pct(std::vector<double> &v, double q) {
+    if (v.empty()) return 0.0;
+    size_t k = (size_t)std::min<double>((double)v.size() - 1, std::floor(q * (double)(v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+    return v[k];
+}
+
+}  // namespace
+
+extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, const ltpmg_batch *pool,
+                               ltpmg_serve_stats *out, ltpmg_serve_record *rec) {
+    if (!e || !cfg || !pool || !out || pool->n_txns == 0) return LTPMG_EINVAL;
+    if (cfg->max_batch == 0 || cfg->n_inflight == 0 || cfg->n_inflight > LTPMG_MAX_SLOTS ||
+        (cfg->mode != LTPMG_MODE_A && cfg->mode != LTPMG_MODE_O) || cfg->arrival_rate < 0)
+        return LTPMG_EINVAL;
+    if (rec && rec->results && cfg->n_arrivals > pool->n_txns) return LTPMG_EINVAL;
+    int rc = ltpmg_check_batch(e, pool);
+    if (rc != LTPMG_OK) return rc;
+    std::memset(out, 0, sizeof *out);
+    const uint32_t stride = ltpmg_result_stride(e);
+    const uint32_t n = cfg->n_arrivals;
+
+    // scheduled arrival times (s): exponential gaps from a SplitMix64 stream
+    std::vector<double> arrive(n, 0.0);
+    if (cfg->arrival_rate > 0) {
+        double t = 0.0;
+        uint64_t s = mix64(cfg->seed + kGolden);
+        for (uint32_t i = 0; i < n; ++i) {
+            s += kGolden;
+            double u = (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0);
+            t += -std::log1p(-u) / cfg->arrival_rate;
+            arrive[i] = t;
+        }
+    }
+    auto pool_txn = [&](uint32_t id) { return id % pool->n_txns; };
+
+    std::deque<uint32_t> queue;
+    std::vector<uint32_t> entries;  // arrival ids of all cut batches, in order
+    std::vector<uint8_t> commit_buf, res_buf;
+    std::deque<Flight> inflight;
+    std::vector<uint32_t> free_slots;
+    for (uint32_t s = cfg->n_inflight; s-- > 0;) free_slots.push_back(s);
+    std::vector<double> latency;
+    latency.reserve(n);
+    uint32_t next = 0, done = 0;
+    double pack_s = 0.0, device_ms = 0.0;
+    uint64_t batches = 0, retries = 0, batch_txn_sum = 0;
+    const unsigned pack_threads = cfg->pack_threads ? cfg->pack_threads : 8;
+    if (rec) {
+        rec->n_batches = rec->n_entries = 0;
+        if (rec->batch_off && rec->cap_batches) rec->batch_off[0] = 0;
+    }
+
+    const auto t0 = Clock::now();
+    auto now_s = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    double last_done = 0.0;
+
+    while (done < n) {
+        bool progressed = false;
+        const double now = now_s();
+        while (next < n && arrive[next] <= now) queue.push_back(next++);
+
+        // completions, in cut order
+        while (!inflight.empty()) {
+            const Flight f = inflight.front();
+            int q = ltpmg_query(e, f.slot);
+            if (q == 1) break;
+            if (q != LTPMG_OK) return q;
+            const double t_done = now_s();
+            last_done = t_done;
+            commit_buf.resize(f.n);
+            const bool want_res = rec && rec->results;
+            if (want_res) res_buf.resize((size_t)f.n_ops * stride);
+            rc = ltpmg_fetch_results(e, f.slot, commit_buf.data(), want_res ? res_buf.data() : nullptr,
+                                     want_res ? res_buf.size() : 0);
+            if (rc != LTPMG_OK) return rc;
+            ltpmg_batch_stats st;
+            if (ltpmg_stats(e, f.slot, &st) == LTPMG_OK) device_ms += st.exec_ms;
+            std::vector<uint32_t> retry;
+            size_t op = 0;
+            for (uint32_t k = 0; k < f.n; ++k) {
+                uint32_t id = entries[f.first + k], pt = pool_txn(id);
+                uint32_t len = pool->txn_off[pt + 1] - pool->txn_off[pt];
+                if (rec && rec->batch_commit && f.first + k < rec->cap_entries)
+                    rec->batch_commit[f.first + k] = commit_buf[k];
+                if (commit_buf[k]) {
+                    latency.push_back(t_done - arrive[id]);
+                    ++done;
+                    if (want_res)
+                        std::memcpy(rec->results + (size_t)pool->txn_off[pt] * stride,
+                                    res_buf.data() + op * stride, (size_t)len * stride);
+                } else {
+                    retry.push_back(id);
+                }
+                op += len;
+            }
+            retries += retry.size();
+            for (size_t k = retry.size(); k-- > 0;) queue.push_front(retry[k]);
+            free_slots.push_back(f.slot);
+            inflight.pop_front();
+            progressed = true;
+        }
+
+        // cut a batch into a free slot
+        if (!free_slots.empty() && !queue.empty()) {
+            bool cut;
+            if (cfg->min_batch) {
+                cut = queue.size() >= cfg->min_batch || (next == n && inflight.empty());
+            } else {
+                cut = queue.size() >= cfg->max_batch ||
+                      (now - arrive[queue.front()]) * 1e6 >= cfg->max_wait_us || next == n;
+            }
+            if (cut) {
+                const auto p0 = Clock::now();
+                const uint32_t k = (uint32_t)std::min<size_t>(queue.size(), cfg->max_batch);
+                const size_t first = entries.size();
+                uint64_t n_ops = 0;
+                for (uint32_t j = 0; j < k; ++j) {
+                    uint32_t id = queue.front();
+                    queue.pop_front();
+                    entries.push_back(id);
+                    uint32_t pt = pool_txn(id);
+                    n_ops += pool->txn_off[pt + 1] - pool->txn_off[pt];
+                }
+                const uint32_t slot = free_slots.back();
+                ltpmg_staging st;
+                rc = ltpmg_stage_batch(e, slot, k, (uint32_t)n_ops, &st);
+                if (rc != LTPMG_OK) return rc;
+                // offsets first (serial prefix), then op copies in parallel
+                st.txn_off[0] = 0;
+                for (uint32_t j = 0; j < k; ++j) {
+                    uint32_t pt = pool_txn(entries[first + j]);
+                    st.txn_off[j + 1] = st.txn_off[j] + (pool->txn_off[pt + 1] - pool->txn_off[pt]);
+                }
+                auto copy = [&](uint32_t a, uint32_t b) {
+                    for (uint32_t j = a; j < b; ++j) {
+                        uint32_t pt = pool_txn(entries[first + j]);
+                        uint32_t src = pool->txn_off[pt], len = pool->txn_off[pt + 1] - src, dst = st.txn_off[j];
+                        std::memcpy(st.rows + dst, pool->rows + src, (size_t)len * 4);
+                        std::memcpy(st.meta + dst, pool->meta + src, (size_t)len * 2);
+                        std::memcpy(st.args + dst, pool->args + src, (size_t)len * 8);
+                    }
+                };
+                const unsigned nt = k >= 65536 ? pack_threads : 1;
+                if (nt == 1) {
+                    copy(0, k);
+                } else {
+                    std::vector<std::thread> th;
+                    for (unsigned w = 0; w < nt; ++w)
+                        th.emplace_back(copy, (uint32_t)((uint64_t)k * w / nt), (uint32_t)((uint64_t)k * (w + 1) / nt));
+                    for (auto &t : th) t.join();
+                }
+                rc = ltpmg_submit_staged(e, slot, LTPMG_SUBMIT_TRUSTED);  // the pool was checked
+                if (rc == LTPMG_OK) rc = ltpmg_execute(e, slot, cfg->mode);
+                if (rc != LTPMG_OK) return rc;
+                free_slots.pop_back();
+                inflight.push_back(Flight{slot, first, k, (uint32_t)n_ops});
+                ++batches;
+                batch_txn_sum += k;
+                if (rec && rec->batch_off && rec->batch_txns) {
+                    if (rec->n_batches < rec->cap_batches && first + k <= rec->cap_entries) {
+                        std::memcpy(rec->batch_txns + first, entries.data() + first, (size_t)k * 4);
+                        rec->batch_off[++rec->n_batches] = (uint32_t)(first + k);
+                        rec->n_entries = (uint32_t)(first + k);
+                    }
+                }
+                pack_s += std::chrono::duration<double>(Clock::now() - p0).count();
+                progressed = true;
+            }
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+
+    out->offered = n;
+    out->committed = done;
+    out->aborted_attempts = retries;
+    out->batches = batches;
+    out->wall_s = last_done - (n ? arrive[0] : 0.0);
+    out->committed_per_s = out->wall_s > 0 ? done / out->wall_s : 0.0;
+    out->p50_ms = 1e3 * pct(latency, 0.50);
+    out->p90_ms = 1e3 * pct(latency, 0.90);
+    out->p99_ms = 1e3 * pct(latency, 0.99);
+    out->p999_ms = 1e3 * pct(latency, 0.999);
+    out->max_ms = latency.empty() ? 0.0 : 1e3 * *std::max_element(latency.begin(), latency.end());
+    out->mean_batch_txns = batches ? (double)batch_txn_sum / batches : 0.0;
+    out->host_pack_s = pack_s;
+    out->device_ms = device_ms;
+    return LTPMG_OK;
+}

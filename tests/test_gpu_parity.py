@@ -258,6 +258,53 @@ def test_batch_sequence_and_determinism():
     assert outs[0][1] == outs[1][1]
 
 
+@pytest.mark.parametrize("mode", BOTH)
+def test_two_slots_apply_in_execute_order(mode):
+    """Batches alternate between two slots without host waits in between; the
+    table must still see them in execute order (the serving form's overlap)."""
+    schema = {0: (3000, [4, 100])}
+    rng = np.random.default_rng(44)
+    batches = [random_batch(rng, 4000, 3000, 2, max_len=8, zipf=0.8) for _ in range(6)]
+    got = []
+    with Engine(n_slots=2) as e:
+        im = _setup(e, schema, 12)
+        for k, b in enumerate(batches):
+            slot = k % 2
+            if k >= 2:
+                got.append(e.fetch(slot))
+            e.submit(b, slot)
+            e.execute(mode, slot)
+        got.append(e.fetch(0))
+        got.append(e.fetch(1))
+        for k, b in enumerate(batches):
+            oc, ores = orc.replay(im, b, mode, e.result_stride)
+            commit, res = got[k]
+            assert (commit == oc).all(), k
+            m = orc.read_mask(b, oc)
+            assert (res[m] == ores[m]).all(), k
+        for c in range(2):
+            assert (e.read_column(0, c) == im.cols[0][c]).all()
+
+
+@pytest.mark.parametrize("trusted", [False, True])
+def test_staged_submit(trusted):
+    rng = np.random.default_rng(55)
+    b = random_batch(rng, 3000, 2000, 3, max_len=10, zipf=0.7)
+    with Engine() as e:
+        im = _setup(e, {0: (2000, [4, 100, 8])}, 6)
+        for mode in BOTH:
+            e.submit_staged(b, trusted=trusted)
+            e.execute(mode)
+            commit, res = e.fetch()
+            oc, ores = orc.replay(im, b, mode, e.result_stride)
+            assert (commit == oc).all()
+            m = orc.read_mask(b, oc)
+            assert (res[m] == ores[m]).all()
+        bad = Batch.from_txns([[(OP_READ, 0, 0, 2000, 0)]])
+        with pytest.raises(LtpmgError):
+            e.submit_staged(bad, trusted=False)
+
+
 # ---------------- error paths ----------------
 def test_invalid_batches_are_rejected():
     with Engine() as e:
