@@ -188,6 +188,12 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
                         uint32_t key_bits);
 int ltpmg_selftest_segscan(ltpmg_engine *e, const uint32_t *vals, const uint8_t *head,
                            uint32_t *out, uint32_t n);
+/* Streaming copy of n_bytes (16-byte vectors) between two device buffers,
+ * iters times; *gbps gets (read + write bytes) / average kernel time.  The
+ * measured HBM copy peak that rooflines are also quoted against, and the
+ * known-size kernel that calibrates hardware byte counters.  No counterpart
+ * in the reference. */
+int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t iters, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,7 @@ typedef struct {
     uint32_t cap_entries;
     uint32_t n_batches;     /* written */
     uint32_t n_entries;     /* written */
+    uint32_t overflow;      /* written: 1 if a cap was hit and recording stopped */
     uint8_t *results;       /* optional, needs n_arrivals <= pool txns: READ result of
                                pool op i at results + i * result_stride, written when
                                its transaction commits */

@@ -838,3 +838,34 @@ int ltpmg_selftest_segscan(ltpmg_engine *e, const uint32_t *vals, const uint8_t 
 }
 
 }  // extern "C"
+
+extern "C" int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t iters, double *gbps) {
+    if (!e || !gbps || iters == 0 || n_bytes < 16) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    hipStream_t st = e->slots[0].stream;
+    const uint64_t n16 = n_bytes / 16, bytes = n16 * 16;
+    void *a = nullptr, *b = nullptr;
+    int rc = dev_alloc(e, &a, bytes);
+    if (rc == LTPMG_OK) rc = dev_alloc(e, &b, bytes);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (rc == LTPMG_OK && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess))
+        rc = fail(e, LTPMG_EHIP, "hipEventCreate failed");
+    if (rc == LTPMG_OK) {
+        launch_zero_u32(st, static_cast<uint32_t *>(a), (uint32_t)std::min<uint64_t>(bytes / 4, 0xFFFFFFFFu));
+        launch_stream_copy(st, a, b, n16);  // warm-up
+        (void)hipEventRecord(t0, st);
+        for (uint32_t i = 0; i < iters; ++i) launch_stream_copy(st, a, b, n16);
+        (void)hipEventRecord(t1, st);
+        float ms = 0.f;
+        if (hipStreamSynchronize(st) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+            rc = fail(e, LTPMG_EHIP, "copy kernels failed");
+        else
+            *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    dev_free(e, a, bytes);
+    dev_free(e, b, bytes);
+    return rc;
+}

@@ -142,6 +142,7 @@ void launch_digest_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t co
 void launch_tile_bounds(hipStream_t s, const uint32_t *txn_off, uint32_t n_txns,
                         uint32_t n_tiles, uint32_t *tile_t0);
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
+void launch_stream_copy(hipStream_t s, const void *src, void *dst, uint64_t n16);
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                        uint32_t n_tiles, uint32_t epoch);
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,

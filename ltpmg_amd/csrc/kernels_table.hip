@@ -76,6 +76,20 @@ void launch_digest_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t co
                        n_rows, acc);
 }
 
+// Plain streaming copy (the bandwidth reference): 16 B per lane, grid-stride.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4 *__restrict__ src,
+                                                        uint4 *__restrict__ dst, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_stream_copy(hipStream_t s, const void *src, void *dst, uint64_t n16) {
+    if (n16 == 0) return;
+    hipLaunchKernelGGL(k_stream_copy, dim3(256 * 64), dim3(kBlock), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = 0;

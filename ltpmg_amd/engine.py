@@ -102,6 +102,8 @@ ENGINE_API = {
                                       C.c_uint32]),
     "ltpmg_selftest_segscan": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_uint32]),
+    "ltpmg_selftest_copy": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32,
+                                      C.POINTER(C.c_double)]),
 }
 
 class _ServeCfg(C.Structure):
@@ -121,7 +123,7 @@ class _ServeStats(C.Structure):
 class _ServeRecord(C.Structure):
     _fields_ = [("batch_off", C.c_void_p), ("batch_txns", C.c_void_p), ("batch_commit", C.c_void_p),
                 ("cap_batches", C.c_uint32), ("cap_entries", C.c_uint32), ("n_batches", C.c_uint32),
-                ("n_entries", C.c_uint32), ("results", C.c_void_p)]
+                ("n_entries", C.c_uint32), ("overflow", C.c_uint32), ("results", C.c_void_p)]
 
 
 WORKLOAD_API = {
@@ -383,12 +385,20 @@ class Engine:
                     self._lib.ltpmg_selftest_segscan(self._h, _ptr(v), _ptr(h), _ptr(out), len(v)))
         return out
 
+    def copy_bandwidth(self, n_bytes: int = 1 << 30, iters: int = 20) -> float:
+        """GB/s (read + write) of a plain streaming copy of n_bytes."""
+        g = C.c_double(0.0)
+        self._check("ltpmg_selftest_copy",
+                    self._lib.ltpmg_selftest_copy(self._h, n_bytes, iters, C.byref(g)))
+        return g.value
+
     def serve(self, pool: Batch, *, arrival_rate: float, n_arrivals: int, max_batch: int,
               min_batch: int = 0, max_wait_us: float = 200.0, n_inflight: int = 2,
               mode: int = MODE_O, seed: int = 1, pack_threads: int = 8, record: bool = False,
-              record_results: bool = False) -> dict:
+              record_results: bool = False, record_cap: int | None = None) -> dict:
         """Open-loop serving run (include/ltpmg_serve.h).  Returns the stats,
-        plus the batch record when asked for."""
+        plus the batch record when asked for (record_cap entries; retried
+        txns of the lowest-TID-wins mode take one entry per attempt)."""
         p = pool.normalized()
         cfg = _ServeCfg(arrival_rate, n_arrivals, max_batch, min_batch, max_wait_us, n_inflight,
                         mode, seed & 0xFFFFFFFFFFFFFFFF, pack_threads)
@@ -396,14 +406,14 @@ class Engine:
         rec = None
         keep = {}
         if record:
-            cap_e = n_arrivals * (64 if mode == MODE_A else 1) + 16
+            cap_e = record_cap or (n_arrivals * (8 if mode == MODE_A else 1) + 16)
             keep["off"] = np.zeros(cap_e + 1, np.uint32)
             keep["txns"] = np.zeros(cap_e, np.uint32)
             keep["commit"] = np.zeros(cap_e, np.uint8)
             if record_results:
                 keep["results"] = np.zeros((p.n_ops, self.result_stride), np.uint8)
             rec = _ServeRecord(keep["off"].ctypes.data, keep["txns"].ctypes.data,
-                               keep["commit"].ctypes.data, cap_e, cap_e, 0, 0,
+                               keep["commit"].ctypes.data, cap_e, cap_e, 0, 0, 0,
                                keep["results"].ctypes.data if record_results else None)
         rc = host_lib().ltpmg_serve_run(self._h, C.byref(cfg), C.byref(p.c_struct()), C.byref(st),
                                         C.byref(rec) if rec is not None else None)
@@ -411,6 +421,7 @@ class Engine:
         out = {f: getattr(st, f) for f, _ in _ServeStats._fields_}
         if rec is not None:
             nb, ne = rec.n_batches, rec.n_entries
+            out["record_overflow"] = bool(rec.overflow)
             out["batch_off"] = keep["off"][:nb + 1].copy()
             out["batch_txns"] = keep["txns"][:ne].copy()
             out["batch_commit"] = keep["commit"][:ne].copy()

@@ -77,7 +77,7 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
     uint64_t batches = 0, retries = 0, batch_txn_sum = 0;
     const unsigned pack_threads = cfg->pack_threads ? cfg->pack_threads : 8;
     if (rec) {
-        rec->n_batches = rec->n_entries = 0;
+        rec->n_batches = rec->n_entries = rec->overflow = 0;
         if (rec->batch_off && rec->cap_batches) rec->batch_off[0] = 0;
     }
 
@@ -187,11 +187,13 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
                 inflight.push_back(Flight{slot, first, k, (uint32_t)n_ops});
                 ++batches;
                 batch_txn_sum += k;
-                if (rec && rec->batch_off && rec->batch_txns) {
+                if (rec && rec->batch_off && rec->batch_txns && !rec->overflow) {
                     if (rec->n_batches < rec->cap_batches && first + k <= rec->cap_entries) {
                         std::memcpy(rec->batch_txns + first, entries.data() + first, (size_t)k * 4);
                         rec->batch_off[++rec->n_batches] = (uint32_t)(first + k);
                         rec->n_entries = (uint32_t)(first + k);
+                    } else {
+                        rec->overflow = 1;
                     }
                 }
                 pack_s += std::chrono::duration<double>(Clock::now() - p0).count();

@@ -26,9 +26,9 @@ uint64_t orc_cellseed(uint64_t seed, uint32_t table, uint32_t row, uint32_t col)
 
 void orc_bytes_from_stream(uint64_t s, uint32_t width, uint8_t *out) {
     /* little-endian bytes of stream(s,0), stream(s,1), ... */
-    for (uint32_t b = 0; b < width; ++b) {
-        uint64_t v = orc_stream(s, b / 8);
-        out[b] = (uint8_t)(v >> (8 * (b % 8)));
+    for (uint32_t w = 0; 8 * w < width; ++w) {
+        uint64_t v = orc_stream(s, w);
+        for (uint32_t k = 0; k < 8 && 8 * w + k < width; ++k) out[8 * w + k] = (uint8_t)(v >> (8 * k));
     }
 }
 
