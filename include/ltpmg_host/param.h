@@ -2,10 +2,12 @@
 // same class, fields and methods, built from the flags in common.h).
 //
 // Benchmarks: YCSB_A..E (the reference's key-value mixes over its 10-column
-// YCSB table), plus this build's KV_C1 (int32 columns, READ whole row /
-// RMW +1) and KV (fields of kv_field_bytes, READ / UPDATE one field).
-// TEST and TPCC_* sizes are kept so existing callers compile; the
-// order-entry adapter that would execute TPCC_* is not built.
+// YCSB table); TPCC_PART (Neworder + Payment by neworder_percent) and OE
+// (45% Neworder, 43% Payment, 12% read-only) over the order-entry schema of
+// include/ltpmg_oe.h, table_size = partitions; plus this build's KV_C1
+// (int32 columns, READ whole row / RMW +1) and KV (fields of kv_field_bytes,
+// READ / UPDATE one field).  TEST and TPCC_ALL keep their sizes; TPCC_ALL's
+// Delivery and the TEST queries are not built.
 #pragma once
 #include "common.h"
 #include "db_structure.h"
@@ -112,6 +114,7 @@ public:
     uint32_t kv_ops = 16;
     uint32_t read_percent = 50;
     uint32_t kv_sub_txn_size = 0;  // upper bound on device ops per batch
+    uint32_t oe_append_cap = 1u << 20;  // order-entry append tables, in transactions
 
     Result result;
 
@@ -138,4 +141,6 @@ public:
 
     bool is_ycsb() const { return benchmark.rfind("YCSB_", 0) == 0; }
     bool is_kv() const { return benchmark == "KV" || benchmark == "KV_C1"; }
+    // order-entry schema (include/ltpmg_oe.h): TPCC_PART, TPCC_ALL and the OE mix
+    bool is_oe() const { return benchmark == "TPCC_PART" || benchmark == "TPCC_ALL" || benchmark == "OE"; }
 };

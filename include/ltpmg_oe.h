@@ -1,0 +1,92 @@
+/*
+ * ltpmg_oe.h -- the order-entry workload adapter (libltpmg_host.so, C ABI).
+ *
+ * The reference's TPCC_* benchmarks: nine tables (data/TPCC.json; table ids
+ * as src/param.cpp:25-100 orders them) and query structs decomposed into
+ * single-row ops by parse<Neworder_Query> / parse<Payment_Query> / ...
+ * (cuda/src/gpuquery.cu:2473-2927).  This adapter keeps the table ids, the
+ * partition keys and the query structs, and turns each query into engine ops
+ * (include/ltpmg.h) on these columns:
+ *
+ *   0 warehouse  W rows          ints w_id w_tax w_ytd; 6 strings of 32 B
+ *   1 district   10 W            ints d_id d_w_id d_next_o_id d_tax d_ytd; 6 strings
+ *   2 customer   30000 W         ints c_id c_d_id c_w_id c_payment_cnt c_delivery_cnt
+ *                                c_discount c_credit_lim c_balance c_ytd_payment;
+ *                                11 strings of 32 B; c_data 256 B   (644 B per row)
+ *   3 neworder   append          ints no_o_id no_d_id no_w_id
+ *   4 history    append          ints h_c_id h_c_d_id h_c_w_id h_d_id h_w_id h_amount;
+ *                                h_date, h_data 32 B
+ *   5 order      append          ints o_id o_d_id o_w_id o_c_id o_carrier_id o_ol_cnt
+ *                                o_all_local; o_entry_d 32 B
+ *   6 orderline  append (x15)    ints ol_o_id ol_d_id ol_w_id ol_number ol_i_id
+ *                                ol_supply_w_id ol_delivery_d ol_quantity ol_amount;
+ *                                ol_dist_info 24 B
+ *   7 stock      100000 W        ints s_i_id s_w_id s_quantity s_ytd s_order_cnt
+ *                                s_remote_cnt; 10 s_dist of 24 B; s_data 32 B (296 B)
+ *   8 item       100000          ints i_id i_im_id i_price; i_name 24 B, i_data 52 B
+ *
+ * Money and rates are integer columns (cents, basis points) so that RMW_ADD
+ * (a u32 add) is the arithmetic the transaction means.  Keys: district
+ * W*10+D, customer (W*10+D)*3000+C, stock W*100000+I (the reference uses
+ * W*10^6, gpuquery.cu:2559, outside its W*10^5-row table).  Inserts go to
+ * append-only tables at rows the generator hands out in order (modulo the
+ * table's capacity), so no two transactions insert the same row.
+ *
+ * Query -> ops (a read of a cell may precede a write of it in one txn):
+ *   Neworder (W, D, C, lines): READ w_tax; READ d_tax, d_next_o_id; RMW
+ *     d_next_o_id +1; READ c_discount, c_last, c_credit; insert order (4
+ *     cells) and neworder (3 cells); per line READ i_price, i_name, RMW
+ *     s_quantity -q, s_ytd +q, s_order_cnt +1 (s_remote_cnt +1 when the
+ *     supplying partition differs), insert the orderline (5 cells).
+ *   Payment (W, D, C, amount): RMW w_ytd, d_ytd +amount; READ w_name,
+ *     d_name; RMW c_balance -amount, c_ytd_payment +amount, c_payment_cnt
+ *     +1; READ c_first, c_last, c_data; insert history (5 cells).
+ *   Orderstatus (read-only): customer balance/first/last, the order row, 5
+ *     orderline rows.
+ *   Stocklevel (read-only): d_next_o_id, 10 orderlines' ol_i_id and those
+ *     items' s_quantity.
+ * The customer-by-last-name path (the C_LAST index, gpuquery.cu:2641-2650)
+ * is not built: Payment and Orderstatus always address the customer by id.
+ */
+#ifndef LTPMG_OE_H
+#define LTPMG_OE_H
+
+#include <stdint.h>
+
+#include "ltpmg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LTPMG_OE_TABLES 9
+
+typedef struct {
+    uint32_t n_txns;
+    uint32_t warehouses;      /* W partitions */
+    uint32_t neworder_pct;    /* percent multi-line orders */
+    uint32_t payment_pct;     /* percent payments; the rest read-only (half each kind) */
+    uint32_t lines_min;       /* order lines per Neworder, uniform */
+    uint32_t lines_max;
+    uint32_t remote_bp;       /* per line: basis points of a remote supplying partition */
+    uint32_t append_cap;      /* capacity of the append tables, in transactions */
+    uint32_t append_next[4];  /* in/out: next row of neworder, history, order, orderline */
+    uint64_t seed;
+} ltpmg_oe_spec;
+
+/* Fills out[9] with the table descriptors for W partitions and an append
+ * capacity of append_cap transactions. */
+int ltpmg_oe_schema(uint32_t warehouses, uint32_t append_cap, ltpmg_table_desc out[LTPMG_OE_TABLES]);
+
+/* Upper bound on device ops of a batch. */
+uint64_t ltpmg_gen_oe_max_ops(const ltpmg_oe_spec *spec);
+
+/* Generates a batch of the order-entry mix (seeded, thread-count
+ * independent) and advances spec->append_next. */
+int ltpmg_gen_oe(ltpmg_oe_spec *spec, uint32_t *txn_off, uint32_t *rows, uint16_t *meta,
+                 uint64_t *args, uint64_t max_ops, uint32_t *n_ops_out, uint32_t n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

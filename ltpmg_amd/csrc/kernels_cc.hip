@@ -62,6 +62,76 @@ __device__ inline void wave_append(bool pred, uint32_t v, uint32_t *list, uint32
     if (pred) list[base + lanemask_lt_popc(m)] = v;
 }
 
+// Moves the cells of the ops in an LDS list, G lanes per op and R ops per
+// round, all independent loads of a round issued before its stores.
+// READS: cell -> result slot (res_stride bytes, zero past the slot).
+// Otherwise: UPDATE writes the payload of args[o], RMW_ADD adds (u32)args[o].
+template <int G, bool VEC16, bool READS>
+__device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &ov, DevTables dt,
+                                uint8_t *results, uint32_t res_stride) {
+    constexpr int R = 4;
+    constexpr uint32_t groups = kBlock / G;
+    const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G, b0 = lane * 16;
+    for (uint32_t base = g; base < n; base += groups * R) {
+        uint32_t o[R];
+        ColDesc cd[R];
+        uint8_t *cell[R];
+        uint64_t arg[R];
+        bool upd[R];
+        u32x4 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t k = base + r * groups;
+            cell[r] = nullptr;
+            cd[r] = ColDesc{nullptr, 0u, 0u};
+            upd[r] = false;
+            o[r] = 0;
+            if (k < n) {
+                o[r] = list[k];
+                const uint16_t m = ov.meta[o[r]];
+                cd[r] = dt.cols[op_tab(m) * LTPMG_MAX_COLS + op_col(m)];
+                cell[r] = cd[r].base + (uint64_t)ov.rows[o[r]] * cd[r].slot;
+                upd[r] = op_kind(m) == LTPMG_OP_UPDATE;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            arg[r] = (!READS && cell[r]) ? ov.args[o[r]] : 0ull;
+            v[r] = u32x4{0u, 0u, 0u, 0u};
+            if (VEC16 && cell[r] && !upd[r] && cd[r].slot >= 16 && b0 < cd[r].slot) v[r] = ld16(cell[r] + b0);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!cell[r]) continue;
+            if (READS) {
+                uint8_t *dst = results + (uint64_t)o[r] * res_stride;
+                if (VEC16 && cd[r].slot >= 16) {
+                    for (uint32_t b = b0; b < res_stride; b += G * 16) {
+                        u32x4 x = b < cd[r].slot ? (b == b0 ? v[r] : ld16(cell[r] + b)) : u32x4{0u, 0u, 0u, 0u};
+                        st16_nt(dst + b, x);
+                    }
+                } else {  // 4- or 8-byte slots (also a narrow column beside wider ones)
+                    for (uint32_t b = lane * 4; b < res_stride; b += G * 4) {
+                        uint32_t x = b < cd[r].slot ? *reinterpret_cast<const uint32_t *>(cell[r] + b) : 0u;
+                        *reinterpret_cast<uint32_t *>(dst + b) = x;
+                    }
+                }
+            } else if (VEC16 && cd[r].slot >= 16) {
+                for (uint32_t b = b0; b < cd[r].slot; b += G * 16) {
+                    u32x4 x = upd[r] ? payload16(arg[r], b, cd[r].width)
+                                     : add16(b == b0 ? v[r] : ld16(cell[r] + b), (uint32_t)arg[r], b, cd[r].width);
+                    st16(cell[r] + b, x);
+                }
+            } else {
+                for (uint32_t b = lane * 4; b < cd[r].slot; b += G * 4) {
+                    uint32_t *c = reinterpret_cast<uint32_t *>(cell[r] + b);
+                    if (b < cd[r].width) *c = upd[r] ? stream_word(arg[r], b >> 2) : *c + (uint32_t)arg[r];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
@@ -133,26 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
     __syncthreads();
     if (threadIdx.x == 0 && n_commit) atomicAdd(&counters[0], n_commit);
 
-    const uint32_t n = n_list;
-    const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G;
-    for (uint32_t k = g; k < n; k += kBlock / G) {
-        uint32_t o = list[k];
-        uint16_t m = ov.meta[o];
-        ColDesc cd = dt.cols[op_tab(m) * LTPMG_MAX_COLS + op_col(m)];
-        const uint8_t *src = cd.base + (uint64_t)ov.rows[o] * cd.slot;
-        uint8_t *dst = results + (uint64_t)o * res_stride;
-        if (VEC16 && cd.slot >= 16) {
-            for (uint32_t b = lane * 16; b < res_stride; b += G * 16) {
-                u32x4 v = b < cd.slot ? ld16(src + b) : u32x4{0, 0, 0, 0};
-                st16_nt(dst + b, v);
-            }
-        } else {  // 4- or 8-byte slots (also a narrow column beside wider ones)
-            for (uint32_t b = lane * 4; b < res_stride; b += G * 4) {
-                uint32_t v = b < cd.slot ? *reinterpret_cast<const uint32_t *>(src + b) : 0u;
-                *reinterpret_cast<uint32_t *>(dst + b) = v;
-            }
-        }
-    }
+    move_ops<G, VEC16, true>(n_list, list, ov, dt, results, res_stride);
 }
 
 template <int G, bool VEC16>
@@ -178,27 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
     }
     __syncthreads();
 
-    const uint32_t n = n_list;
-    const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G;
-    for (uint32_t k = g; k < n; k += kBlock / G) {
-        uint32_t o = list[k];
-        uint16_t m = ov.meta[o];
-        ColDesc cd = dt.cols[op_tab(m) * LTPMG_MAX_COLS + op_col(m)];
-        uint8_t *cell = cd.base + (uint64_t)ov.rows[o] * cd.slot;
-        uint64_t arg = ov.args[o];
-        bool upd = op_kind(m) == LTPMG_OP_UPDATE;
-        if (VEC16 && cd.slot >= 16) {
-            for (uint32_t b = lane * 16; b < cd.slot; b += G * 16) {
-                u32x4 v = upd ? payload16(arg, b, cd.width) : add16(ld16(cell + b), (uint32_t)arg, b, cd.width);
-                st16(cell + b, v);
-            }
-        } else {
-            for (uint32_t b = lane * 4; b < cd.slot; b += G * 4) {
-                uint32_t *c = reinterpret_cast<uint32_t *>(cell + b);
-                if (b < cd.width) *c = upd ? stream_word(arg, b >> 2) : *c + (uint32_t)arg;
-            }
-        }
-    }
+    move_ops<G, VEC16, false>(n_list, list, ov, dt, nullptr, 0u);
 }
 
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,

@@ -124,18 +124,18 @@ __global__ __launch_bounds__(kBlock) void k_seg_reduce(Src src, uint32_t *tflag,
 __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
                                                       const uint32_t *tdelta, uint32_t n_tiles,
                                                       uint32_t *cset, uint32_t *cdelta) {
-    Seg run{0u, kNoSet, 0u};
-    for (uint32_t base = 0; base < n_tiles; base += kBlock) {
-        uint32_t k = base + threadIdx.x;
-        Seg x = k < n_tiles ? Seg{tflag[k], tset[k], tdelta[k]} : Seg{0u, kNoSet, 0u};
-        Seg tot;
-        Seg ex = block_seg_exclusive(x, &tot);
-        Seg c = seg_op(run, ex);
-        if (k < n_tiles) {
-            cset[k] = c.set;
-            cdelta[k] = c.d;
-        }
-        run = seg_op(run, tot);
+    // thread t folds the contiguous tiles [lo, hi), one block-wide scan of
+    // the folds, then each thread writes its tiles' carries
+    const uint32_t per = (n_tiles + kBlock - 1) / kBlock;
+    const uint32_t lo = min(n_tiles, threadIdx.x * per), hi = min(n_tiles, lo + per);
+    Seg agg{0u, kNoSet, 0u};
+    for (uint32_t k = lo; k < hi; ++k) agg = seg_op(agg, Seg{tflag[k], tset[k], tdelta[k]});
+    Seg tot;
+    Seg run = block_seg_exclusive(agg, &tot);
+    for (uint32_t k = lo; k < hi; ++k) {
+        cset[k] = run.set;
+        cdelta[k] = run.d;
+        run = seg_op(run, Seg{tflag[k], tset[k], tdelta[k]});
     }
 }
 
@@ -183,32 +183,77 @@ struct KeyDecode {
     }
 };
 
-// Value of a cell after state st: (st has an UPDATE ? payload : cell) + d,
-// 16 bytes at byte b (VEC16) or one word.
-template <int G, bool VEC16>
-__device__ inline void emit_value(const ColDesc &cd, const uint8_t *cell, St st,
-                                  const uint64_t *args, uint8_t *dst, uint32_t span, uint32_t lane,
-                                  bool nontemporal) {
-    const bool has_set = st.set < kRmwOnly;
-    const uint64_t arg = has_set ? args[st.set] : 0;
-    if (VEC16 && cd.slot >= 16) {
-        for (uint32_t b = lane * 16; b < span; b += G * 16) {
-            u32x4 v{0, 0, 0, 0};
-            if (b < cd.slot) {
-                v = has_set ? payload16(arg, b, cd.width) : ld16(cell + b);
-                v = add16(v, st.d, b, cd.width);
+// Pipelined cell moves.  A group of G lanes takes R entries per round and
+// issues every independent source load (the payload seed of the UPDATE an
+// entry sees, or the snapshot cell) before any store, so each lane keeps R
+// HBM requests in flight instead of one.  fetch(k, key, st, dst) gives entry
+// k's cell key, state and destination: a result slot (to_result; span bytes,
+// zero past the slot) or the cell itself (final writes).
+template <int G, bool VEC16, class Fetch>
+__device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
+                                  const uint64_t *args, uint32_t res_span, bool to_result,
+                                  uint32_t g, uint32_t groups) {
+    constexpr int R = 4;
+    const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
+    for (uint32_t base = g; base < n; base += groups * R) {
+        uint32_t key[R];
+        St st[R];
+        uint8_t *dst[R];
+        ColDesc cd[R];
+        const uint8_t *cell[R];
+        uint64_t arg[R];
+        u32x4 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t k = base + r * groups;
+            cell[r] = nullptr;
+            cd[r] = ColDesc{nullptr, 0u, 0u};
+            st[r] = St{kNoSet, 0u};
+            dst[r] = nullptr;
+            if (k < n) {
+                fetch(k, key[r], st[r], dst[r]);
+                uint32_t tab, row, col;
+                if (dec(key[r], tab, row, col)) {
+                    cd[r] = dt.cols[tab * LTPMG_MAX_COLS + col];
+                    cell[r] = cd[r].base + (uint64_t)row * cd[r].slot;
+                    if (!to_result) dst[r] = cd[r].base + (uint64_t)row * cd[r].slot;
+                }
             }
-            if (nontemporal) st16_nt(dst + b, v);
-            else st16(dst + b, v);
         }
-    } else {
-        for (uint32_t b = lane * 4; b < span; b += G * 4) {
-            uint32_t v = 0;
-            if (b < cd.width) {
-                v = has_set ? stream_word(arg, b >> 2) : *reinterpret_cast<const uint32_t *>(cell + b);
-                v += st.d;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool has_set = st[r].set < kRmwOnly;
+            arg[r] = (cell[r] && has_set) ? args[st[r].set] : 0ull;
+            v[r] = u32x4{0u, 0u, 0u, 0u};
+            if (VEC16 && cell[r] && !has_set && cd[r].slot >= 16 && b0 < cd[r].slot)
+                v[r] = ld16(cell[r] + b0);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!cell[r]) continue;
+            const uint32_t span = to_result ? res_span : cd[r].slot;
+            const bool has_set = st[r].set < kRmwOnly;
+            if (VEC16 && cd[r].slot >= 16) {
+                for (uint32_t b = b0; b < span; b += G * 16) {
+                    u32x4 x{0u, 0u, 0u, 0u};
+                    if (b < cd[r].slot) {
+                        x = has_set ? payload16(arg[r], b, cd[r].width) : (b == b0 ? v[r] : ld16(cell[r] + b));
+                        x = add16(x, st[r].d, b, cd[r].width);
+                    }
+                    if (to_result) st16_nt(dst[r] + b, x);
+                    else st16(dst[r] + b, x);
+                }
+            } else {  // 4- and 8-byte slots: one word per lane step
+                for (uint32_t b = lane * 4; b < span; b += G * 4) {
+                    uint32_t x = 0;
+                    if (b < cd[r].width) {
+                        x = has_set ? stream_word(arg[r], b >> 2)
+                                    : *reinterpret_cast<const uint32_t *>(cell[r] + b);
+                        x += st[r].d;
+                    }
+                    *reinterpret_cast<uint32_t *>(dst[r] + b) = x;
+                }
             }
-            *reinterpret_cast<uint32_t *>(dst + b) = v;
         }
     }
 }
@@ -223,31 +268,61 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     __shared__ uint32_t n_list;
     if (threadIdx.x == 0) n_list = 0;
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
-    St run = thread_prefix(src, cset, cdelta, first);  // contains __syncthreads
+    // the thread's 8 sorted elements, read once (two 16-byte loads when in range)
+    uint32_t key[kSegItems], val[kSegItems];
+    if (first + kSegItems <= src.n) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(src.keys + first);
+        const uint4 *v4 = reinterpret_cast<const uint4 *>(src.vals + first);
+        uint4 a = k4[0], b = k4[1], c = v4[0], d = v4[1];
+        key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
+        key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
+        val[0] = c.x; val[1] = c.y; val[2] = c.z; val[3] = c.w;
+        val[4] = d.x; val[5] = d.y; val[6] = d.z; val[7] = d.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kSegItems; ++j) {
+            key[j] = first + j < src.n ? src.keys[first + j] : 0xFFFFFFFFu;
+            val[j] = first + j < src.n ? src.vals[first + j] : kInvalid;
+        }
+    }
+    // keys are < 2^31, so all-ones never equals a real key
+    const uint32_t prev = (first > 0 && first - 1 < src.n) ? src.keys[first - 1] : 0xFFFFFFFFu;
+    const uint32_t next = first + kSegItems < src.n ? src.keys[first + kSegItems] : 0xFFFFFFFFu;
+    Seg agg{0u, kNoSet, 0u};
+#pragma unroll
+    for (uint32_t j = 0; j < kSegItems; ++j) {
+        if (first + j >= src.n) break;
+        St s = elem_state(val[j], src.args);
+        bool h = key[j] != (j ? key[j - 1] : prev);
+        agg = seg_op(agg, Seg{h ? 1u : 0u, s.set, s.d});
+    }
+    Seg tot;
+    Seg pre = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, block_seg_exclusive(agg, &tot));
+    St run{pre.set, pre.d};
+#pragma unroll
     for (uint32_t j = 0; j < kSegItems; ++j) {
         uint32_t i = first + j;
         bool rd = false, wr = false;
-        uint32_t key = 0, op = 0;
+        uint32_t op = 0;
         St ex{kNoSet, 0u};
         if (i < src.n) {
-            key = src.keys[i];
-            uint32_t val = src.vals[i];
-            op = val & kOpMask;
-            St s = elem_state(val, src.args);
-            bool h = i == 0 || src.keys[i - 1] != key;
+            op = val[j] & kOpMask;
+            St s = elem_state(val[j], src.args);
+            bool h = key[j] != (j ? key[j - 1] : prev);
             ex = h ? St{kNoSet, 0u} : run;
             run = h ? s : combine(run, s);
-            rd = (val >> 30) == LTPMG_OP_READ && !(val & kInvalid);
-            bool last = i + 1 == src.n || src.keys[i + 1] != key;
+            rd = (val[j] >> 30) == LTPMG_OP_READ && !(val[j] & kInvalid);
+            bool last = key[j] != (j + 1 < kSegItems ? key[j + 1] : next);
             wr = last && run.set != kNoSet;
         }
+        const uint32_t k_j = key[j];
         // READ -> LDS list (one LDS atomic per wave)
         uint64_t m = __ballot(rd);
         if (m) {
             uint32_t leader = __builtin_ctzll(m), base = 0;
             if (__lane_id() == leader) base = atomicAdd(&n_list, (uint32_t)__popcll(m));
             base = __shfl(base, (int)leader);
-            if (rd) list[base + lanemask_lt_popc(m)] = make_uint4(op, ex.set, ex.d, key);
+            if (rd) list[base + lanemask_lt_popc(m)] = make_uint4(op, ex.set, ex.d, k_j);
         }
         // segment end with a write -> global final-write list
         uint64_t mw = __ballot(wr);
@@ -257,24 +332,19 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
             base = __shfl(base, (int)leader);
             if (wr) {
                 uint32_t p = (base + lanemask_lt_popc(mw)) * 3;
-                fin[p] = key;
+                fin[p] = k_j;
                 fin[p + 1] = run.set;
                 fin[p + 2] = run.d;
             }
         }
     }
     __syncthreads();
-    const uint32_t n = n_list;
-    const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G;
-    for (uint32_t k = g; k < n; k += kBlock / G) {
+    move_cells<G, VEC16>(n_list, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[k];
-        uint32_t tab, row, col;
-        if (!dec(e.w, tab, row, col)) continue;
-        ColDesc cd = dt.cols[tab * LTPMG_MAX_COLS + col];
-        const uint8_t *cell = cd.base + (uint64_t)row * cd.slot;
-        emit_value<G, VEC16>(cd, cell, St{e.y, e.z}, src.args,
-                             results + (uint64_t)e.x * res_stride, res_stride, lane, true);
-    }
+        key_out = e.w;
+        st = St{e.y, e.z};
+        dst = results + (uint64_t)e.x * res_stride;
+    }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G);
 }
 
 template <int G, bool VEC16>
@@ -283,16 +353,15 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint32_t *fin,
                                                             DevTables dt, KeyDecode dec,
                                                             const uint64_t *args) {
     const uint32_t n = *fin_count;
-    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    for (uint32_t k = g; k < n; k += groups) {
-        uint32_t key = fin[3 * k], set = fin[3 * k + 1], d = fin[3 * k + 2];
-        uint32_t tab, row, col;
-        if (!dec(key, tab, row, col)) continue;
-        ColDesc cd = dt.cols[tab * LTPMG_MAX_COLS + col];
-        uint8_t *cell = cd.base + (uint64_t)row * cd.slot;
-        emit_value<G, VEC16>(cd, cell, St{set, d}, args, cell, cd.slot, lane, false);
-    }
+    // each touched cell appears once in the list, so its load and store stay
+    // within one lane group
+    move_cells<G, VEC16>(n, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
+        key = fin[3 * k];
+        st = St{fin[3 * k + 1], fin[3 * k + 2]};
+        dst = nullptr;  // the cell itself, set below
+    }, dt, dec, args, 0u, false, g, groups);
 }
 
 // ---------------- radix sort ----------------
@@ -316,6 +385,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *keys, uint32
     hist[(uint64_t)d * n_tiles + blockIdx.x] = c;
 }
 
+__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total);
+
 // Stable scatter: wave w of tile k ranks its 16 chunks of 64 keys in order;
 // lanes holding the same digit are found with 8 ballots (wave64 multisplit).
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin,
@@ -324,6 +395,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, cons
                                                        uint32_t n_tiles, const uint32_t *offs) {
     __shared__ uint32_t cnt[kBlock / 64][256];
     __shared__ uint32_t gbase[256];
+    __shared__ uint32_t tile_off[256];
+    __shared__ uint32_t sk[kSortTile], sv[kSortTile];
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
     gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
@@ -355,24 +428,39 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, cons
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    {  // exclusive offsets of each wave within the tile, per digit
+    {  // per digit: offsets of each wave inside the digit's run, and the run's
+       // start inside the tile (exclusive scan over digits)
         uint32_t d = threadIdx.x, run = 0;
         for (uint32_t i = 0; i < kBlock / 64; ++i) {
             uint32_t t = cnt[i][d];
             cnt[i][d] = run;
             run += t;
         }
+        uint32_t total;
+        tile_off[d] = block_exclusive_sum(run, &total);
     }
-    __syncthreads();
+    __syncthreads();  // every tile_off[d] written before any lane reads it
+    // stage the tile in LDS in (digit, input) order ...
 #pragma unroll
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
         if (idx < n) {
             uint32_t d = (k[c] >> shift) & 255u;
-            uint32_t pos = gbase[d] + cnt[w][d] + r[c];
-            kout[pos] = k[c];
-            vout[pos] = v[c];
+            uint32_t lp = tile_off[d] + cnt[w][d] + r[c];
+            sk[lp] = k[c];
+            sv[lp] = v[c];
         }
+    }
+    __syncthreads();
+    // ... then write each digit's run contiguously: neighbouring lanes store
+    // neighbouring addresses
+    const uint32_t tile0 = blockIdx.x * kSortTile;
+    const uint32_t tile_n = n - tile0 < kSortTile ? n - tile0 : kSortTile;
+    for (uint32_t j = threadIdx.x; j < tile_n; j += kBlock) {
+        uint32_t key = sk[j], d = (key >> shift) & 255u;
+        uint32_t pos = gbase[d] + (j - tile_off[d]);
+        kout[pos] = key;
+        vout[pos] = sv[j];
     }
 }
 

@@ -133,6 +133,20 @@ WORKLOAD_API = {
     "ltpmg_zeta": (C.c_double, [C.c_uint64, C.c_double]),
 }
 
+class _OeSpec(C.Structure):
+    _fields_ = [("n_txns", C.c_uint32), ("warehouses", C.c_uint32), ("neworder_pct", C.c_uint32),
+                ("payment_pct", C.c_uint32), ("lines_min", C.c_uint32), ("lines_max", C.c_uint32),
+                ("remote_bp", C.c_uint32), ("append_cap", C.c_uint32),
+                ("append_next", C.c_uint32 * 4), ("seed", C.c_uint64)]
+
+
+OE_API = {
+    "ltpmg_oe_schema": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ltpmg_gen_oe_max_ops": (C.c_uint64, [C.POINTER(_OeSpec)]),
+    "ltpmg_gen_oe": (C.c_int, [C.POINTER(_OeSpec), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32]),
+}
+
 SERVE_API = {
     "ltpmg_serve_run": (C.c_int, [C.c_void_p, C.POINTER(_ServeCfg), C.POINTER(_Batch),
                                   C.POINTER(_ServeStats), C.POINTER(_ServeRecord)]),
@@ -163,7 +177,7 @@ def hip_lib() -> C.CDLL:
 
 def host_lib() -> C.CDLL:
     hip_lib()  # the host library links against it
-    return _load("libltpmg_host.so", {**WORKLOAD_API, **SERVE_API})
+    return _load("libltpmg_host.so", {**WORKLOAD_API, **SERVE_API, **OE_API})
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -234,6 +248,44 @@ def gen_kv(n_txns: int, n_rows: int, n_cols: int, *, ops: int | tuple[int, int] 
         raise LtpmgError("ltpmg_gen_kv", rc, "invalid workload spec")
     k = n.value
     return Batch(off, rows[:k].copy(), metas[:k].copy(), args[:k].copy())
+
+
+def oe_schema(warehouses: int, append_cap: int) -> dict[int, tuple[int, list[int]]]:
+    """Order-entry tables (include/ltpmg_oe.h): {table id: (rows, column bytes)}."""
+    descs = (_TableDesc * 9)()
+    rc = host_lib().ltpmg_oe_schema(warehouses, append_cap, C.cast(descs, C.c_void_p))
+    if rc != 0:
+        raise LtpmgError("ltpmg_oe_schema", rc, "bad order-entry sizes")
+    return {t: (d.n_rows, list(d.col_bytes[:d.n_cols])) for t, d in enumerate(descs)}
+
+
+class OeStream:
+    """Successive order-entry batches; append rows continue from batch to batch."""
+
+    def __init__(self, warehouses: int, append_cap: int, *, neworder_pct: int = 45,
+                 payment_pct: int = 43, lines: tuple[int, int] = (5, 15), remote_bp: int = 100,
+                 seed: int = 1):
+        self.spec = _OeSpec(0, warehouses, neworder_pct, payment_pct, lines[0], lines[1], remote_bp,
+                            append_cap, (C.c_uint32 * 4)(0, 0, 0, 0), seed & 0xFFFFFFFFFFFFFFFF)
+        self.batches = 0
+
+    def next(self, n_txns: int, n_threads: int = 0) -> Batch:
+        lib = host_lib()
+        self.spec.n_txns = n_txns
+        self.spec.seed = (self.spec.seed + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        cap = int(lib.ltpmg_gen_oe_max_ops(C.byref(self.spec)))
+        off = np.zeros(n_txns + 1, np.uint32)
+        rows = np.empty(cap, np.uint32)
+        metas = np.empty(cap, np.uint16)
+        args = np.empty(cap, np.uint64)
+        n = C.c_uint32(0)
+        rc = lib.ltpmg_gen_oe(C.byref(self.spec), _ptr(off), _ptr(rows), _ptr(metas), _ptr(args),
+                              cap, C.byref(n), n_threads)
+        if rc != 0:
+            raise LtpmgError("ltpmg_gen_oe", rc, "invalid order-entry spec")
+        self.batches += 1
+        k = n.value
+        return Batch(off, rows[:k].copy(), metas[:k].copy(), args[:k].copy())
 
 
 def zeta(n: int, theta: float) -> float:

@@ -24,6 +24,7 @@ DEFINE_uint32(kv_field_bytes, 100, "KV: bytes per field (KV_C1 uses 4).");
 DEFINE_uint32(kv_ops, 16, "KV / KV_C1: logical ops per txn.");
 DEFINE_uint32(read_percent, 50, "KV / KV_C1: percent of ops that READ.");
 DEFINE_string(dump_path, "", "write epoch 0's batch, results and image digest under this prefix.");
+DEFINE_uint32(append_cap, 1u << 20, "TPCC_PART / OE: capacity of the append-only tables, in transactions.");
 DEFINE_uint32(minloglevel, 0, "drop log lines below this severity (0 INFO .. 3 FATAL).");
 
 namespace ltpmg_log {

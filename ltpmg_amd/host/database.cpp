@@ -10,6 +10,7 @@
 #include "database.h"
 
 #include "ltpmg.h"
+#include "ltpmg_oe.h"
 
 namespace {
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
@@ -57,6 +58,21 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
         tables_for_GPU_info[i].table_cnt = n_tables;
         tables_for_GPU[i] = Global_Table{nullptr, nullptr, nullptr};
         index_for_GPU[i].index = nullptr;
+    }
+    if (param->is_oe()) {
+        // nine order-entry tables with the widths of include/ltpmg_oe.h; the
+        // per-type counts of Global_Table_Info cannot express them, so only
+        // the row counts are recorded and the engine fills the tables
+        ltpmg_table_desc d[LTPMG_OE_TABLES];
+        if (n_tables != LTPMG_OE_TABLES || ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, d))
+            LOG(FATAL) << "order-entry schema: " << param->warehouse_size << " partitions, append capacity "
+                       << param->oe_append_cap;
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            tables_for_GPU_info[i].table_size = d[i].n_rows;
+            tables_for_GPU_info[i].int_size = d[i].n_cols;
+        }
+        LOG(INFO) << "order-entry tables for " << param->warehouse_size << " partitions are filled on the device";
+        return;
     }
     if (!param->is_ycsb() && !param->is_kv())
         LOG(FATAL) << "benchmark " << param->benchmark << ": schema not built in this engine";

@@ -18,6 +18,8 @@
 // no-ops (gpulauncher.cu:8-165), so these value rules are this build's own.
 #include "database.h"
 #include "ltpmg.h"
+#include "ltpmg_oe.h"
+#include "oe.h"
 #include "query.h"
 #include "scheduler.h"
 
@@ -96,6 +98,17 @@ bool pack_ycsb(const std::any &a, Packer &p, int variant) {
     return true;
 }
 
+// order-entry queries: include/ltpmg_oe.h (the parse<Neworder_Query> ... of
+// cuda/src/gpuquery.cu:2473-2927, with this build's keys and columns)
+template <typename Q>
+bool pack_oe(const std::any &a, oe::Sink &sink, KV_Batch &out) {
+    const Q *q = std::any_cast<Q>(&a);
+    if (!q) return false;
+    oe::decompose(*q, sink);
+    out.txn_off.push_back((uint32_t)out.rows.size());
+    return true;
+}
+
 // vector<any> -> SoA batch
 void pack(const std::vector<std::any> &batch, KV_Batch &out) {
     out = KV_Batch{};
@@ -105,11 +118,14 @@ void pack(const std::vector<std::any> &batch, KV_Batch &out) {
     }
     if (g.info.empty()) LOG(FATAL) << "transfer_query_to_GPU before transfer_database_to_GPU";
     Packer p{out, n_cols(g.info[0]), g.seed ^ 0x5943534Full};
+    oe::Sink sink{out.rows, out.meta, out.args, g.seed ^ 0x4F45ull};
     out.txn_off.push_back(0);
     for (const std::any &a : batch) {
         if (pack_ycsb<YCSB_A_Query>(a, p, 'A') || pack_ycsb<YCSB_B_Query>(a, p, 'B') ||
             pack_ycsb<YCSB_C_Query>(a, p, 'C') || pack_ycsb<YCSB_D_Query>(a, p, 'D') ||
-            pack_ycsb<YCSB_E_Query>(a, p, 'E'))
+            pack_ycsb<YCSB_E_Query>(a, p, 'E') || pack_oe<Neworder_Query>(a, sink, out) ||
+            pack_oe<Payment_Query>(a, sink, out) || pack_oe<Orderstatus_Query>(a, sink, out) ||
+            pack_oe<Stocklevel_Query>(a, sink, out))
             continue;
         LOG(FATAL) << "query type " << a.type().name() << " is not handled by this engine";
     }
@@ -140,6 +156,16 @@ void transfer_database_to_GPU(std::shared_ptr<Param> param, Global_Table_Info *i
     ensure_open(param);
     g.seed = param->seed;
     g.info.assign(info, info + param->table_cnt);
+    if (param->is_oe()) {  // include/ltpmg_oe.h: created and filled on the device
+        ltpmg_table_desc d[LTPMG_OE_TABLES];
+        check(ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, d), "ltpmg_oe_schema");
+        for (uint32_t t = 0; t < LTPMG_OE_TABLES; ++t) {
+            check(ltpmg_create_table(g.e, t, &d[t]), "ltpmg_create_table");
+            check(ltpmg_fill_table(g.e, t, param->seed), "ltpmg_fill_table");
+        }
+        LOG(INFO) << "transfer_database_to_GPU: order-entry tables resident in HBM";
+        return;
+    }
     for (uint32_t t = 0; t < param->table_cnt; ++t) {
         ltpmg_table_desc d{};
         d.n_rows = info[t].table_size;

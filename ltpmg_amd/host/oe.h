@@ -1,0 +1,93 @@
+// oe.h -- order-entry query decomposition shared by the generator
+// (order_entry.cpp) and the host-API shim (gpu_shim.cpp).  The table and
+// column layout is documented in include/ltpmg_oe.h.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "ltpmg.h"
+#include "ltpmg_oe.h"
+#include "rng.h"
+#include "txn_structure.h"
+
+namespace oe {
+
+enum Table : uint32_t {
+    WAREHOUSE = 0, DISTRICT = 1, CUSTOMER = 2, NEWORDER = 3, HISTORY = 4,
+    ORDER = 5, ORDERLINE = 6, STOCK = 7, ITEM = 8
+};
+
+// column indices (ints first, then strings, as the schema lists them)
+namespace w { enum : uint32_t { ID, TAX, YTD, NAME }; }
+namespace d { enum : uint32_t { ID, W_ID, NEXT_O_ID, TAX, YTD, NAME }; }
+namespace c { enum : uint32_t { ID, D_ID, W_ID, PAYMENT_CNT, DELIVERY_CNT, DISCOUNT, CREDIT_LIM,
+                                BALANCE, YTD_PAYMENT, FIRST, MIDDLE, LAST, STREET_1, STREET_2, CITY,
+                                STATE, ZIP, PHONE, SINCE, CREDIT, DATA }; }
+namespace no { enum : uint32_t { O_ID, D_ID, W_ID }; }
+namespace h { enum : uint32_t { C_ID, C_D_ID, C_W_ID, D_ID, W_ID, AMOUNT, DATE, DATA }; }
+namespace o { enum : uint32_t { ID, D_ID, W_ID, C_ID, CARRIER_ID, OL_CNT, ALL_LOCAL, ENTRY_D }; }
+namespace ol { enum : uint32_t { O_ID, D_ID, W_ID, NUMBER, I_ID, SUPPLY_W_ID, DELIVERY_D, QUANTITY,
+                                 AMOUNT, DIST_INFO }; }
+namespace s { enum : uint32_t { I_ID, W_ID, QUANTITY, YTD, ORDER_CNT, REMOTE_CNT, DIST_01 }; }
+namespace i { enum : uint32_t { ID, IM_ID, PRICE, NAME, DATA }; }
+
+constexpr uint32_t kDistricts = 10, kCustomers = 3000, kStock = 100000, kItems = 100000;
+constexpr uint32_t kMaxLines = 15;
+
+inline uint32_t district_row(uint32_t w, uint32_t dd) { return w * kDistricts + dd; }
+inline uint32_t customer_row(uint32_t w, uint32_t dd, uint32_t cc) { return district_row(w, dd) * kCustomers + cc; }
+inline uint32_t stock_row(uint32_t w, uint32_t item) { return w * kStock + item; }
+
+// Collects ops of one batch in SoA form.
+struct Sink {
+    std::vector<uint32_t> &rows;
+    std::vector<uint16_t> &meta;
+    std::vector<uint64_t> &args;
+    uint64_t seed;  // payload seeds of inserted cells
+    void put(uint32_t kind, uint32_t table, uint32_t col, uint32_t row, uint64_t arg) {
+        rows.push_back(row);
+        meta.push_back(LTPMG_META(kind, table, col));
+        args.push_back(arg);
+    }
+    void read(uint32_t t, uint32_t col, uint32_t row) { put(LTPMG_OP_READ, t, col, row, 0); }
+    void add(uint32_t t, uint32_t col, uint32_t row, int32_t delta) {
+        put(LTPMG_OP_RMW_ADD, t, col, row, (uint64_t)(uint32_t)delta);
+    }
+    // an inserted cell gets a payload seeded by (seed, table, row, col)
+    void insert(uint32_t t, uint32_t col, uint32_t row) {
+        uint64_t id = ((uint64_t)row << 10) | ((uint64_t)t << 6) | col;
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (id + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        put(LTPMG_OP_UPDATE, t, col, row, z ^ (z >> 31));
+    }
+};
+
+// Append rows are carried in the query's own id fields: Neworder O_ID (order
+// row), N_O_ID (neworder row), O_OL_ID (first orderline row); Payment H_ID
+// (history row); Orderstatus O_ID / OL_ID; Stocklevel O_OL_ID[k].
+void decompose(const Neworder_Query &q, Sink &out);
+void decompose(const Payment_Query &q, Sink &out);
+void decompose(const Orderstatus_Query &q, Sink &out);
+void decompose(const Stocklevel_Query &q, Sink &out);
+
+// Append rows owned by the transaction at position t of a batch: one row of
+// every append table and 15 orderlines, whatever the transaction's kind.
+struct Slots {
+    uint32_t order, neworder, history, orderline;
+};
+Slots slots_for(const uint32_t append_next[4], uint32_t cap, uint64_t t);
+void set_slots(Neworder_Query &q, const Slots &s);
+void set_slots(Payment_Query &q, const Slots &s);
+
+// Query builders drawing from a transaction's own random stream.
+struct Mix {
+    uint32_t W, lines_min, lines_max, remote_bp, cap;
+};
+Neworder_Query make_neworder(ltpmg_host::Rng &r, const Mix &m);
+Payment_Query make_payment(ltpmg_host::Rng &r, const Mix &m);
+Orderstatus_Query make_orderstatus(ltpmg_host::Rng &r, const Mix &m);
+Stocklevel_Query make_stocklevel(ltpmg_host::Rng &r, const Mix &m);
+
+}  // namespace oe

@@ -1,0 +1,261 @@
+// order_entry.cpp -- the order-entry adapter: schema, query generation and
+// query -> op decomposition (include/ltpmg_oe.h).
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oe.h"
+#include "rng.h"
+
+using ltpmg_host::Rng;
+
+namespace oe {
+
+void decompose(const Neworder_Query &q, Sink &out) {
+    const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.W_ID, q.D_ID, q.C_ID);
+    out.read(WAREHOUSE, w::TAX, q.W_ID);
+    out.read(DISTRICT, d::TAX, dr);
+    out.read(DISTRICT, d::NEXT_O_ID, dr);      // the order id this txn takes ...
+    out.add(DISTRICT, d::NEXT_O_ID, dr, 1);    // ... and the increment after it
+    out.read(CUSTOMER, c::DISCOUNT, cr);
+    out.read(CUSTOMER, c::LAST, cr);
+    out.read(CUSTOMER, c::CREDIT, cr);
+    for (uint32_t col : {o::C_ID, o::OL_CNT, o::ALL_LOCAL, o::ENTRY_D}) out.insert(ORDER, col, q.O_ID);
+    for (uint32_t col : {no::O_ID, no::D_ID, no::W_ID}) out.insert(NEWORDER, col, q.N_O_ID);
+    for (uint32_t l = 0; l < q.O_OL_CNT && l < kMaxLines; ++l) {
+        const auto &ln = q.INFO[l];
+        const uint32_t sr = stock_row(ln.OL_SUPPLY_W_ID, ln.OL_I_ID);
+        out.read(ITEM, i::PRICE, ln.OL_I_ID);
+        out.read(ITEM, i::NAME, ln.OL_I_ID);
+        out.add(STOCK, s::QUANTITY, sr, -(int32_t)ln.OL_QUANTITY);
+        out.add(STOCK, s::YTD, sr, (int32_t)ln.OL_QUANTITY);
+        out.add(STOCK, s::ORDER_CNT, sr, 1);
+        if (ln.OL_SUPPLY_W_ID != q.W_ID) out.add(STOCK, s::REMOTE_CNT, sr, 1);
+        for (uint32_t col : {ol::I_ID, ol::SUPPLY_W_ID, ol::QUANTITY, ol::AMOUNT, ol::DIST_INFO})
+            out.insert(ORDERLINE, col, q.O_OL_ID + l);
+    }
+}
+
+void decompose(const Payment_Query &q, Sink &out) {
+    const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.C_W_ID, q.C_D_ID, q.C_ID);
+    out.add(WAREHOUSE, w::YTD, q.W_ID, (int32_t)q.H_AMOUNT);
+    out.read(WAREHOUSE, w::NAME, q.W_ID);
+    out.add(DISTRICT, d::YTD, dr, (int32_t)q.H_AMOUNT);
+    out.read(DISTRICT, d::NAME, dr);
+    out.add(CUSTOMER, c::BALANCE, cr, -(int32_t)q.H_AMOUNT);
+    out.add(CUSTOMER, c::YTD_PAYMENT, cr, (int32_t)q.H_AMOUNT);
+    out.add(CUSTOMER, c::PAYMENT_CNT, cr, 1);
+    out.read(CUSTOMER, c::FIRST, cr);
+    out.read(CUSTOMER, c::LAST, cr);
+    out.read(CUSTOMER, c::DATA, cr);
+    for (uint32_t col : {h::C_ID, h::D_ID, h::W_ID, h::AMOUNT, h::DATA}) out.insert(HISTORY, col, q.H_ID);
+}
+
+void decompose(const Orderstatus_Query &q, Sink &out) {
+    const uint32_t cr = customer_row(q.W_ID, q.D_ID, q.C_ID);
+    out.read(CUSTOMER, c::BALANCE, cr);
+    out.read(CUSTOMER, c::FIRST, cr);
+    out.read(CUSTOMER, c::LAST, cr);
+    out.read(ORDER, o::ID, q.O_ID);
+    out.read(ORDER, o::CARRIER_ID, q.O_ID);
+    out.read(ORDER, o::ENTRY_D, q.O_ID);
+    for (uint32_t l = 0; l < 5; ++l) {
+        out.read(ORDERLINE, ol::I_ID, q.OL_ID + l);
+        out.read(ORDERLINE, ol::QUANTITY, q.OL_ID + l);
+        out.read(ORDERLINE, ol::AMOUNT, q.OL_ID + l);
+    }
+}
+
+void decompose(const Stocklevel_Query &q, Sink &out) {
+    out.read(DISTRICT, d::NEXT_O_ID, district_row(q.W_ID, q.D_ID));
+    for (uint32_t k = 0; k < 10; ++k) {
+        out.read(ORDERLINE, ol::I_ID, q.O_OL_ID[k]);
+        out.read(STOCK, s::QUANTITY, stock_row(q.W_ID, q.I_ID[k]));
+    }
+}
+
+Slots slots_for(const uint32_t next[4], uint32_t cap, uint64_t t) {
+    return Slots{(uint32_t)((next[2] + t) % cap), (uint32_t)((next[0] + t) % cap),
+                 (uint32_t)((next[1] + t) % cap),
+                 (uint32_t)((next[3] + t * kMaxLines) % ((uint64_t)cap * kMaxLines))};
+}
+
+void set_slots(Neworder_Query &q, const Slots &s) {
+    q.O_ID = s.order;
+    q.N_O_ID = s.neworder;
+    q.O_OL_ID = s.orderline;
+}
+
+void set_slots(Payment_Query &q, const Slots &s) { q.H_ID = s.history; }
+
+Neworder_Query make_neworder(Rng &r, const Mix &m) {
+    Neworder_Query q{};
+    q.W_ID = (uint32_t)r.below(m.W);
+    q.D_ID = (uint32_t)r.below(kDistricts);
+    q.C_ID = (uint32_t)r.nurand(1023, 0, kCustomers - 1);
+    q.O_OL_CNT = (uint32_t)r.range(m.lines_min, m.lines_max);
+    for (uint32_t l = 0; l < q.O_OL_CNT; ++l) {
+        uint32_t item, sw;
+        bool dup;
+        do {  // a stock cell is written once per txn
+            item = (uint32_t)r.nurand(8191, 0, kItems - 1);
+            sw = (m.W > 1 && r.below(10000) < m.remote_bp) ? (uint32_t)((q.W_ID + 1 + r.below(m.W - 1)) % m.W)
+                                                         : q.W_ID;
+            dup = false;
+            for (uint32_t p = 0; p < l; ++p) dup |= q.INFO[p].OL_I_ID == item && q.INFO[p].OL_SUPPLY_W_ID == sw;
+        } while (dup);
+        q.INFO[l].OL_I_ID = item;
+        q.INFO[l].OL_SUPPLY_W_ID = sw;
+        q.INFO[l].OL_QUANTITY = (uint32_t)r.range(1, 10);
+    }
+    return q;
+}
+
+Payment_Query make_payment(Rng &r, const Mix &m) {
+    Payment_Query q{};
+    q.W_ID = (uint32_t)r.below(m.W);
+    q.D_ID = (uint32_t)r.below(kDistricts);
+    q.C_ID = (uint32_t)r.nurand(1023, 0, kCustomers - 1);
+    q.C_W_ID = q.W_ID;
+    q.C_D_ID = q.D_ID;
+    q.H_AMOUNT = (uint32_t)r.range(100, 500000);
+    return q;
+}
+
+Orderstatus_Query make_orderstatus(Rng &r, const Mix &m) {
+    Orderstatus_Query q{};
+    q.W_ID = (uint32_t)r.below(m.W);
+    q.D_ID = (uint32_t)r.below(kDistricts);
+    q.C_ID = (uint32_t)r.nurand(1023, 0, kCustomers - 1);
+    q.O_ID = (uint32_t)r.below(m.cap);
+    q.OL_ID = (uint32_t)r.below((uint64_t)m.cap * kMaxLines - 5);
+    return q;
+}
+
+Stocklevel_Query make_stocklevel(Rng &r, const Mix &m) {
+    Stocklevel_Query q{};
+    q.W_ID = (uint32_t)r.below(m.W);
+    q.D_ID = (uint32_t)r.below(kDistricts);
+    for (uint32_t k = 0; k < 10; ++k) {
+        q.I_ID[k] = (uint32_t)r.nurand(8191, 0, kItems - 1);
+        q.O_OL_ID[k] = (uint32_t)r.below((uint64_t)m.cap * kMaxLines);
+    }
+    return q;
+}
+
+}  // namespace oe
+
+namespace {
+
+using namespace oe;
+
+void table_desc(ltpmg_table_desc &t, uint32_t rows, std::initializer_list<uint32_t> widths) {
+    std::memset(&t, 0, sizeof t);
+    t.n_rows = rows;
+    for (uint32_t wd : widths) t.col_bytes[t.n_cols++] = wd;
+}
+
+}  // namespace
+
+extern "C" int ltpmg_oe_schema(uint32_t W, uint32_t cap, ltpmg_table_desc out[LTPMG_OE_TABLES]) {
+    if (!out || W == 0 || cap == 0 || (uint64_t)W * kStock > 0xFFFFFFFFull ||
+        (uint64_t)cap * kMaxLines > 0xFFFFFFFFull)
+        return LTPMG_EINVAL;
+    const uint32_t S = 32;
+    table_desc(out[WAREHOUSE], W, {4, 4, 4, S, S, S, S, S, S});
+    table_desc(out[DISTRICT], W * kDistricts, {4, 4, 4, 4, 4, S, S, S, S, S, S});
+    table_desc(out[CUSTOMER], W * kDistricts * kCustomers,
+               {4, 4, 4, 4, 4, 4, 4, 4, 4, S, S, S, S, S, S, S, S, S, S, S, 256});
+    table_desc(out[NEWORDER], cap, {4, 4, 4});
+    table_desc(out[HISTORY], cap, {4, 4, 4, 4, 4, 4, S, S});
+    table_desc(out[ORDER], cap, {4, 4, 4, 4, 4, 4, 4, S});
+    table_desc(out[ORDERLINE], cap * kMaxLines, {4, 4, 4, 4, 4, 4, 4, 4, 4, 24});
+    table_desc(out[STOCK], W * kStock, {4, 4, 4, 4, 4, 4, 24, 24, 24, 24, 24, 24, 24, 24, 24, 24, S});
+    table_desc(out[ITEM], kItems, {4, 4, 4, 24, 52});
+    return LTPMG_OK;
+}
+
+extern "C" uint64_t ltpmg_gen_oe_max_ops(const ltpmg_oe_spec *sp) {
+    // a Neworder of 15 remote lines is the longest: 14 + 15 * 11 ops
+    return sp ? (uint64_t)sp->n_txns * (14 + kMaxLines * 11) : 0;
+}
+
+extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows, uint16_t *meta,
+                            uint64_t *args, uint64_t max_ops, uint32_t *n_ops_out, uint32_t n_threads) {
+    if (!sp || !txn_off || !n_ops_out || sp->warehouses == 0 || sp->append_cap == 0 ||
+        sp->neworder_pct + sp->payment_pct > 100 || sp->lines_min < 1 || sp->lines_max > kMaxLines ||
+        sp->lines_min > sp->lines_max || sp->remote_bp > 10000)
+        return LTPMG_EINVAL;
+    if (n_threads == 0) n_threads = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t W = sp->warehouses, cap = sp->append_cap;
+    const uint32_t n_chunks = std::min<uint32_t>(n_threads, std::max<uint32_t>(1, sp->n_txns));
+    struct Chunk {
+        std::vector<uint32_t> len, rows;
+        std::vector<uint16_t> meta;
+        std::vector<uint64_t> args;
+    };
+    std::vector<Chunk> chunks(n_chunks);
+    const ltpmg_oe_spec spec = *sp;
+    auto gen = [&](uint32_t ci) {
+        Chunk &ch = chunks[ci];
+        Sink sink{ch.rows, ch.meta, ch.args, spec.seed ^ 0x4F45ull};
+        const uint64_t t0 = (uint64_t)spec.n_txns * ci / n_chunks, t1 = (uint64_t)spec.n_txns * (ci + 1) / n_chunks;
+        const Mix mix{W, spec.lines_min, spec.lines_max, spec.remote_bp, cap};
+        for (uint64_t t = t0; t < t1; ++t) {
+            Rng r(spec.seed, t);
+            const size_t before = ch.rows.size();
+            // each txn owns one row of every append table (and 15 orderlines),
+            // whichever kind it is, so rows never depend on other txns
+            const Slots sl = slots_for(spec.append_next, cap, t);
+            const uint32_t kind = (uint32_t)r.below(100);
+            if (kind < spec.neworder_pct) {
+                Neworder_Query q = make_neworder(r, mix);
+                q.TID = (UINT32)t;
+                set_slots(q, sl);
+                decompose(q, sink);
+            } else if (kind < spec.neworder_pct + spec.payment_pct) {
+                Payment_Query q = make_payment(r, mix);
+                q.TID = (UINT32)t;
+                set_slots(q, sl);
+                decompose(q, sink);
+            } else if (r.below(2) == 0) {
+                Orderstatus_Query q = make_orderstatus(r, mix);
+                q.TID = (UINT32)t;
+                decompose(q, sink);
+            } else {
+                Stocklevel_Query q = make_stocklevel(r, mix);
+                q.TID = (UINT32)t;
+                decompose(q, sink);
+            }
+            ch.len.push_back((uint32_t)(ch.rows.size() - before));
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t c = 0; c < n_chunks; ++c) th.emplace_back(gen, c);
+    for (auto &t : th) t.join();
+
+    uint64_t total = 0;
+    for (auto &ch : chunks) total += ch.rows.size();
+    if (total > max_ops || total >= (1ull << 29)) return LTPMG_EINVAL;
+    uint64_t op = 0, t = 0;
+    txn_off[0] = 0;
+    for (auto &ch : chunks) {
+        if (!ch.rows.empty()) {
+            if (!rows || !meta || !args) return LTPMG_EINVAL;
+            std::memcpy(rows + op, ch.rows.data(), ch.rows.size() * 4);
+            std::memcpy(meta + op, ch.meta.data(), ch.meta.size() * 2);
+            std::memcpy(args + op, ch.args.data(), ch.args.size() * 8);
+        }
+        for (uint32_t l : ch.len) {
+            op += l;
+            txn_off[++t] = (uint32_t)op;
+        }
+    }
+    *n_ops_out = (uint32_t)total;
+    sp->append_next[0] = (uint32_t)((sp->append_next[0] + (uint64_t)sp->n_txns) % cap);
+    sp->append_next[1] = (uint32_t)((sp->append_next[1] + (uint64_t)sp->n_txns) % cap);
+    sp->append_next[2] = (uint32_t)((sp->append_next[2] + (uint64_t)sp->n_txns) % cap);
+    sp->append_next[3] = (uint32_t)((sp->append_next[3] + (uint64_t)sp->n_txns * kMaxLines) % ((uint64_t)cap * kMaxLines));
+    return LTPMG_OK;
+}

@@ -33,14 +33,20 @@ Param::Param()
         test_2_size = 24;
         table_cnt = 2;
         bitmap_table_cnt = 2;
-    } else if (benchmark == "TPCC_ALL" || benchmark == "TPCC_PART") {
+    } else if (is_oe()) {
         filepath = "../data/TPCC.json";
+        oe_append_cap = FLAGS_append_cap;
         neworder_query_subtxn_cnt = 5 + 15 * 3;
         payment_query_subtxn_cnt = 7;
         orderstatus_query_subtxn_cnt = 3;
         delivery_query_subtxn_cnt = 5 * 10;
         stocklevel_query_subtxn_cnt = 1 + 2 * 10;
-        if (benchmark == "TPCC_PART") {
+        if (benchmark == "OE") {
+            neworder_query_batch_size = batch_size * 45 / 100;
+            payment_query_batch_size = batch_size * 43 / 100;
+            orderstatus_query_batch_size = stocklevel_query_batch_size =
+                (batch_size - neworder_query_batch_size - payment_query_batch_size) / 2;
+        } else if (benchmark == "TPCC_PART") {
             neworder_query_batch_size = batch_size * neworder_percent / 100;
             payment_query_batch_size = batch_size - neworder_query_batch_size;
             tpcc_part_sub_txn_size = neworder_query_subtxn_cnt * neworder_query_batch_size +

@@ -4,6 +4,9 @@
 // queries here are already i.i.d. draws, so no shuffle is needed).
 #include "query.h"
 
+#include "oe.h"
+#include "rng.h"
+
 namespace {
 // The batch is handed to the extern functions without a copy; the reference
 // copies the whole vector on every call (src/query.cpp:503-519).
@@ -35,13 +38,33 @@ void Query::generate_txn(std::shared_ptr<Param> param) {
     else if (b == "YCSB_C") append(txn_generator_ycsb_c_query, param, param->ycsb_c_query_batch_size, transactions_batch);
     else if (b == "YCSB_D") append(txn_generator_ycsb_d_query, param, param->ycsb_d_query_batch_size, transactions_batch);
     else if (b == "YCSB_E") append(txn_generator_ycsb_e_query, param, param->ycsb_e_query_batch_size, transactions_batch);
-    else if (param->is_kv()) {
+    else if (param->is_kv() || b == "OE") {
         Txn_Generator<KV_Batch> gen;
         gen.initialize(param);
         gen.generate_txn(param, param->batch_size);
         transactions_batch.emplace_back(std::move(gen.get_txn().front()));
+    } else if (b == "TPCC_PART") {
+        append(txn_generator_neworder_query, param, param->neworder_query_batch_size, transactions_batch);
+        append(txn_generator_payment_query, param, param->payment_query_batch_size, transactions_batch);
+        // interleave the kinds with a seeded Fisher-Yates shuffle (the
+        // reference sorts by type-name pointer, src/query.cpp:115-116), then
+        // TID = position, and each TID owns its append rows
+        ltpmg_host::Rng r(param->seed, 0x5348554646ull);
+        for (size_t i = transactions_batch.size(); i > 1; --i)
+            std::swap(transactions_batch[i - 1], transactions_batch[r.below(i)]);
+        const uint32_t start[4] = {0, 0, 0, 0};
+        for (size_t t = 0; t < transactions_batch.size(); ++t) {
+            const oe::Slots sl = oe::slots_for(start, param->oe_append_cap, t);
+            if (auto *q = std::any_cast<Neworder_Query>(&transactions_batch[t])) {
+                q->TID = (UINT32)t;
+                oe::set_slots(*q, sl);
+            } else if (auto *p = std::any_cast<Payment_Query>(&transactions_batch[t])) {
+                p->TID = (UINT32)t;
+                oe::set_slots(*p, sl);
+            }
+        }
     } else {
-        LOG(FATAL) << "benchmark " << b << ": the order-entry and TEST adapters are not built in this engine";
+        LOG(FATAL) << "benchmark " << b << ": the Delivery and TEST queries are not built in this engine";
     }
 }
 

@@ -5,6 +5,7 @@
 #include "scheduler.h"
 
 #include "ltpmg.h"
+#include "ltpmg_oe.h"
 
 namespace {
 
@@ -31,13 +32,19 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info) {
     j << "{\"benchmark\": \"" << param->benchmark << "\", \"seed\": " << param->seed
       << ", \"mode\": " << param->cc_mode << ", \"result_stride\": " << r.result_stride
       << ", \"committed\": " << param->result.committed << ", \"tables\": [";
+    ltpmg_table_desc oe_desc[LTPMG_OE_TABLES];
+    if (param->is_oe()) ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, oe_desc);
     for (uint32_t t = 0; t < param->table_cnt; ++t) {
         const Global_Table_Info &ti = info[t];
         j << (t ? ", " : "") << "{\"rows\": " << ti.table_size << ", \"col_bytes\": [";
         uint32_t k = 0;
-        for (uint32_t c = 0; c < ti.int_size; ++c) j << (k++ ? ", " : "") << 4;
-        for (uint32_t c = 0; c < ti.double_size; ++c) j << (k++ ? ", " : "") << 8;
-        for (uint32_t c = 0; c < ti.string_size; ++c) j << (k++ ? ", " : "") << 4 * ti.string_length;
+        if (param->is_oe()) {
+            for (uint32_t c = 0; c < oe_desc[t].n_cols; ++c) j << (k++ ? ", " : "") << oe_desc[t].col_bytes[c];
+        } else {
+            for (uint32_t c = 0; c < ti.int_size; ++c) j << (k++ ? ", " : "") << 4;
+            for (uint32_t c = 0; c < ti.double_size; ++c) j << (k++ ? ", " : "") << 8;
+            for (uint32_t c = 0; c < ti.string_size; ++c) j << (k++ ? ", " : "") << 4 * ti.string_length;
+        }
         j << "], \"digest\": " << table_digest_on_GPU(param, t) << "}";
     }
     j << "]}\n";

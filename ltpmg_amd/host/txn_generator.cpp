@@ -4,13 +4,16 @@
 // Zipf(zipf_config) over the YCSB table (src/txn_generator.cpp:602-993 draws
 // the same shape with rand()).  Query t uses its own Random stream, started
 // from (seed, type, t), so the batch does not depend on the thread count.
-// KV_Batch: the whole batch in SoA form from ltpmg_gen_kv (ltpmg_workload.h).
-// The order-entry (TPC-C) and TEST generators are declared in the header but
-// not built: that adapter is the next component of this engine.
+// KV_Batch: the whole batch in SoA form from ltpmg_gen_kv (ltpmg_workload.h)
+// or, for the OE benchmark, ltpmg_gen_oe (ltpmg_oe.h).  Neworder, Payment,
+// Orderstatus and Stocklevel queries come from the order-entry builders
+// (oe.h).  The Delivery and TEST generators are declared but not built.
 #include "txn_generator.h"
 
 #include "ltpmg.h"
+#include "ltpmg_oe.h"
 #include "ltpmg_workload.h"
+#include "oe.h"
 
 namespace {
 
@@ -96,8 +99,75 @@ void Txn_Generator<YCSB_E_Query>::generate_txn(std::shared_ptr<Param> param, siz
     LOG(INFO) << "YCSB_E_Query transactions.size()=" << transactions.size();
 }
 
+// Order-entry queries (the reference's src/txn_generator.cpp:231-599 draws
+// them with rand(); here each query has its own seeded stream).  Append rows
+// (O_ID, N_O_ID, O_OL_ID, H_ID) are set by Query::generate_txn once TIDs are
+// known.
+namespace {
+template <typename Q, typename Make>
+void gen_oe_queries(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, uint64_t tag,
+                    Make make) {
+    const oe::Mix mix{param->warehouse_size, 5, 15, 100, param->oe_append_cap};
+    out.resize(size);
+    for (size_t i = 0; i < size; ++i) {
+        ltpmg_host::Rng r(param->seed ^ (tag << 48), i);
+        out[i] = make(r, mix);
+        out[i].TID = (UINT32)i;
+    }
+}
+}  // namespace
+
+template <>
+void Txn_Generator<Neworder_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    gen_oe_queries(param, size, transactions, 11, oe::make_neworder);
+    LOG(INFO) << "Neworder_Query transactions.size()=" << transactions.size();
+}
+template <>
+void Txn_Generator<Payment_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    gen_oe_queries(param, size, transactions, 12, oe::make_payment);
+    LOG(INFO) << "Payment_Query transactions.size()=" << transactions.size();
+}
+template <>
+void Txn_Generator<Orderstatus_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    gen_oe_queries(param, size, transactions, 13, oe::make_orderstatus);
+    LOG(INFO) << "Orderstatus_Query transactions.size()=" << transactions.size();
+}
+template <>
+void Txn_Generator<Stocklevel_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    gen_oe_queries(param, size, transactions, 14, oe::make_stocklevel);
+    LOG(INFO) << "Stocklevel_Query transactions.size()=" << transactions.size();
+}
+
 template <>
 void Txn_Generator<KV_Batch>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    if (param->benchmark == "OE") {  // the order-entry mix, straight to SoA
+        ltpmg_oe_spec sp{};
+        sp.n_txns = (uint32_t)size;
+        sp.warehouses = param->warehouse_size;
+        sp.neworder_pct = 45;
+        sp.payment_pct = 43;
+        sp.lines_min = 5;
+        sp.lines_max = 15;
+        sp.remote_bp = 100;
+        sp.append_cap = param->oe_append_cap;
+        sp.seed = param->seed ^ 0x4F45ull;
+        KV_Batch b;
+        uint64_t cap = ltpmg_gen_oe_max_ops(&sp);
+        b.txn_off.resize(size + 1);
+        b.rows.resize(cap);
+        b.meta.resize(cap);
+        b.args.resize(cap);
+        uint32_t n = 0;
+        if (ltpmg_gen_oe(&sp, b.txn_off.data(), b.rows.data(), b.meta.data(), b.args.data(), cap, &n, 0) != 0)
+            LOG(FATAL) << "ltpmg_gen_oe rejected the order-entry spec";
+        b.rows.resize(n);
+        b.meta.resize(n);
+        b.args.resize(n);
+        transactions.clear();
+        transactions.push_back(std::move(b));
+        LOG(INFO) << "order-entry batch: " << size << " txns, " << n << " ops";
+        return;
+    }
     const bool c1 = param->benchmark == "KV_C1";
     ltpmg_kv_spec sp{};
     sp.n_txns = (uint32_t)size;

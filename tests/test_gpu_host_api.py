@@ -68,6 +68,15 @@ def test_driver_ycsb(tmp_path, bench):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bench,mode", [("TPCC_PART", "A"), ("TPCC_PART", "O"), ("OE", "O")])
+def test_driver_order_entry(tmp_path, bench, mode):
+    meta = run_driver(tmp_path, f"--benchmark={bench}", "--table_size=2", "--append_cap=20000",
+                      "--batch_size=4096", "--epoch_tp=1", f"--cc_mode={mode}")
+    assert len(meta["tables"]) == 9
+    check_against_oracle(meta)
+
+
+@pytest.mark.gpu
 def test_driver_kv_zipf_ordered(tmp_path):
     meta = run_driver(tmp_path, "--benchmark=KV", "--kv_rows=200000", "--kv_fields=10",
                       "--kv_field_bytes=100", "--kv_ops=16", "--batch_size=16384", "--epoch_tp=3",
