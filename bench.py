@@ -33,6 +33,7 @@ METRIC = "committed txn/s + p50/p99 commit latency, served vs plain batch, 1 MI3
 HBM_PEAK_GBS = 8000.0
 TABLE_SEED = 0x4C54504D47000001
 BATCH_SEED = 0x4C54504D47000002
+SEG_TILE = 2048  # sorted elements per tile of the ordered-mode apply kernel
 
 
 def log(msg: str) -> None:
@@ -56,9 +57,17 @@ def batch_counts(batch, width: int) -> dict:
     reads = sk == 0
     snapshot_reads = int(np.count_nonzero(reads & (cm == 0)))
     written_cells = int(np.unique(sc[sk != 0]).size)
+    # cells whose final write waits for the deferred pass: the segment spans a
+    # 2048-element tile boundary of the sorted list and holds a write
+    seg_has_write = np.zeros(int(seg[-1]) + 1 if n else 0, bool)
+    seg_has_write[seg[sk != 0]] = True
+    b = np.arange(SEG_TILE, n, SEG_TILE)
+    crossing = np.unique(seg[b][sc[b] == sc[b - 1]]) if n else np.zeros(0, np.int64)
+    deferred = int(np.count_nonzero(seg_has_write[crossing]))
     return {"n_ops": n, "n_txns": batch.n_txns, "reads": int(reads.sum()),
             "writes": int((sk != 0).sum()), "snapshot_reads": snapshot_reads,
-            "written_cells": written_cells, "width": width}
+            "written_cells": written_cells, "deferred_writes": deferred,
+            "seg_tiles": (n + SEG_TILE - 1) // SEG_TILE, "width": width}
 
 
 def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | None:
@@ -70,11 +79,13 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
         "rs_scatter": 16 * n,                  # key+val in and out
         "mo_seg_scan": 8 * n,                  # sorted key+val in
         # sorted key+val; per READ its 100-B result, plus the snapshot cell or
-        # the 8-B payload seed of the in-batch UPDATE it sees; 12-B final-write entry per cell
+        # the 8-B payload seed of the in-batch UPDATE it sees; per cell whose
+        # segment ends in its tile, payload seed 8 + the cell write
         "mo_apply_reads": 8 * n + c["reads"] * w + c["snapshot_reads"] * w
-                          + 8 * (c["reads"] - c["snapshot_reads"]) + 12 * c["written_cells"],
-        # per written cell: entry 12 + payload seed 8 + cell write
-        "mo_final_writes": c["written_cells"] * (12 + 8 + w),
+                          + 8 * (c["reads"] - c["snapshot_reads"])
+                          + (c["written_cells"] - c["deferred_writes"]) * (8 + w),
+        # one 16-B entry per tile; per deferred cell payload seed 8 + cell write
+        "mo_final_writes": 16 * c["seg_tiles"] + c["deferred_writes"] * (8 + w),
     }
     if commit is not None and batch is not None:
         kind = batch.meta & 3
@@ -94,6 +105,24 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
             "cc_write": 6 * n + 5 * t + cw * (8 + w),
         })
     return per.get(kernel)
+
+
+def pmc_traffic(workload_key: str) -> dict:
+    """HBM bytes per launch by kernel, from the committed counter summaries
+    under profiles/ (lines "- `kernel` traffic per launch: N B [workload]",
+    written from rocprofv3 FETCH_SIZE + WRITE_SIZE runs scaled by the copy
+    kernel's calibration).  Empty when no summary matches this workload."""
+    import glob
+    import re
+    out = {}
+    pat = re.compile(r"^- `(\w+)` traffic per launch: (\d+) B \[(.+)\]")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(3) == workload_key:
+                    out[m.group(1)] = int(m.group(2))
+    return out
 
 
 def run_mode(e, batch, mode: int, steps: int, warmup: int) -> dict:
@@ -151,6 +180,81 @@ def serve_compare(e, pool, args) -> dict:
     return out
 
 
+def batching_compare(args) -> dict:
+    """The batching form against the plain per-batch path on one stream of
+    uneven batches, all resident in HBM before timing:
+      sizes    floor(exp(U(ln 1e3, ln 1e6))) txns per batch (fixed seed)
+      batches  alternately the key-value table (ops per txn U{1..64}, zipf
+               0.5 and 0.99 in turn, 50% READ / 50% UPDATE) and the W=64
+               order-entry schema (45% NewOrder, 43% Payment, 12% read-only)
+      plain    every batch alone through the three-kernel / ordered
+               pipeline, host wait after each batch
+      batching the plan ltpmg_stream_tune picks (merged ordered groups,
+               persistent fused lowest-TID-wins kernel, one captured graph)
+    Both modes are run over the whole stream.  Value: committed txns of one
+    pass over the stream / device time of that pass."""
+    from ltpmg_amd import MODE_A, MODE_O, Engine, Stream, StreamPlan
+    from ltpmg_amd.engine import OeStream, oe_schema
+    rng = np.random.default_rng(BATCH_SEED + 4)
+    nb = args.batching_batches
+    sizes = np.floor(np.exp(rng.uniform(np.log(args.batching_min_txns),
+                                        np.log(args.batching_max_txns), nb))).astype(np.int64)
+    W, cap, kv_tab, n_cols, width = 64, int(sizes.max()), 9, 10, 100
+    e = Engine()
+    schema = dict(oe_schema(W, cap))
+    schema[kv_tab] = (args.rows, [width] * n_cols)
+    for t, (n, widths) in sorted(schema.items()):
+        e.create_table(t, n, widths)
+        e.fill_table(t, TABLE_SEED)
+    from ltpmg_amd import OP_UPDATE, gen_kv
+    oe = OeStream(W, cap, seed=BATCH_SEED + 5)
+    batches, kinds = [], []
+    t0 = time.perf_counter()
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            theta = 0.5 if i % 4 == 0 else 0.99
+            batches.append(gen_kv(int(n), args.rows, n_cols, ops=(1, 64), zipf_theta=theta,
+                                  read_pct=50, write_kind=OP_UPDATE, table_id=kv_tab,
+                                  seed=BATCH_SEED + 100 + i))
+            kinds.append(f"kv{theta}")
+        else:
+            batches.append(oe.next(int(n)))
+            kinds.append("oe")
+    txns = int(sizes.sum())
+    ops = int(sum(b.n_ops for b in batches))
+    log(f"batching stream: {nb} batches, {txns} txns, {ops} ops, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = {"batches": nb, "txns": txns, "ops": ops,
+           "batch_txns_min": int(sizes.min()), "batch_txns_max": int(sizes.max()),
+           "order_entry_partitions": W, "kv_rows": args.rows, "modes": {}}
+    for name, mode in (("ordered", MODE_O), ("lowest_tid_wins", MODE_A)):
+        with Stream(e, batches, mode) as st:
+            plain_ms = [st.run(StreamPlan.plain()) for _ in range(2)]
+            committed = int(st.committed().sum())
+            plan, trials = st.tune()
+            tuned = []
+            for _ in range(args.batching_runs):
+                w0 = time.perf_counter()
+                ms = st.run()
+                tuned.append((ms, time.perf_counter() - w0))
+            assert int(st.committed().sum()) == committed
+            n_groups = len(st.groups())
+        p_ms = min(plain_ms)
+        b_ms = statistics.median(t[0] for t in tuned)
+        row = {"committed": committed,
+               "plain": {"device_ms": p_ms, "committed_txn_per_s": committed / (p_ms * 1e-3)},
+               "batching": {"device_ms": b_ms, "committed_txn_per_s": committed / (b_ms * 1e-3),
+                            "wall_ms": 1e3 * statistics.median(t[1] for t in tuned),
+                            "launch_groups": n_groups, "tuning_runs": trials,
+                            "plan": plan.__dict__},
+               "speedup": p_ms / b_ms}
+        out["modes"][name] = row
+        log(f"batching {name}: plain {p_ms:.1f} ms, tuned {b_ms:.1f} ms "
+            f"({row['speedup']:.2f}x, {n_groups} groups, plan {plan})")
+    e.close()
+    return out
+
+
 def pct(xs, q):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
@@ -198,6 +302,15 @@ def main() -> int:
     ap.add_argument("--plain-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-wait-us", type=float, default=2000.0)
+    ap.add_argument("--no-batching", action="store_true",
+                    help="skip the batching-form stream comparison")
+    ap.add_argument("--batching-batches", type=int, default=200)
+    ap.add_argument("--batching-min-txns", type=int, default=1000)
+    ap.add_argument("--batching-max-txns", type=int, default=1_000_000)
+    ap.add_argument("--batching-runs", type=int, default=3)
+    ap.add_argument("--copy-selftest", type=int, default=0, metavar="BYTES",
+                    help="only time the plain streaming-copy kernel over BYTES and exit "
+                         "(the counter calibration run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the batched transaction path on a "
@@ -205,6 +318,13 @@ def main() -> int:
         return 2
 
     from ltpmg_amd import MODE_A, MODE_O, OP_UPDATE, Engine, gen_kv
+
+    if args.copy_selftest:
+        e = Engine(n_slots=1)
+        gbs = e.copy_bandwidth(args.copy_selftest, 5)
+        print(json.dumps({"copy_bytes": args.copy_selftest, "iters": 6, "copy_gbs": gbs}), flush=True)
+        e.close()
+        return 0
 
     n_cols, width = 10, 100
     log(f"generating batch: {args.txns} txns x {args.ops} ops, zipf {args.theta}, {args.rows} rows")
@@ -244,9 +364,15 @@ def main() -> int:
     dom = max(avg, key=lambda k: avg[k])
     alg = algorithmic_bytes(dom, counts)
     achieved = alg / (avg[dom] * 1e-3) / 1e9 if alg else None
+    copy_gbs = e.copy_bandwidth(1 << 30, 10)
+    wkey = f"kv {args.rows}x{n_cols}x{width}B, {args.txns}x{args.ops} ops, zipf {args.theta}"
+    traffic = pmc_traffic(wkey)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                "traffic": None, "avg_launch_ms": avg[dom], "algorithmic_bytes": alg}
+                "traffic": traffic.get(dom), "avg_launch_ms": avg[dom], "algorithmic_bytes": alg,
+                "measured_copy_gbs": copy_gbs,
+                "frac_of_measured_copy": achieved / copy_gbs if achieved and copy_gbs else None}
+    log(f"dominant kernel {dom}: {avg[dom]:.3f} ms, {achieved:.0f} GB/s; copy kernel {copy_gbs:.0f} GB/s")
     per_kernel = {}
     for name, r in res.items():
         for k, ms in r["kernels_ms"].items():
@@ -254,11 +380,17 @@ def main() -> int:
             l = passes if k.startswith("rs_") else 1
             per_kernel[f"{name}/{k}"] = {
                 "ms": ms, "launches": l,
-                "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None}
+                "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None,
+                "traffic_per_launch": traffic.get(k) if name == "ordered" else None}
 
     serving = None
     if args.serve_rates:
         serving = serve_compare(e, batch, args)
+
+    e.close()
+    batching = None
+    if not args.no_batching:
+        batching = batching_compare(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -292,10 +424,9 @@ def main() -> int:
         "modes": modes,
         "upload_ms": head["upload_ms"], "upload_wall_s": upload_wall,
         "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
-        "batch_counts": counts,
+        "batching": batching, "batch_counts": counts,
     }
     print(json.dumps(out), flush=True)
-    e.close()
     return 0
 
 

@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/ltpmg_stream.h"
 #include "internal.h"
 
 using namespace ltpmg;
@@ -78,7 +79,10 @@ struct ltpmg_engine {
     uint32_t flags = 0;
     uint64_t budget = 0, allocated = 0;
     uint32_t default_txns = 0, default_ops = 0;
-    uint32_t epoch = 0;
+    int n_cu = 0;
+    // device words: [0] lowest-TID-wins epoch (moved on by each such batch),
+    // [1] grid-barrier count, [2] barrier generation, [3] barrier timeout flag
+    uint32_t *d_sync = nullptr;
     int tail_slot = -1;  // slot of the most recently executed batch
     Table tables[LTPMG_MAX_TABLES];
     ColDesc *d_cols = nullptr;
@@ -168,6 +172,42 @@ int sync_descriptors(ltpmg_engine *e) {
     return LTPMG_OK;
 }
 
+// Ordered-mode scratch for up to ops ops: sort buffers, per-tile scan state
+// and the deferred final writes.
+void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
+    const size_t st = ops / kSortTile + 1, sg = seg_tiles((uint32_t)ops) + 1;
+    for (int i = 0; i < 2; ++i) {
+        dev_free(e, ob.sort.keys[i], ops * 4);
+        dev_free(e, ob.sort.vals[i], ops * 4);
+    }
+    dev_free(e, ob.sort.hist, 256 * st * 4);
+    dev_free(e, ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)) * 4);
+    dev_free(e, ob.tile_flag, sg * 4 * 5);
+    dev_free(e, ob.deferred, sg * 16);
+    ob = OrderBufs{};
+}
+
+int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
+    const size_t st = ops / kSortTile + 1, sg = seg_tiles((uint32_t)ops) + 1;
+    int rc = LTPMG_OK;
+#define ALLOC(ptr, n) if (rc == LTPMG_OK) rc = dev_alloc_n(e, &(ptr), (n))
+    for (int i = 0; i < 2; ++i) {
+        ALLOC(ob.sort.keys[i], ops);
+        ALLOC(ob.sort.vals[i], ops);
+    }
+    ALLOC(ob.sort.hist, 256 * st);
+    ALLOC(ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)));
+    ALLOC(ob.tile_flag, sg * 5);
+    ALLOC(ob.deferred, sg);
+#undef ALLOC
+    if (rc != LTPMG_OK) return rc;
+    ob.tile_set = ob.tile_flag + sg;
+    ob.tile_delta = ob.tile_flag + 2 * sg;
+    ob.carry_set = ob.tile_flag + 3 * sg;
+    ob.carry_delta = ob.tile_flag + 4 * sg;
+    return LTPMG_OK;
+}
+
 void free_slot_buffers(ltpmg_engine *e, Slot &s) {
     const size_t ops = s.cap_ops, txns = s.cap_txns;
     dev_free(e, s.d_txn_off, (txns + 1) * 4);
@@ -176,24 +216,14 @@ void free_slot_buffers(ltpmg_engine *e, Slot &s) {
     dev_free(e, s.d_args, ops * 8);
     dev_free(e, s.d_commit, txns);
     dev_free(e, s.d_results, ops * (size_t)s.cap_res_stride);
-    const size_t tiles = ops / kTileOps + 2, st = ops / kSortTile + 1, sg = seg_tiles((uint32_t)ops) + 1;
-    dev_free(e, s.d_tile_t0, tiles * 4);
+    dev_free(e, s.d_tile_t0, (ops / kTileOps + 2) * 4);
     dev_free(e, s.d_counters, 16);
-    for (int i = 0; i < 2; ++i) {
-        dev_free(e, s.ob.sort.keys[i], ops * 4);
-        dev_free(e, s.ob.sort.vals[i], ops * 4);
-    }
-    dev_free(e, s.ob.sort.hist, 256 * st * 4);
-    dev_free(e, s.ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)) * 4);
-    dev_free(e, s.ob.tile_flag, sg * 4 * 5);
-    dev_free(e, s.ob.fin, ops * 12);
-    dev_free(e, s.ob.counters, 16);
+    free_order_bufs(e, s.ob, ops);
     if (s.h_stage) (void)hipHostFree(s.h_stage);
     s.d_txn_off = s.d_rows = s.d_tile_t0 = s.d_counters = nullptr;
     s.d_meta = nullptr;
     s.d_args = nullptr;
     s.d_commit = s.d_results = nullptr;
-    s.ob = OrderBufs{};
     s.h_stage = nullptr;
     s.stage_bytes = 0;
     s.cap_txns = s.cap_ops = s.cap_res_stride = 0;
@@ -207,7 +237,7 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
     free_slot_buffers(e, s);
     txns = std::max(txns, e->default_txns);
     ops = std::max(ops, e->default_ops);
-    const size_t tiles = ops / kTileOps + 2, st = ops / kSortTile + 1, sg = seg_tiles(ops) + 1;
+    const size_t tiles = ops / kTileOps + 2;
     int rc = LTPMG_OK;
 #define ALLOC(ptr, n) if (rc == LTPMG_OK) rc = dev_alloc_n(e, &(ptr), (n))
     ALLOC(s.d_txn_off, (size_t)txns + 1);
@@ -218,16 +248,8 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
     ALLOC(s.d_results, (size_t)ops * res_stride);
     ALLOC(s.d_tile_t0, tiles);
     ALLOC(s.d_counters, 4);
-    for (int i = 0; i < 2; ++i) {
-        ALLOC(s.ob.sort.keys[i], ops);
-        ALLOC(s.ob.sort.vals[i], ops);
-    }
-    ALLOC(s.ob.sort.hist, 256 * st);
-    ALLOC(s.ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)));
-    ALLOC(s.ob.tile_flag, sg * 5);
-    ALLOC(s.ob.fin, (size_t)ops * 3);
-    ALLOC(s.ob.counters, 4);
 #undef ALLOC
+    if (rc == LTPMG_OK) rc = alloc_order_bufs(e, s.ob, ops);
     s.cap_txns = txns;
     s.cap_ops = ops;
     s.cap_res_stride = res_stride;
@@ -235,10 +257,6 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
         free_slot_buffers(e, s);
         return rc;
     }
-    s.ob.tile_set = s.ob.tile_flag + sg;
-    s.ob.tile_delta = s.ob.tile_flag + 2 * sg;
-    s.ob.carry_set = s.ob.tile_flag + 3 * sg;
-    s.ob.carry_delta = s.ob.tile_flag + 4 * sg;
     s.stage_bytes = ((size_t)txns + 1) * 4 + (size_t)ops * (4 + 2 + 8) + 64;
     hipError_t hs = hipHostMalloc(reinterpret_cast<void **>(&s.h_stage), s.stage_bytes, hipHostMallocDefault);
     if (hs != hipSuccess) {
@@ -249,10 +267,12 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
     return LTPMG_OK;
 }
 
-int validate_batch(ltpmg_engine *e, const ltpmg_batch *b) {
+// Checks a batch as submitted; max_ops is 2^29 for one batch (ordered-mode
+// op indices have 29 bits) and larger for a stream of batches.
+int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ull << 29) {
     if (!b || !b->txn_off || (b->n_ops && (!b->rows || !b->meta || !b->args)))
         return fail(e, LTPMG_EINVAL, "null batch array");
-    if (b->n_ops >= (1u << 29)) return fail(e, LTPMG_EINVAL, "n_ops must be < 2^29");
+    if (b->n_ops >= max_ops) return fail(e, LTPMG_EINVAL, "n_ops must be < %llu", (unsigned long long)max_ops);
     if (b->txn_off[0] != 0) return fail(e, LTPMG_EINVAL, "txn_off[0] must be 0");
     for (uint32_t t = 0; t < b->n_txns; ++t) {
         uint32_t len = b->txn_off[t + 1] - b->txn_off[t];
@@ -328,6 +348,97 @@ struct StageRec {
     }
 };
 
+// Lowest-TID-wins on the ops of ov with the three-kernel pipeline.
+// wait_tables() runs before the first access to lock words or cells.
+template <class Rec, class Wait>
+int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *tile_t0,
+                  uint32_t *counters, uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec,
+                  Wait wait_tables) {
+    const uint32_t n_tiles = std::max(1u, (ov.n_ops + kTileOps - 1) / kTileOps);
+    int rc = wait_tables();
+    if (rc != LTPMG_OK) return rc;
+    DevTables dt{e->d_cols, e->d_tabs};
+    rec.begin("tile_bounds");
+    launch_zero_u32(st, counters, 4);
+    launch_tile_bounds(st, ov.txn_off, ov.n_txns, n_tiles, tile_t0, e->d_sync);
+    rec.end();
+    rec.begin("cc_reserve");
+    launch_cc_reserve(st, dt, ov, tile_t0, n_tiles, e->d_sync);
+    rec.end();
+    rec.begin("cc_validate_read");
+    launch_cc_validate_read(st, dt, ov, tile_t0, n_tiles, e->d_sync, commit, results, rs, counters);
+    rec.end();
+    rec.begin("cc_write");
+    launch_cc_write(st, dt, ov, tile_t0, n_tiles, commit, rs);
+    rec.end();
+    return LTPMG_OK;
+}
+
+// Key geometry of ordered mode: cell key = (global row << col_bits) | col.
+int order_key_bits(ltpmg_engine *e, uint32_t *col_bits, uint32_t *key_bits) {
+    uint64_t total_rows = 0;
+    uint32_t max_cols = 1;
+    for (const Table &t : e->tables)
+        if (t.live) {
+            total_rows += t.n_rows;
+            max_cols = std::max(max_cols, t.n_cols);
+        }
+    *col_bits = ceil_log2(max_cols);
+    *key_bits = ceil_log2(total_rows) + *col_bits;
+    if (*key_bits > 31)
+        return fail(e, LTPMG_EINVAL, "ordered mode needs <= 31 key bits (rows x columns), got %u", *key_bits);
+    return LTPMG_OK;
+}
+
+// Ordered mode on the ops of ov (results indexed by op within ov).  Keys,
+// sort and scan touch only the pipeline's own buffers; wait_tables() runs
+// before the first cell access.
+template <class Rec, class Wait>
+int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &ob,
+                  uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables) {
+    uint32_t col_bits = 0, key_bits = 0;
+    int rc = order_key_bits(e, &col_bits, &key_bits);
+    if (rc != LTPMG_OK) return rc;
+    DevTables dt{e->d_cols, e->d_tabs};
+    const uint32_t n_tabs = n_tabs_span(e);
+    rec.begin("mo_keys");
+    launch_mo_keys(st, dt, ov, col_bits, ob.sort.keys[0], ob.sort.vals[0]);
+    rec.end();
+    int sorted = 0;
+    if (ov.n_ops > 1) {
+        for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+            rec.begin("rs_hist");
+            launch_rs_hist(st, ob.sort, sorted, ov.n_ops, shift);
+            rec.end();
+            rec.begin("rs_scan");
+            launch_rs_scan(st, ob.sort, ov.n_ops);
+            rec.end();
+            rec.begin("rs_scatter");
+            launch_rs_scatter(st, ob.sort, sorted, ov.n_ops, shift);
+            rec.end();
+            sorted ^= 1;
+        }
+    }
+    rec.begin("mo_seg_scan");
+    launch_mo_seg_scan(st, ov, ob, sorted);
+    rec.end();
+    rc = wait_tables();
+    if (rc != LTPMG_OK) return rc;
+    rec.begin("mo_apply_reads");
+    launch_mo_apply_reads(st, dt, ov, ob, sorted, col_bits, n_tabs, results, rs);
+    rec.end();
+    rec.begin("mo_final_writes");
+    launch_mo_final_writes(st, dt, ov, ob, col_bits, n_tabs, rs);
+    launch_fill_u8(st, commit, ov.n_txns, 1);
+    rec.end();
+    return LTPMG_OK;
+}
+
+struct NoRec {
+    void begin(const char *) {}
+    void end() {}
+};
+
 }  // namespace
 
 extern "C" {
@@ -355,6 +466,11 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
     int rc = dev_alloc_n(e, &e->d_cols, LTPMG_MAX_TABLES * LTPMG_MAX_COLS);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_tabs, LTPMG_MAX_TABLES);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_digest, 1);
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_sync, 4);
+    if (rc == LTPMG_OK && hipMemset(e->d_sync, 0, 16) != hipSuccess) rc = LTPMG_EHIP;
+    if (rc == LTPMG_OK &&
+        hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess)
+        rc = LTPMG_EHIP;
     if (rc == LTPMG_OK) rc = sync_descriptors(e);
     e->slots.resize(cfg->n_slots);
     for (Slot &s : e->slots) {
@@ -394,6 +510,7 @@ int ltpmg_close(ltpmg_engine *e) {
     if (e->d_cols) (void)hipFree(e->d_cols);
     if (e->d_tabs) (void)hipFree(e->d_tabs);
     if (e->d_digest) (void)hipFree(e->d_digest);
+    if (e->d_sync) (void)hipFree(e->d_sync);
     delete e;
     return LTPMG_OK;
 }
@@ -468,13 +585,28 @@ static int column_io(ltpmg_engine *e, uint32_t table_id, uint32_t col, uint32_t 
                     (unsigned long long)row0 + n_rows);
     if (n_rows == 0) return LTPMG_OK;
     for (Slot &s : e->slots) HIP_TRY(e, hipStreamSynchronize(s.stream));
-    uint8_t *dev = t.col[col] + (size_t)row0 * t.slot[col];
-    if (to_device)
-        HIP_TRY(e, hipMemcpy2D(dev, t.slot[col], host, t.width[col], t.width[col], n_rows,
-                               hipMemcpyHostToDevice));
-    else
-        HIP_TRY(e, hipMemcpy2D(host, t.width[col], dev, t.slot[col], t.width[col], n_rows,
-                               hipMemcpyDeviceToHost));
+    const size_t w = t.width[col], sl = t.slot[col];
+    uint8_t *dev = t.col[col] + (size_t)row0 * sl;
+    uint8_t *h = static_cast<uint8_t *>(host);
+    if (w == sl) {
+        HIP_TRY(e, hipMemcpy(to_device ? (void *)dev : (void *)h, to_device ? (void *)h : (void *)dev,
+                             w * n_rows, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+        return LTPMG_OK;
+    }
+    // padded slots: repack on the host in chunks and copy each chunk flat (a
+    // strided 2-D copy from pageable memory goes row by row)
+    const size_t chunk = std::max<size_t>(1, (size_t(64) << 20) / sl);
+    std::vector<uint8_t> buf(std::min<size_t>(chunk, n_rows) * sl, 0);
+    for (size_t r0 = 0; r0 < n_rows; r0 += chunk) {
+        const size_t nr = std::min<size_t>(chunk, n_rows - r0);
+        if (to_device) {
+            for (size_t r = 0; r < nr; ++r) std::memcpy(&buf[r * sl], h + (r0 + r) * w, w);
+            HIP_TRY(e, hipMemcpy(dev + r0 * sl, buf.data(), nr * sl, hipMemcpyHostToDevice));
+        } else {
+            HIP_TRY(e, hipMemcpy(buf.data(), dev + r0 * sl, nr * sl, hipMemcpyDeviceToHost));
+            for (size_t r = 0; r < nr; ++r) std::memcpy(h + (r0 + r) * w, &buf[r * sl], w);
+        }
+    }
     return LTPMG_OK;
 }
 
@@ -619,78 +751,22 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
     if (mode != LTPMG_MODE_A && mode != LTPMG_MODE_O) return fail(e, LTPMG_EINVAL, "mode %u", mode);
     const uint32_t rs = max_slot_all(e);
     if (rs > s.cap_res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the batch was submitted");
-    DevTables dt{e->d_cols, e->d_tabs};
     OpView ov{s.d_txn_off, s.d_rows, s.d_meta, s.d_args, s.n_txns, s.n_ops};
     StageRec rec{e, s, profile};
     s.n_stages = 0;
     s.mode = mode;
     HIP_TRY(e, hipEventRecord(s.x0, s.stream));
-    if (mode == LTPMG_MODE_A) {
-        const uint32_t epoch = ++e->epoch;
-        if (epoch == 0) return fail(e, LTPMG_ESTATE, "epoch counter wrapped");
-        const uint32_t n_tiles = std::max(1u, (s.n_ops + kTileOps - 1) / kTileOps);
-        // lock words and table rows: after the previous batch, whatever its slot
+    // table rows and lock words: after the previous batch, whatever its slot
+    auto wait_tail = [&]() -> int {
         if (e->tail_slot >= 0 && e->tail_slot != (int)slot)
             HIP_TRY(e, hipStreamWaitEvent(s.stream, e->slots[e->tail_slot].done, 0));
-        rec.begin("tile_bounds");
-        launch_zero_u32(s.stream, s.d_counters, 4);
-        launch_tile_bounds(s.stream, s.d_txn_off, s.n_txns, n_tiles, s.d_tile_t0);
-        rec.end();
-        rec.begin("cc_reserve");
-        launch_cc_reserve(s.stream, dt, ov, s.d_tile_t0, n_tiles, epoch);
-        rec.end();
-        rec.begin("cc_validate_read");
-        launch_cc_validate_read(s.stream, dt, ov, s.d_tile_t0, n_tiles, epoch, s.d_commit,
-                                s.d_results, rs, s.d_counters);
-        rec.end();
-        rec.begin("cc_write");
-        launch_cc_write(s.stream, dt, ov, s.d_tile_t0, n_tiles, s.d_commit, rs);
-        rec.end();
-    } else {
-        uint64_t total_rows = 0;
-        uint32_t max_cols = 1;
-        for (const Table &t : e->tables)
-            if (t.live) {
-                total_rows += t.n_rows;
-                max_cols = std::max(max_cols, t.n_cols);
-            }
-        const uint32_t col_bits = ceil_log2(max_cols), key_bits = ceil_log2(total_rows) + col_bits;
-        if (key_bits > 31)
-            return fail(e, LTPMG_EINVAL, "ordered mode needs <= 31 key bits (rows x columns), got %u", key_bits);
-        const uint32_t n_tabs = n_tabs_span(e);
-        rec.begin("mo_keys");
-        launch_mo_keys(s.stream, dt, ov, col_bits, s.ob.sort.keys[0], s.ob.sort.vals[0]);
-        rec.end();
-        int sorted = 0;
-        if (s.n_ops > 1) {
-            for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-                rec.begin("rs_hist");
-                launch_rs_hist(s.stream, s.ob.sort, sorted, s.n_ops, shift);
-                rec.end();
-                rec.begin("rs_scan");
-                launch_rs_scan(s.stream, s.ob.sort, s.n_ops);
-                rec.end();
-                rec.begin("rs_scatter");
-                launch_rs_scatter(s.stream, s.ob.sort, sorted, s.n_ops, shift);
-                rec.end();
-                sorted ^= 1;
-            }
-        }
-        rec.begin("mo_seg_scan");
-        launch_mo_seg_scan(s.stream, ov, s.ob, sorted);
-        rec.end();
-        // keys, sort and scan only touch this slot's buffers, so they overlap
-        // the previous batch; reading and writing cells waits for it
-        if (e->tail_slot >= 0 && e->tail_slot != (int)slot)
-            HIP_TRY(e, hipStreamWaitEvent(s.stream, e->slots[e->tail_slot].done, 0));
-        rec.begin("mo_apply_reads");
-        launch_mo_apply_reads(s.stream, dt, ov, s.ob, sorted, col_bits, n_tabs, s.d_results, rs);
-        rec.end();
-        rec.begin("mo_final_writes");
-        launch_mo_final_writes(s.stream, dt, ov, s.ob, col_bits, n_tabs, rs);
-        launch_fill_u8(s.stream, s.d_commit, s.n_txns, 1);
-        rec.end();
-    }
+        return LTPMG_OK;
+    };
+    int rc = mode == LTPMG_MODE_A
+                 ? launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit, s.d_results, rs,
+                                 rec, wait_tail)
+                 : launch_mode_o(e, s.stream, ov, s.ob, s.d_commit, s.d_results, rs, rec, wait_tail);
+    if (rc != LTPMG_OK) return rc;
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(s.done, s.stream));
     e->tail_slot = (int)slot;
@@ -869,3 +945,508 @@ extern "C" int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t i
     dev_free(e, b, bytes);
     return rc;
 }
+
+// ---------------- batching form (include/ltpmg_stream.h) ----------------
+
+struct ltpmg_stream {
+    uint32_t n_batches = 0;
+    uint32_t flags = 0;
+    std::vector<uint32_t> txn_base, op_base;  // cumulative, n_batches + 1 entries
+    std::vector<uint8_t> modes;
+    uint32_t max_batch_ops = 0;
+    uint32_t res_stride = 0;
+    // the pool: batch b's own txn offsets (starting at 0) are at
+    // d_txn_off + txn_base[b] + b; its ops at op_base[b]
+    uint32_t *d_txn_off = nullptr, *d_rows = nullptr;
+    uint16_t *d_meta = nullptr;
+    uint64_t *d_args = nullptr;
+    uint8_t *d_commit = nullptr;  // per txn of the stream
+    // scratch of one launch group
+    uint8_t *d_results = nullptr;
+    uint64_t res_cap_ops = 0;
+    uint32_t *d_tile_t0 = nullptr, *d_counters = nullptr;
+    uint64_t tile_cap_ops = 0;
+    OrderBufs ob{};
+    uint64_t ob_cap_ops = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> gev;  // per-group marks of a timed (tuning) run
+    hipGraphExec_t graph = nullptr;
+    ltpmg_stream_plan graph_plan{};
+    bool tuned = false, ran = false;
+    ltpmg_stream_plan tuned_plan{};
+};
+
+namespace {
+
+struct Group {
+    uint32_t b0, b1, mode;
+    bool fused;
+    uint32_t grid;
+};
+
+const uint32_t kTiles[3] = {1024, 2048, 4096};
+
+bool plan_ok(const ltpmg_stream_plan &p) {
+    bool tile = false;
+    for (uint32_t t : kTiles) tile |= p.fused_tile_ops == t;
+    return tile && p.fused_wgs_per_cu >= 1 && p.fused_wgs_per_cu <= 16 && p.merge_max_ops < (1u << 29) &&
+           (p.flags & ~LTPMG_PLAN_SYNC_EACH) == 0;
+}
+
+// Workgroups of the fused kernel that fit on the device at once.
+uint32_t fused_grid_cap(ltpmg_engine *e, uint32_t tile, uint32_t wgs_per_cu, uint32_t rs) {
+    int per_cu = 0;
+    const void *fn = cc_fused_kernel(tile, rs);
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess || per_cu < 1)
+        return 0;
+    return (uint32_t)e->n_cu * std::min<uint32_t>(wgs_per_cu, (uint32_t)per_cu);
+}
+
+std::vector<Group> plan_groups(ltpmg_engine *e, const ltpmg_stream &s, const ltpmg_stream_plan &p) {
+    std::vector<Group> g;
+    const uint32_t cap = fused_grid_cap(e, p.fused_tile_ops, p.fused_wgs_per_cu, s.res_stride);
+    const bool each = (p.flags & LTPMG_PLAN_SYNC_EACH) != 0;
+    for (uint32_t b = 0; b < s.n_batches; ++b) {
+        const uint32_t ops = s.op_base[b + 1] - s.op_base[b];
+        if (s.modes[b] == LTPMG_MODE_O) {
+            if (!each && !g.empty() && g.back().mode == LTPMG_MODE_O &&
+                (uint64_t)s.op_base[b + 1] - s.op_base[g.back().b0] <= p.merge_max_ops) {
+                g.back().b1 = b + 1;
+                continue;
+            }
+            g.push_back(Group{b, b + 1, LTPMG_MODE_O, false, 0});
+        } else {
+            const uint32_t n_tiles = (ops + p.fused_tile_ops - 1) / p.fused_tile_ops;
+            const bool fused = cap && ops <= p.fused_max_ops && n_tiles <= (uint64_t)kFusedMaxTiles * cap;
+            g.push_back(Group{b, b + 1, LTPMG_MODE_A, fused, fused ? cc_fused_grid(ops, p.fused_tile_ops, cap) : 0});
+        }
+    }
+    return g;
+}
+
+void stream_free_scratch(ltpmg_engine *e, ltpmg_stream &s) {
+    if (!(s.flags & LTPMG_STREAM_KEEP_RESULTS)) {
+        dev_free(e, s.d_results, s.res_cap_ops * s.res_stride);
+        s.d_results = nullptr;
+        s.res_cap_ops = 0;
+    }
+    dev_free(e, s.d_tile_t0, (s.tile_cap_ops / kTileOps + 2) * 4);
+    s.d_tile_t0 = nullptr;
+    s.tile_cap_ops = 0;
+    free_order_bufs(e, s.ob, s.ob_cap_ops);
+    s.ob_cap_ops = 0;
+}
+
+// Scratch for the largest group of the plan.
+int stream_scratch(ltpmg_engine *e, ltpmg_stream &s, const std::vector<Group> &groups) {
+    uint64_t max_o = 0, max_a = 0, max_any = 0;
+    for (const Group &g : groups) {
+        const uint64_t ops = (uint64_t)s.op_base[g.b1] - s.op_base[g.b0];
+        max_any = std::max(max_any, ops);
+        if (g.mode == LTPMG_MODE_O) max_o = std::max(max_o, ops);
+        else if (!g.fused) max_a = std::max(max_a, ops);
+    }
+    if (!(s.flags & LTPMG_STREAM_KEEP_RESULTS) && max_any > s.res_cap_ops) {
+        dev_free(e, s.d_results, s.res_cap_ops * s.res_stride);
+        s.d_results = nullptr;
+        s.res_cap_ops = 0;
+        int rc = dev_alloc(e, reinterpret_cast<void **>(&s.d_results), std::max<uint64_t>(max_any, 1) * s.res_stride);
+        if (rc != LTPMG_OK) return rc;
+        s.res_cap_ops = std::max<uint64_t>(max_any, 1);
+    }
+    if (max_a > s.tile_cap_ops || !s.d_tile_t0) {
+        dev_free(e, s.d_tile_t0, (s.tile_cap_ops / kTileOps + 2) * 4);
+        s.d_tile_t0 = nullptr;
+        s.tile_cap_ops = 0;
+        int rc = dev_alloc_n(e, &s.d_tile_t0, max_a / kTileOps + 2);
+        if (rc != LTPMG_OK) return rc;
+        s.tile_cap_ops = max_a;
+    }
+    if (max_o > s.ob_cap_ops) {
+        free_order_bufs(e, s.ob, s.ob_cap_ops);
+        s.ob_cap_ops = 0;
+        int rc = alloc_order_bufs(e, s.ob, max_o);
+        if (rc != LTPMG_OK) return rc;
+        s.ob_cap_ops = max_o;
+    }
+    return LTPMG_OK;
+}
+
+int issue_group(ltpmg_engine *e, ltpmg_stream &s, const Group &g, const ltpmg_stream_plan &p) {
+    const uint32_t op0 = s.op_base[g.b0], t0 = s.txn_base[g.b0];
+    const uint32_t ops = s.op_base[g.b1] - op0, txns = s.txn_base[g.b1] - t0;
+    const uint32_t rs = s.res_stride;
+    uint8_t *results = (s.flags & LTPMG_STREAM_KEEP_RESULTS) ? s.d_results + (uint64_t)op0 * rs : s.d_results;
+    OpView ov{s.d_txn_off + t0 + g.b0, s.d_rows + op0, s.d_meta + op0, s.d_args + op0, txns, ops};
+    NoRec rec;
+    auto now = []() { return LTPMG_OK; };
+    if (g.mode == LTPMG_MODE_O) return launch_mode_o(e, s.stream, ov, s.ob, s.d_commit + t0, results, rs, rec, now);
+    if (!g.fused)
+        return launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit + t0, results, rs, rec, now);
+    GridBar bar{e->d_sync + 1, e->d_sync + 2, e->d_sync + 3, 1u << 24};
+    DevTables dt{e->d_cols, e->d_tabs};
+    HIP_TRY(e, launch_cc_fused(s.stream, dt, ov, p.fused_tile_ops, g.grid, e->d_sync, s.d_commit + t0, results,
+                               rs, s.d_counters, bar));
+    return LTPMG_OK;
+}
+
+// Reads (and clears) the grid-barrier timeout flag after the stream is idle.
+int check_barrier(ltpmg_engine *e) {
+    uint32_t w[4] = {};
+    HIP_TRY(e, hipMemcpy(w, e->d_sync, sizeof w, hipMemcpyDeviceToHost));
+    if (w[3] == 0) return LTPMG_OK;
+    w[1] = w[2] = w[3] = 0;  // keep the epoch
+    HIP_TRY(e, hipMemcpy(e->d_sync, w, sizeof w, hipMemcpyHostToDevice));
+    return fail(e, LTPMG_EHIP, "a fused kernel's grid barrier timed out (grid not co-resident); tables are undefined");
+}
+
+void stream_destroy_locked(ltpmg_engine *e, ltpmg_stream *s);
+
+void drop_graph(ltpmg_stream &s) {
+    if (s.graph) (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+}
+
+bool same_plan(const ltpmg_stream_plan &a, const ltpmg_stream_plan &b) {
+    return std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+// One run of the stream; marks (if given) gets per-group device times.
+int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan &p, double *ms,
+                      std::vector<float> *marks) {
+    if (!plan_ok(p)) return fail(e, LTPMG_EINVAL, "bad stream plan");
+    if (max_slot_all(e) > s.res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the stream was created");
+    for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+    e->tail_slot = -1;
+    std::vector<Group> groups = plan_groups(e, s, p);
+    int rc = stream_scratch(e, s, groups);
+    if (rc != LTPMG_OK) return rc;
+    const bool each = (p.flags & LTPMG_PLAN_SYNC_EACH) != 0;
+    if (p.use_graph && !each && !marks) {
+        if (!s.graph || !same_plan(s.graph_plan, p)) {
+            drop_graph(s);
+            HIP_TRY(e, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+            for (const Group &g : groups) {
+                rc = issue_group(e, s, g, p);
+                if (rc != LTPMG_OK) break;
+            }
+            hipGraph_t gr = nullptr;
+            hipError_t st = hipStreamEndCapture(s.stream, &gr);
+            if (rc != LTPMG_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            if (st != hipSuccess) return fail(e, LTPMG_EHIP, "graph capture failed: %s", hipGetErrorString(st));
+            st = hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (st != hipSuccess) {
+                s.graph = nullptr;
+                return fail(e, LTPMG_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(st));
+            }
+            s.graph_plan = p;
+        }
+        HIP_TRY(e, hipEventRecord(s.ev0, s.stream));
+        HIP_TRY(e, hipGraphLaunch(s.graph, s.stream));
+        HIP_TRY(e, hipEventRecord(s.ev1, s.stream));
+    } else {
+        if (marks)
+            while (s.gev.size() < groups.size() + 1) {
+                hipEvent_t ev = nullptr;
+                HIP_TRY(e, hipEventCreate(&ev));
+                s.gev.push_back(ev);
+            }
+        HIP_TRY(e, hipEventRecord(s.ev0, s.stream));
+        for (size_t i = 0; i < groups.size(); ++i) {
+            if (marks) HIP_TRY(e, hipEventRecord(s.gev[i], s.stream));
+            rc = issue_group(e, s, groups[i], p);
+            if (rc != LTPMG_OK) return rc;
+            if (each) HIP_TRY(e, hipStreamSynchronize(s.stream));
+        }
+        if (marks) HIP_TRY(e, hipEventRecord(s.gev[groups.size()], s.stream));
+        HIP_TRY(e, hipEventRecord(s.ev1, s.stream));
+    }
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipStreamSynchronize(s.stream));
+    rc = check_barrier(e);
+    if (rc != LTPMG_OK) return rc;
+    s.ran = true;
+    float f = 0.f;
+    HIP_TRY(e, hipEventElapsedTime(&f, s.ev0, s.ev1));
+    if (ms) *ms = f;
+    if (marks) {
+        marks->assign(s.n_batches, 0.f);
+        for (size_t i = 0; i < groups.size(); ++i) {
+            float g = 0.f;
+            HIP_TRY(e, hipEventElapsedTime(&g, s.gev[i], s.gev[i + 1]));
+            (*marks)[groups[i].b0] = g;  // tuning runs cut every batch alone
+        }
+    }
+    return LTPMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ltpmg_stream_default_plan(ltpmg_stream_plan *p) {
+    if (!p) return;
+    *p = ltpmg_stream_plan{1u << 24, 1u << 18, 2048, 2, 1, 0};
+}
+
+int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *batch_txns, const uint8_t *modes,
+                        const ltpmg_batch *all, uint32_t flags, ltpmg_stream **out) {
+    if (!e || !out || !batch_txns || !modes || !all || n_batches == 0) return LTPMG_EINVAL;
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (flags & ~LTPMG_STREAM_KEEP_RESULTS) return fail(e, LTPMG_EINVAL, "stream flags 0x%x", flags);
+    int rc = validate_batch(e, all, 0xFFFFFFFFull);
+    if (rc != LTPMG_OK) return rc;
+    ltpmg_stream *s = new ltpmg_stream();
+    s->n_batches = n_batches;
+    s->flags = flags;
+    s->modes.assign(modes, modes + n_batches);
+    s->txn_base.assign(n_batches + 1, 0);
+    s->op_base.assign(n_batches + 1, 0);
+    uint64_t t = 0;
+    for (uint32_t b = 0; b < n_batches; ++b) {
+        if (batch_txns[b] == 0 || modes[b] > LTPMG_MODE_O) {
+            delete s;
+            return fail(e, LTPMG_EINVAL, "batch %u: %u txns, mode %u", b, batch_txns[b], modes[b]);
+        }
+        t += batch_txns[b];
+        if (t > all->n_txns) {
+            delete s;
+            return fail(e, LTPMG_EINVAL, "batch sizes add up to more than the %u txns given", all->n_txns);
+        }
+        s->txn_base[b + 1] = (uint32_t)t;
+        s->op_base[b + 1] = all->txn_off[t];
+        const uint32_t ops = s->op_base[b + 1] - s->op_base[b];
+        if (ops >= (1u << 29)) {
+            delete s;
+            return fail(e, LTPMG_EINVAL, "batch %u has %u ops (must be < 2^29)", b, ops);
+        }
+        s->max_batch_ops = std::max(s->max_batch_ops, ops);
+    }
+    if (t != all->n_txns) {
+        delete s;
+        return fail(e, LTPMG_EINVAL, "batch sizes add up to %llu txns, %u given", (unsigned long long)t, all->n_txns);
+    }
+    s->res_stride = max_slot_all(e);
+    const uint64_t n_ops = all->n_ops, n_txns = all->n_txns;
+    // per-batch offsets, each batch starting at 0
+    std::vector<uint32_t> rel(n_txns + n_batches);
+    for (uint32_t b = 0; b < n_batches; ++b) {
+        const uint32_t base = all->txn_off[s->txn_base[b]];
+        for (uint32_t k = s->txn_base[b]; k <= s->txn_base[b + 1]; ++k) rel[k + b] = all->txn_off[k] - base;
+    }
+#define ALLOC(ptr, n) if (rc == LTPMG_OK) rc = dev_alloc_n(e, &(ptr), (n))
+    ALLOC(s->d_txn_off, rel.size());
+    ALLOC(s->d_rows, std::max<uint64_t>(n_ops, 1));
+    ALLOC(s->d_meta, std::max<uint64_t>(n_ops, 1));
+    ALLOC(s->d_args, std::max<uint64_t>(n_ops, 1));
+    ALLOC(s->d_commit, n_txns);
+    ALLOC(s->d_counters, 4);
+    if (flags & LTPMG_STREAM_KEEP_RESULTS) {
+        ALLOC(s->d_results, std::max<uint64_t>(n_ops, 1) * s->res_stride);
+        s->res_cap_ops = std::max<uint64_t>(n_ops, 1);
+    }
+#undef ALLOC
+    if (rc == LTPMG_OK) {
+        hipError_t st = hipMemcpy(s->d_txn_off, rel.data(), rel.size() * 4, hipMemcpyHostToDevice);
+        if (st == hipSuccess && n_ops) st = hipMemcpy(s->d_rows, all->rows, n_ops * 4, hipMemcpyHostToDevice);
+        if (st == hipSuccess && n_ops) st = hipMemcpy(s->d_meta, all->meta, n_ops * 2, hipMemcpyHostToDevice);
+        if (st == hipSuccess && n_ops) st = hipMemcpy(s->d_args, all->args, n_ops * 8, hipMemcpyHostToDevice);
+        if (st == hipSuccess) st = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (st == hipSuccess) st = hipEventCreate(&s->ev0);
+        if (st == hipSuccess) st = hipEventCreate(&s->ev1);
+        if (st != hipSuccess) rc = fail(e, LTPMG_EHIP, "stream upload failed: %s", hipGetErrorString(st));
+    }
+    if (rc != LTPMG_OK) {
+        stream_destroy_locked(e, s);
+        return rc;
+    }
+    *out = s;
+    return LTPMG_OK;
+}
+
+int ltpmg_stream_destroy(ltpmg_engine *e, ltpmg_stream *s) {
+    if (!e || !s) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    (void)hipSetDevice(e->device);
+    stream_destroy_locked(e, s);
+    return LTPMG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+void stream_destroy_locked(ltpmg_engine *e, ltpmg_stream *s) {
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graph(*s);
+    stream_free_scratch(e, *s);
+    const uint64_t n_ops = s->op_base.empty() ? 0 : s->op_base.back();
+    const uint64_t n_txns = s->txn_base.empty() ? 0 : s->txn_base.back();
+    dev_free(e, s->d_txn_off, (n_txns + s->n_batches) * 4);
+    dev_free(e, s->d_rows, std::max<uint64_t>(n_ops, 1) * 4);
+    dev_free(e, s->d_meta, std::max<uint64_t>(n_ops, 1) * 2);
+    dev_free(e, s->d_args, std::max<uint64_t>(n_ops, 1) * 8);
+    dev_free(e, s->d_commit, n_txns);
+    dev_free(e, s->d_counters, 16);
+    if (s->flags & LTPMG_STREAM_KEEP_RESULTS) dev_free(e, s->d_results, s->res_cap_ops * s->res_stride);
+    for (hipEvent_t ev : s->gev) (void)hipEventDestroy(ev);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+}  // namespace
+
+extern "C" {
+
+int ltpmg_stream_run(ltpmg_engine *e, ltpmg_stream *s, const ltpmg_stream_plan *plan, double *device_ms) {
+    if (!e || !s) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    ltpmg_stream_plan p;
+    if (plan) p = *plan;
+    else if (s->tuned) p = s->tuned_plan;
+    else ltpmg_stream_default_plan(&p);
+    return stream_run_locked(e, *s, p, device_ms, nullptr);
+}
+
+int ltpmg_stream_tune(ltpmg_engine *e, ltpmg_stream *s, ltpmg_stream_plan *best, uint32_t *trials_out) {
+    if (!e || !s) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    ltpmg_stream_plan p;
+    ltpmg_stream_default_plan(&p);
+    uint32_t trials = 0;
+    bool any_a = false, any_o = false;
+    for (uint8_t m : s->modes) (m == LTPMG_MODE_A ? any_a : any_o) = true;
+    int rc = LTPMG_OK;
+    if (any_a) {
+        // per-batch times of the three-kernel pipeline and of each fused
+        // configuration; then the size cut that minimises their sum
+        std::vector<float> multi, t;
+        ltpmg_stream_plan q = p;
+        q.fused_max_ops = 0;
+        q.merge_max_ops = 0;
+        q.use_graph = 0;
+        rc = stream_run_locked(e, *s, q, nullptr, &multi);
+        if (rc != LTPMG_OK) return rc;
+        ++trials;
+        std::vector<uint32_t> a_ops;
+        for (uint32_t b = 0; b < s->n_batches; ++b)
+            if (s->modes[b] == LTPMG_MODE_A) a_ops.push_back(s->op_base[b + 1] - s->op_base[b]);
+        std::sort(a_ops.begin(), a_ops.end());
+        double best_ms = 0;
+        for (uint32_t b = 0; b < s->n_batches; ++b)
+            if (s->modes[b] == LTPMG_MODE_A) best_ms += multi[b];
+        p.fused_max_ops = 0;
+        for (uint32_t tile : kTiles) {
+            uint32_t last_cap = 0;
+            for (uint32_t wgs : {1u, 2u, 4u, 8u}) {
+                const uint32_t cap = fused_grid_cap(e, tile, wgs, s->res_stride);
+                if (cap == 0 || cap == last_cap) continue;  // occupancy-clamped repeat
+                last_cap = cap;
+                q.fused_tile_ops = tile;
+                q.fused_wgs_per_cu = wgs;
+                q.fused_max_ops = 0xFFFFFFFFu;
+                rc = stream_run_locked(e, *s, q, nullptr, &t);
+                if (rc != LTPMG_OK) return rc;
+                ++trials;
+                for (uint32_t cut : a_ops) {
+                    double sum = 0;
+                    for (uint32_t b = 0; b < s->n_batches; ++b)
+                        if (s->modes[b] == LTPMG_MODE_A)
+                            sum += (s->op_base[b + 1] - s->op_base[b] <= cut) ? t[b] : multi[b];
+                    if (sum < best_ms) {
+                        best_ms = sum;
+                        p.fused_max_ops = cut;
+                        p.fused_tile_ops = tile;
+                        p.fused_wgs_per_cu = wgs;
+                    }
+                }
+            }
+        }
+    }
+    if (any_o) {
+        double best_ms = -1;
+        uint32_t best_merge = 0;
+        for (uint32_t merge : {0u, 1u << 20, 1u << 22, 1u << 24, 1u << 26, (1u << 28)}) {
+            ltpmg_stream_plan q = p;
+            q.merge_max_ops = merge;
+            double ms = 0;
+            rc = stream_run_locked(e, *s, q, &ms, nullptr);
+            if (rc != LTPMG_OK) return rc;
+            ++trials;
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                best_merge = merge;
+            }
+        }
+        p.merge_max_ops = best_merge;
+    }
+    p.use_graph = 1;
+    s->tuned = true;
+    s->tuned_plan = p;
+    if (best) *best = p;
+    if (trials_out) *trials_out = trials;
+    return LTPMG_OK;
+}
+
+int ltpmg_stream_fetch(ltpmg_engine *e, ltpmg_stream *s, uint32_t batch, uint8_t *commit, void *results,
+                       uint64_t results_bytes) {
+    if (!e || !s) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (batch >= s->n_batches) return fail(e, LTPMG_EINVAL, "batch %u >= %u", batch, s->n_batches);
+    if (!s->ran) return fail(e, LTPMG_ESTATE, "the stream has not run");
+    const uint32_t t0 = s->txn_base[batch], nt = s->txn_base[batch + 1] - t0;
+    const uint64_t op0 = s->op_base[batch], no = s->op_base[batch + 1] - op0;
+    if (commit) HIP_TRY(e, hipMemcpy(commit, s->d_commit + t0, nt, hipMemcpyDeviceToHost));
+    if (results) {
+        if (!(s->flags & LTPMG_STREAM_KEEP_RESULTS))
+            return fail(e, LTPMG_ESTATE, "the stream was created without LTPMG_STREAM_KEEP_RESULTS");
+        const uint64_t need = no * s->res_stride;
+        if (results_bytes < need)
+            return fail(e, LTPMG_EINVAL, "results buffer holds %llu bytes, %llu needed",
+                        (unsigned long long)results_bytes, (unsigned long long)need);
+        if (need) HIP_TRY(e, hipMemcpy(results, s->d_results + op0 * s->res_stride, need, hipMemcpyDeviceToHost));
+    }
+    return LTPMG_OK;
+}
+
+int ltpmg_stream_committed(ltpmg_engine *e, ltpmg_stream *s, uint32_t *per_batch) {
+    if (!e || !s || !per_batch) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (!s->ran) return fail(e, LTPMG_ESTATE, "the stream has not run");
+    std::vector<uint8_t> c(s->txn_base.back());
+    if (!c.empty()) HIP_TRY(e, hipMemcpy(c.data(), s->d_commit, c.size(), hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < s->n_batches; ++b) {
+        uint32_t n = 0;
+        for (uint32_t t = s->txn_base[b]; t < s->txn_base[b + 1]; ++t) n += c[t];
+        per_batch[b] = n;
+    }
+    return LTPMG_OK;
+}
+
+int ltpmg_stream_groups(ltpmg_engine *e, ltpmg_stream *s, const ltpmg_stream_plan *plan, uint32_t *group_first,
+                        uint32_t cap) {
+    if (!e || !s) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    ltpmg_stream_plan p;
+    if (plan) p = *plan;
+    else if (s->tuned) p = s->tuned_plan;
+    else ltpmg_stream_default_plan(&p);
+    if (!plan_ok(p)) return fail(e, LTPMG_EINVAL, "bad stream plan");
+    std::vector<Group> gr = plan_groups(e, *s, p);
+    for (size_t i = 0; i < gr.size() && i < cap && group_first; ++i) group_first[i] = gr[i].b0;
+    return (int)gr.size();
+}
+
+}  // extern "C"

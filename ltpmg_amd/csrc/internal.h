@@ -139,17 +139,35 @@ void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
 void launch_digest_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
                           uint32_t n_rows, unsigned long long *acc);
 
+// tile_t0[k] = first txn of CC tile k; also moves the device epoch on
 void launch_tile_bounds(hipStream_t s, const uint32_t *txn_off, uint32_t n_txns,
-                        uint32_t n_tiles, uint32_t *tile_t0);
+                        uint32_t n_tiles, uint32_t *tile_t0, uint32_t *epoch_ctr);
 void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_stream_copy(hipStream_t s, const void *src, void *dst, uint64_t n16);
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                       uint32_t n_tiles, uint32_t epoch);
+                       uint32_t n_tiles, const uint32_t *epoch_ctr);
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                             uint32_t n_tiles, uint32_t epoch, uint8_t *commit,
+                             uint32_t n_tiles, const uint32_t *epoch_ctr, uint8_t *commit,
                              uint8_t *results, uint32_t res_stride, uint32_t *counters);
 void launch_cc_write(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                      uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot);
+
+// Grid barrier state in device memory (zero at start; reset by the last
+// arriver, so it is reused by every launch).  spin_limit bounds each wait.
+struct GridBar {
+    uint32_t *count;
+    uint32_t *gen;
+    uint32_t *err;
+    uint32_t spin_limit;
+};
+constexpr uint32_t kFusedMaxTiles = 64;  // tiles per workgroup in the fused CC kernel
+// The persistent lowest-TID-wins kernel (reserve | validate + read | write in
+// one launch).  tile_ops is 1024, 2048 or 4096; grid must be co-resident.
+const void *cc_fused_kernel(uint32_t tile_ops, uint32_t res_stride);
+uint32_t cc_fused_grid(uint32_t n_ops, uint32_t tile_ops, uint32_t grid);
+hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile_ops,
+                           uint32_t grid, uint32_t *epoch_ctr, uint8_t *commit, uint8_t *results,
+                           uint32_t res_stride, uint32_t *counters, GridBar bar);
 
 struct SortBufs {
     uint32_t *keys[2];
@@ -175,8 +193,7 @@ struct OrderBufs {
     uint32_t *tile_delta;
     uint32_t *carry_set;    // per tile carry-in
     uint32_t *carry_delta;
-    uint32_t *fin;          // final-write list: 3 u32 per entry {key, set, delta}
-    uint32_t *counters;     // [0] unused, [1] final-write list length
+    uint4 *deferred;        // per seg-scan tile: a final write that waits for every tile's reads
 };
 void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, uint32_t *keys,
                     uint32_t *vals);

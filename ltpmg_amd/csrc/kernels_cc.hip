@@ -21,6 +21,15 @@
 // 2048-op tile, so it validates whole txns without cross-workgroup
 // traffic; the op -> txn map of the tile is rebuilt in LDS from txn_off
 // (4 bytes per txn) instead of shipping a TID per op.
+//
+// The epoch lives in device memory: tile_bounds (or the fused kernel) moves
+// it on once per batch, so a captured launch sequence replays correctly.
+//
+// cc_fused runs the three passes in one persistent launch for small and
+// mid-size batches: each workgroup takes a contiguous run of tiles, finds
+// their txn bounds itself, and passes two grid barriers (reserve | validate
+// + read | write); with one tile per workgroup the op -> txn map and the
+// commit flags stay in LDS across all three passes.
 #include "internal.h"
 
 namespace ltpmg {
@@ -31,18 +40,11 @@ struct Tile {
     uint32_t t0, nt, op0, nops;
 };
 
-// Loads the tile's txn range and fills owner[o - op0] = txn index in tile.
-__device__ inline Tile load_tile(const OpView &ov, const uint32_t *tile_t0, uint16_t *owner) {
-    __shared__ uint32_t s_t[2], s_o[2];
-    if (threadIdx.x == 0) {
-        uint32_t t0 = tile_t0[blockIdx.x], t1 = tile_t0[blockIdx.x + 1];
-        s_t[0] = t0;
-        s_t[1] = t1;
-        s_o[0] = ov.txn_off[t0];
-        s_o[1] = ov.txn_off[t1];
-    }
-    __syncthreads();
-    Tile t{s_t[0], s_t[1] - s_t[0], s_o[0], s_o[1] - s_o[0]};
+// Txns [t0, t1) of a tile; fills owner[o - op0] = txn index in the tile.
+__device__ inline Tile fill_owner(const OpView &ov, uint32_t t0, uint32_t t1, uint16_t *owner) {
+    __syncthreads();  // the previous user of owner[] is done
+    const uint32_t op0 = ov.txn_off[t0];
+    Tile t{t0, t1 - t0, op0, ov.txn_off[t1] - op0};
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) {
         uint32_t a = ov.txn_off[t.t0 + j] - t.op0, b = ov.txn_off[t.t0 + j + 1] - t.op0;
         for (uint32_t o = a; o < b; ++o) owner[o] = (uint16_t)j;
@@ -132,12 +134,10 @@ __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &
     }
 }
 
-}  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
-                                                       const uint32_t *tile_t0, uint32_t epoch) {
-    __shared__ uint16_t owner[kTileOpsMax];
-    Tile t = load_tile(ov, tile_t0, owner);
+// Pass 1 on one tile: every valid write op offers its TID for its row.
+__device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const Tile &t,
+                                    const uint16_t *owner, uint32_t epoch) {
     const unsigned long long ep = (unsigned long long)epoch << 32;
     for (uint32_t i = threadIdx.x; i < t.nops; i += blockDim.x) {
         uint32_t o = t.op0 + i;
@@ -153,23 +153,18 @@ __global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
     }
 }
 
-// G lanes cooperate on one op; each moves 16 bytes (VEC16) or a 4-byte word.
+// Pass 2 on one tile: commit flags into ok[] and commit[], committed READs
+// copied to their result slots.  Returns the tile's committed count.
 template <int G, bool VEC16>
-__global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpView ov,
-                                                             const uint32_t *tile_t0,
-                                                             uint32_t epoch, uint8_t *commit,
-                                                             uint8_t *results,
-                                                             uint32_t res_stride,
-                                                             uint32_t *counters) {
-    __shared__ uint16_t owner[kTileOpsMax];
-    __shared__ uint8_t ok[kTileOps];
-    __shared__ uint32_t list[kTileOpsMax];
+__device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView &ov, const Tile &t,
+                                              const uint16_t *owner, uint32_t epoch, uint8_t *ok,
+                                              uint32_t *list, uint8_t *commit, uint8_t *results,
+                                              uint32_t res_stride) {
     __shared__ uint32_t n_list, n_commit;
-    Tile t = load_tile(ov, tile_t0, owner);
+    __syncthreads();  // the previous pass's readers of ok[] and list[] are done
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) ok[j] = 1;
     if (threadIdx.x == 0) n_list = n_commit = 0;
     __syncthreads();
-
     for (uint32_t i = threadIdx.x; i < t.nops; i += blockDim.x) {
         uint32_t o = t.op0 + i;
         uint16_t m = ov.meta[o];
@@ -184,7 +179,6 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
         if ((uint32_t)(lk >> 32) == epoch && (uint32_t)~(uint32_t)lk < tid) ok[owner[i]] = 0;
     }
     __syncthreads();
-
     uint32_t mine = 0;
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) {
         commit[t.t0 + j] = ok[j];
@@ -192,7 +186,6 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
     }
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if (__lane_id() == 0 && mine) atomicAdd(&n_commit, mine);
-
     // committed READs of this tile -> LDS list (loop bound uniform per wave)
     for (uint32_t base = 0; base < t.nops; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
@@ -201,21 +194,21 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
         wave_append(take, t.op0 + i, list, &n_list);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && n_commit) atomicAdd(&counters[0], n_commit);
-
+    const uint32_t committed = n_commit;
     move_ops<G, VEC16, true>(n_list, list, ov, dt, results, res_stride);
+    return committed;
 }
 
+// Pass 3 on one tile: committed writes.  ok[] holds the tile's commit
+// flags, or is filled here from commit[] when fill_ok is set.
 template <int G, bool VEC16>
-__global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
-                                                     const uint32_t *tile_t0,
-                                                     const uint8_t *commit) {
-    __shared__ uint16_t owner[kTileOpsMax];
-    __shared__ uint8_t ok[kTileOps];
-    __shared__ uint32_t list[kTileOpsMax];
+__device__ inline void tile_write(const DevTables &dt, const OpView &ov, const Tile &t,
+                                  const uint16_t *owner, uint8_t *ok, bool fill_ok,
+                                  const uint8_t *commit, uint32_t *list) {
     __shared__ uint32_t n_list;
-    Tile t = load_tile(ov, tile_t0, owner);
-    for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) ok[j] = commit[t.t0 + j];
+    __syncthreads();  // the previous pass's readers of ok[] and list[] are done
+    if (fill_ok)
+        for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) ok[j] = commit[t.t0 + j];
     if (threadIdx.x == 0) n_list = 0;
     __syncthreads();
     for (uint32_t base = 0; base < t.nops; base += blockDim.x) {
@@ -228,47 +221,270 @@ __global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
         wave_append(take, t.op0 + i, list, &n_list);
     }
     __syncthreads();
-
     move_ops<G, VEC16, false>(n_list, list, ov, dt, nullptr, 0u);
 }
 
+// Smallest t in [0, n] with a[t] >= target, given a[n] >= target and a
+// nondecreasing.  256 samples per round: a 1M-txn batch takes 3 rounds of
+// one load per thread.
+__device__ inline uint32_t block_lower_bound(const uint32_t *a, uint32_t n, uint64_t target) {
+    __shared__ uint32_t s_lo, s_hi, s_first;
+    __syncthreads();  // earlier readers of s_lo / s_hi are done
+    if (threadIdx.x == 0) {
+        s_lo = 0;
+        s_hi = n;
+    }
+    __syncthreads();
+    while (true) {
+        const uint32_t lo = s_lo, hi = s_hi;
+        if (lo >= hi) break;  // uniform: every thread read the same values
+        const uint32_t p = lo + (uint32_t)(((uint64_t)(hi - lo) * threadIdx.x) / (kBlock - 1));
+        const bool ge = (uint64_t)a[p] >= target;
+        if (threadIdx.x == 0) s_first = kBlock - 1;  // a[hi] >= target
+        __syncthreads();  // s_first reset, s_lo / s_hi read by all
+        if (ge) atomicMin(&s_first, threadIdx.x);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t j = s_first;
+            if (j == 0) {
+                s_hi = lo;
+            } else {
+                s_lo = lo + (uint32_t)(((uint64_t)(hi - lo) * (j - 1)) / (kBlock - 1)) + 1;
+                s_hi = lo + (uint32_t)(((uint64_t)(hi - lo) * j) / (kBlock - 1));
+            }
+        }
+        __syncthreads();
+    }
+    return s_lo;
+}
+
+// Reusable grid barrier (count + generation).  The last workgroup to arrive
+// resets the count and moves the generation on; the others wait for the
+// generation with a bounded spin, so every wave leaves even if the grid were
+// not co-resident (then err is set and the kernel stops).
+__device__ inline bool grid_sync(const GridBar &b) {
+    __shared__ uint32_t s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = __hip_atomic_load(b.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();  // this workgroup's writes before its arrival
+        uint32_t ok = 1;
+        if (atomicAdd(b.count, 1u) == gridDim.x - 1) {
+            __hip_atomic_store(b.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            atomicAdd(b.gen, 1u);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(b.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > b.spin_limit) {
+                    atomicOr(b.err, 1u);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();  // the other workgroups' writes after leaving
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+}  // namespace
+
+// Every txn has >= 1 op, so a tile owns <= kTileOps txns and
+// < kTileOps + LTPMG_MAX_TXN_OPS ops.
+__global__ void k_tile_bounds(const uint32_t *txn_off, uint32_t n_txns, uint32_t n_tiles,
+                              uint32_t *tile_t0, uint32_t *epoch_ctr) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *epoch_ctr += 1;  // this batch's epoch, read by the next kernels
+    if (k > n_tiles) return;
+    uint64_t target = (uint64_t)k * kTileOps;
+    uint32_t lo = 0, hi = n_txns;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)txn_off[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    tile_t0[k] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
+                                                       const uint32_t *tile_t0,
+                                                       const uint32_t *epoch_ctr) {
+    __shared__ uint16_t owner[kTileOpsMax];
+    const uint32_t epoch = *epoch_ctr;
+    Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
+    tile_reserve(dt, ov, t, owner, epoch);
+}
+
+// G lanes cooperate on one op; each moves 16 bytes (VEC16) or a 4-byte word.
+template <int G, bool VEC16>
+__global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpView ov,
+                                                             const uint32_t *tile_t0,
+                                                             const uint32_t *epoch_ctr,
+                                                             uint8_t *commit, uint8_t *results,
+                                                             uint32_t res_stride,
+                                                             uint32_t *counters) {
+    __shared__ uint16_t owner[kTileOpsMax];
+    __shared__ uint8_t ok[kTileOps];
+    __shared__ uint32_t list[kTileOpsMax];
+    const uint32_t epoch = *epoch_ctr;
+    Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
+    uint32_t c = tile_validate_read<G, VEC16>(dt, ov, t, owner, epoch, ok, list, commit, results,
+                                              res_stride);
+    if (threadIdx.x == 0 && c) atomicAdd(&counters[0], c);
+}
+
+template <int G, bool VEC16>
+__global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
+                                                     const uint32_t *tile_t0,
+                                                     const uint8_t *commit) {
+    __shared__ uint16_t owner[kTileOpsMax];
+    __shared__ uint8_t ok[kTileOps];
+    __shared__ uint32_t list[kTileOpsMax];
+    Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
+    tile_write<G, VEC16>(dt, ov, t, owner, ok, true, commit, list);
+}
+
+// All three passes in one persistent launch.  Workgroup b owns tiles
+// [b * tpb, (b + 1) * tpb) of TILE nominal ops; the host sizes the grid so
+// that it is co-resident (the occupancy of this kernel times the CUs).
+template <int G, bool VEC16, uint32_t TILE>
+__global__ __launch_bounds__(kBlock) void k_cc_fused(DevTables dt, OpView ov, uint32_t n_tiles,
+                                                     uint32_t tpb, uint32_t *epoch_ctr,
+                                                     uint8_t *commit, uint8_t *results,
+                                                     uint32_t res_stride, uint32_t *counters,
+                                                     GridBar bar) {
+    __shared__ uint16_t owner[TILE + LTPMG_MAX_TXN_OPS];
+    __shared__ uint8_t ok[TILE];
+    __shared__ uint32_t list[TILE + LTPMG_MAX_TXN_OPS];
+    __shared__ uint32_t s_t0[kFusedMaxTiles + 1];
+    const uint32_t epoch = *epoch_ctr + 1;  // every workgroup reads it before the first barrier
+    const uint32_t k0 = min(n_tiles, blockIdx.x * tpb), k1 = min(n_tiles, k0 + tpb);
+    const uint32_t nk = k1 - k0;
+    if (nk) {
+        // s_t0[j] = first txn of tile k0 + j: a search for the first, then
+        // one coalesced sweep over the run's txns for the rest
+        const uint32_t first = block_lower_bound(ov.txn_off, ov.n_txns, (uint64_t)k0 * TILE);
+        for (uint32_t j = threadIdx.x; j <= nk; j += blockDim.x) s_t0[j] = j ? ov.n_txns : first;
+        __syncthreads();
+        const uint64_t end = (uint64_t)k1 * TILE;
+        for (uint32_t t = first + 1 + threadIdx.x;; t += blockDim.x) {
+            bool more = false;
+            if (t <= ov.n_txns) {
+                const uint64_t lo = ov.txn_off[t - 1], hi = ov.txn_off[t];
+                // tiles k with lo < k * TILE <= hi start at txn t
+                for (uint64_t k = lo / TILE + 1; k * TILE <= hi; ++k)
+                    if (k > k0 && k <= k1) s_t0[k - k0] = (uint32_t)t;
+                more = hi < end;
+            }
+            if (!__syncthreads_or(more)) break;
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[0] = 0;
+    Tile t{0u, 0u, 0u, 0u};
+    for (uint32_t j = 0; j < nk; ++j) {
+        t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        tile_reserve(dt, ov, t, owner, epoch);
+    }
+    if (!grid_sync(bar)) return;
+    uint32_t committed = 0;
+    for (uint32_t j = 0; j < nk; ++j) {
+        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        committed += tile_validate_read<G, VEC16>(dt, ov, t, owner, epoch, ok, list, commit,
+                                                  results, res_stride);
+    }
+    if (threadIdx.x == 0 && committed) atomicAdd(&counters[0], committed);
+    if (!grid_sync(bar)) return;
+    for (uint32_t j = 0; j < nk; ++j) {
+        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        tile_write<G, VEC16>(dt, ov, t, owner, ok, nk > 1, commit, list);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *epoch_ctr = epoch;
+}
+
+void launch_tile_bounds(hipStream_t s, const uint32_t *txn_off, uint32_t n_txns,
+                        uint32_t n_tiles, uint32_t *tile_t0, uint32_t *epoch_ctr) {
+    uint32_t blocks = (n_tiles + 1 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_tile_bounds, dim3(blocks), dim3(kBlock), 0, s, txn_off, n_txns,
+                       n_tiles, tile_t0, epoch_ctr);
+}
+
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                       uint32_t n_tiles, uint32_t epoch) {
-    hipLaunchKernelGGL(k_cc_reserve, dim3(n_tiles), dim3(kBlock), 0, s, dt, ov, tile_t0, epoch);
+                       uint32_t n_tiles, const uint32_t *epoch_ctr) {
+    hipLaunchKernelGGL(k_cc_reserve, dim3(n_tiles), dim3(kBlock), 0, s, dt, ov, tile_t0,
+                       epoch_ctr);
 }
 
 // Lanes per op: one 16-byte chunk per lane, up to 8 lanes (128-byte slots).
-#define LTPMG_CC_DISPATCH(KERNEL, ...)                                                     \
+#define LTPMG_CC_DISPATCH(KERNEL, GRID, ...)                                               \
     do {                                                                                   \
         if (max_slot >= 128)                                                               \
-            hipLaunchKernelGGL((KERNEL<8, true>), dim3(n_tiles), dim3(kBlock), 0, s,       \
-                               __VA_ARGS__);                                               \
+            hipLaunchKernelGGL((KERNEL<8, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
         else if (max_slot >= 64)                                                           \
-            hipLaunchKernelGGL((KERNEL<4, true>), dim3(n_tiles), dim3(kBlock), 0, s,       \
-                               __VA_ARGS__);                                               \
+            hipLaunchKernelGGL((KERNEL<4, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
         else if (max_slot >= 32)                                                           \
-            hipLaunchKernelGGL((KERNEL<2, true>), dim3(n_tiles), dim3(kBlock), 0, s,       \
-                               __VA_ARGS__);                                               \
+            hipLaunchKernelGGL((KERNEL<2, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
         else if (max_slot >= 16)                                                           \
-            hipLaunchKernelGGL((KERNEL<1, true>), dim3(n_tiles), dim3(kBlock), 0, s,       \
-                               __VA_ARGS__);                                               \
+            hipLaunchKernelGGL((KERNEL<1, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
         else                                                                               \
-            hipLaunchKernelGGL((KERNEL<1, false>), dim3(n_tiles), dim3(kBlock), 0, s,      \
-                               __VA_ARGS__);                                               \
+            hipLaunchKernelGGL((KERNEL<1, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
 
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                             uint32_t n_tiles, uint32_t epoch, uint8_t *commit,
+                             uint32_t n_tiles, const uint32_t *epoch_ctr, uint8_t *commit,
                              uint8_t *results, uint32_t res_stride, uint32_t *counters) {
     // the result stride is the largest slot, so it selects the vector width
     uint32_t max_slot = res_stride;
-    LTPMG_CC_DISPATCH(k_cc_validate_read, dt, ov, tile_t0, epoch, commit, results, res_stride,
-                      counters);
+    LTPMG_CC_DISPATCH(k_cc_validate_read, dim3(n_tiles), dt, ov, tile_t0, epoch_ctr, commit,
+                      results, res_stride, counters);
 }
 
 void launch_cc_write(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                      uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot) {
-    LTPMG_CC_DISPATCH(k_cc_write, dt, ov, tile_t0, commit);
+    LTPMG_CC_DISPATCH(k_cc_write, dim3(n_tiles), dt, ov, tile_t0, commit);
+}
+
+namespace {
+template <int G, bool VEC16>
+const void *fused_fn(uint32_t tile) {
+    if (tile == 1024) return reinterpret_cast<const void *>(&k_cc_fused<G, VEC16, 1024>);
+    if (tile == 2048) return reinterpret_cast<const void *>(&k_cc_fused<G, VEC16, 2048>);
+    if (tile == 4096) return reinterpret_cast<const void *>(&k_cc_fused<G, VEC16, 4096>);
+    return nullptr;
+}
+}  // namespace
+
+const void *cc_fused_kernel(uint32_t tile_ops, uint32_t res_stride) {
+    if (res_stride >= 128) return fused_fn<8, true>(tile_ops);
+    if (res_stride >= 64) return fused_fn<4, true>(tile_ops);
+    if (res_stride >= 32) return fused_fn<2, true>(tile_ops);
+    if (res_stride >= 16) return fused_fn<1, true>(tile_ops);
+    return fused_fn<1, false>(tile_ops);
+}
+
+uint32_t cc_fused_grid(uint32_t n_ops, uint32_t tile_ops, uint32_t grid) {
+    const uint32_t n_tiles = (n_ops + tile_ops - 1) / tile_ops;
+    if (n_tiles == 0 || grid == 0) return 0;
+    const uint32_t tpb = (n_tiles + grid - 1) / grid;
+    return (n_tiles + tpb - 1) / tpb;
+}
+
+hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile_ops,
+                           uint32_t grid, uint32_t *epoch_ctr, uint8_t *commit, uint8_t *results,
+                           uint32_t res_stride, uint32_t *counters, GridBar bar) {
+    const void *fn = cc_fused_kernel(tile_ops, res_stride);
+    if (!fn || ov.n_txns == 0 || ov.n_ops == 0 || grid == 0) return hipErrorInvalidValue;
+    uint32_t n_tiles = (ov.n_ops + tile_ops - 1) / tile_ops;
+    uint32_t tpb = (n_tiles + grid - 1) / grid;
+    if (tpb > kFusedMaxTiles) return hipErrorInvalidValue;
+    grid = (n_tiles + tpb - 1) / tpb;
+    void *args[] = {&dt, &ov, &n_tiles, &tpb, &epoch_ctr, &commit, &results, &res_stride,
+                    &counters, &bar};
+    return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, s);
 }
 
 }  // namespace ltpmg

@@ -258,15 +258,24 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
     }
 }
 
+// One tile of the sorted (cell, op) list.  Every READ gets its value: the
+// payload of the last UPDATE before it on the cell, else the snapshot, plus
+// the deltas of the RMW_ADDs in between.  A segment (one cell) that starts
+// and ends in this tile has its final value written here, after the block's
+// reads: no other tile reads that cell.  Only the tile's first segment can
+// have started in an earlier tile; if it ends here with a write, that write
+// waits for the next kernel (one deferred entry per tile, no global atomics).
 template <int G, bool VEC16>
 __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *cset,
                                                            const uint32_t *cdelta, DevTables dt,
                                                            KeyDecode dec, uint8_t *results,
-                                                           uint32_t res_stride, uint32_t *fin,
-                                                           uint32_t *fin_count) {
-    __shared__ uint4 list[kSegTile];  // {op, set, d, key}
-    __shared__ uint32_t n_list;
-    if (threadIdx.x == 0) n_list = 0;
+                                                           uint32_t res_stride, uint4 *deferred) {
+    // READs from the front {op, set, d, key}; in-tile final writes from the
+    // back {0, set, d, key}.  A write entry needs a write op in its segment,
+    // so reads + write entries <= tile elements and the two ends never meet.
+    __shared__ uint4 list[kSegTile];
+    __shared__ uint32_t n_rd, n_wr, cont_key;
+    __shared__ uint4 s_def;
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
     // the thread's 8 sorted elements, read once (two 16-byte loads when in range)
     uint32_t key[kSegItems], val[kSegItems];
@@ -288,6 +297,12 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // keys are < 2^31, so all-ones never equals a real key
     const uint32_t prev = (first > 0 && first - 1 < src.n) ? src.keys[first - 1] : 0xFFFFFFFFu;
     const uint32_t next = first + kSegItems < src.n ? src.keys[first + kSegItems] : 0xFFFFFFFFu;
+    if (threadIdx.x == 0) {
+        n_rd = 0;
+        n_wr = 0;
+        cont_key = prev == key[0] ? key[0] : 0xFFFFFFFFu;  // first segment began earlier
+        s_def = make_uint4(0xFFFFFFFFu, kNoSet, 0u, 0u);
+    }
     Seg agg{0u, kNoSet, 0u};
 #pragma unroll
     for (uint32_t j = 0; j < kSegItems; ++j) {
@@ -296,9 +311,11 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         bool h = key[j] != (j ? key[j - 1] : prev);
         agg = seg_op(agg, Seg{h ? 1u : 0u, s.set, s.d});
     }
+    __syncthreads();  // n_rd, n_wr, cont_key, s_def set before any lane uses them
     Seg tot;
     Seg pre = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, block_seg_exclusive(agg, &tot));
     St run{pre.set, pre.d};
+    const uint32_t ck = cont_key;
 #pragma unroll
     for (uint32_t j = 0; j < kSegItems; ++j) {
         uint32_t i = first + j;
@@ -316,51 +333,55 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
             wr = last && run.set != kNoSet;
         }
         const uint32_t k_j = key[j];
-        // READ -> LDS list (one LDS atomic per wave)
+        if (wr && k_j == ck) {  // end of a segment that began in an earlier tile
+            s_def = make_uint4(k_j, run.set, run.d, 0u);
+            wr = false;
+        }
         uint64_t m = __ballot(rd);
         if (m) {
             uint32_t leader = __builtin_ctzll(m), base = 0;
-            if (__lane_id() == leader) base = atomicAdd(&n_list, (uint32_t)__popcll(m));
+            if (__lane_id() == leader) base = atomicAdd(&n_rd, (uint32_t)__popcll(m));
             base = __shfl(base, (int)leader);
             if (rd) list[base + lanemask_lt_popc(m)] = make_uint4(op, ex.set, ex.d, k_j);
         }
-        // segment end with a write -> global final-write list
         uint64_t mw = __ballot(wr);
         if (mw) {
             uint32_t leader = __builtin_ctzll(mw), base = 0;
-            if (__lane_id() == leader) base = atomicAdd(fin_count, (uint32_t)__popcll(mw));
+            if (__lane_id() == leader) base = atomicAdd(&n_wr, (uint32_t)__popcll(mw));
             base = __shfl(base, (int)leader);
-            if (wr) {
-                uint32_t p = (base + lanemask_lt_popc(mw)) * 3;
-                fin[p] = k_j;
-                fin[p + 1] = run.set;
-                fin[p + 2] = run.d;
-            }
+            if (wr) list[kSegTile - 1 - (base + lanemask_lt_popc(mw))] = make_uint4(0u, run.set, run.d, k_j);
         }
     }
     __syncthreads();
-    move_cells<G, VEC16>(n_list, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+    if (threadIdx.x == 0) deferred[blockIdx.x] = s_def;
+    move_cells<G, VEC16>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[k];
         key_out = e.w;
         st = St{e.y, e.z};
         dst = results + (uint64_t)e.x * res_stride;
     }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G);
+    __syncthreads();  // every read of this tile's cells is done
+    move_cells<G, VEC16>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+        uint4 e = list[kSegTile - 1 - k];
+        key_out = e.w;
+        st = St{e.y, e.z};
+        dst = nullptr;  // the cell itself
+    }, dt, dec, src.args, 0u, false, threadIdx.x / G, kBlock / G);
 }
 
+// The deferred final writes, at most one per tile; an entry with an all-ones
+// key is empty and decodes to no cell.
 template <int G, bool VEC16>
-__global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint32_t *fin,
-                                                            const uint32_t *fin_count,
+__global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *deferred, uint32_t n,
                                                             DevTables dt, KeyDecode dec,
                                                             const uint64_t *args) {
-    const uint32_t n = *fin_count;
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    // each touched cell appears once in the list, so its load and store stay
-    // within one lane group
     move_cells<G, VEC16>(n, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
-        key = fin[3 * k];
-        st = St{fin[3 * k + 1], fin[3 * k + 2]};
-        dst = nullptr;  // the cell itself, set below
+        uint4 e = deferred[k];
+        key = e.x;
+        st = St{e.y, e.z};
+        dst = nullptr;
     }, dt, dec, args, 0u, false, g, groups);
 }
 
@@ -608,7 +629,6 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, u
 
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
     const uint32_t n = ov.n_ops;
-    launch_zero_u32(s, ob.counters, 2);
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
@@ -627,17 +647,17 @@ void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
     KeyDecode dec{dt.tabs, col_bits, n_tabs};
     LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.carry_set, ob.carry_delta, dt, dec,
-                      results, res_stride, ob.fin, ob.counters + 1);
+                      results, res_stride, ob.deferred);
 }
 
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
                             uint32_t col_bits, uint32_t n_tabs, uint32_t res_stride) {
-    const uint32_t n = ov.n_ops;
-    if (n == 0) return;
+    if (ov.n_ops == 0) return;
+    const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
     KeyDecode dec{dt.tabs, col_bits, n_tabs};
-    uint32_t blocks = (n + kBlock - 1) / kBlock;
-    if (blocks > 8192) blocks = 8192;
-    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.fin, ob.counters + 1, dt, dec, ov.args);
+    // one lane group per entry; res_stride >= every slot, so G lanes cover a cell
+    const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
+    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.deferred, n_tiles, dt, dec, ov.args);
 }
 
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,

@@ -1,5 +1,5 @@
-"""ctypes binding of the engine's C ABI (include/ltpmg.h) and of the workload
-generator (include/ltpmg_workload.h).
+"""ctypes binding of the engine's C ABI (include/ltpmg.h, include/ltpmg_stream.h)
+and of the workload generators (include/ltpmg_workload.h, include/ltpmg_oe.h).
 
 The engine runs only on the GPU: there is no CPU fallback, and loading fails
 with a RuntimeError when libltpmg_hip.so has not been built.
@@ -106,6 +106,33 @@ ENGINE_API = {
                                       C.POINTER(C.c_double)]),
 }
 
+class _StreamPlan(C.Structure):
+    _fields_ = [("merge_max_ops", C.c_uint32), ("fused_max_ops", C.c_uint32),
+                ("fused_tile_ops", C.c_uint32), ("fused_wgs_per_cu", C.c_uint32),
+                ("use_graph", C.c_uint32), ("flags", C.c_uint32)]
+
+
+STREAM_KEEP_RESULTS = 0x1
+PLAN_SYNC_EACH = 0x1
+
+# Every symbol include/ltpmg_stream.h declares (also in libltpmg_hip.so).
+STREAM_API = {
+    "ltpmg_stream_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.POINTER(_Batch), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "ltpmg_stream_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ltpmg_stream_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_StreamPlan),
+                                   C.POINTER(C.c_double)]),
+    "ltpmg_stream_tune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_StreamPlan),
+                                    C.POINTER(C.c_uint32)]),
+    "ltpmg_stream_default_plan": (None, [C.POINTER(_StreamPlan)]),
+    "ltpmg_stream_fetch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_uint64]),
+    "ltpmg_stream_committed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ltpmg_stream_groups": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_StreamPlan), C.c_void_p,
+                                      C.c_uint32]),
+}
+
+
 class _ServeCfg(C.Structure):
     _fields_ = [("arrival_rate", C.c_double), ("n_arrivals", C.c_uint32), ("max_batch", C.c_uint32),
                 ("min_batch", C.c_uint32), ("max_wait_us", C.c_double), ("n_inflight", C.c_uint32),
@@ -172,7 +199,7 @@ def _load(name: str, api: dict) -> C.CDLL:
 
 
 def hip_lib() -> C.CDLL:
-    return _load("libltpmg_hip.so", ENGINE_API)
+    return _load("libltpmg_hip.so", {**ENGINE_API, **STREAM_API})
 
 
 def host_lib() -> C.CDLL:
@@ -480,6 +507,135 @@ class Engine:
             if record_results:
                 out["results"] = keep["results"]
         return out
+
+
+@dataclass
+class StreamPlan:
+    """Plan of a batch stream (include/ltpmg_stream.h)."""
+    merge_max_ops: int = 1 << 24
+    fused_max_ops: int = 1 << 18
+    fused_tile_ops: int = 2048
+    fused_wgs_per_cu: int = 2
+    use_graph: bool = True
+    sync_each: bool = False
+
+    def c_struct(self) -> _StreamPlan:
+        return _StreamPlan(self.merge_max_ops, self.fused_max_ops, self.fused_tile_ops,
+                           self.fused_wgs_per_cu, int(self.use_graph),
+                           PLAN_SYNC_EACH if self.sync_each else 0)
+
+    @staticmethod
+    def from_c(p: _StreamPlan) -> "StreamPlan":
+        return StreamPlan(p.merge_max_ops, p.fused_max_ops, p.fused_tile_ops, p.fused_wgs_per_cu,
+                          bool(p.use_graph), bool(p.flags & PLAN_SYNC_EACH))
+
+    @staticmethod
+    def plain() -> "StreamPlan":
+        """Every batch alone, three-kernel / ordered pipeline, host wait after each."""
+        return StreamPlan(0, 0, 2048, 1, False, True)
+
+    @staticmethod
+    def default() -> "StreamPlan":
+        p = _StreamPlan()
+        hip_lib().ltpmg_stream_default_plan(C.byref(p))
+        return StreamPlan.from_c(p)
+
+
+def concat_batches(batches: list[Batch]) -> Batch:
+    """The batches back to back, txn offsets running over all of them."""
+    offs, base = [np.zeros(1, np.uint64)], 0
+    for b in batches:
+        offs.append(b.txn_off[1:].astype(np.uint64) + base)
+        base += b.n_ops
+    if base >= 1 << 32:
+        raise ValueError("a stream holds fewer than 2^32 ops")
+    return Batch(np.concatenate(offs).astype(np.uint32),
+                 np.concatenate([b.rows for b in batches]).astype(np.uint32),
+                 np.concatenate([b.meta for b in batches]).astype(np.uint16),
+                 np.concatenate([b.args for b in batches]).astype(np.uint64))
+
+
+class Stream:
+    """A stream of batches resident in HBM (the batching form)."""
+
+    def __init__(self, engine: "Engine", batches: list[Batch], modes: list[int] | int,
+                 keep_results: bool = False):
+        self.engine = engine
+        self.batch_txns = np.array([b.n_txns for b in batches], np.uint32)
+        self.batch_ops = np.array([b.n_ops for b in batches], np.int64)
+        m = [modes] * len(batches) if isinstance(modes, int) else list(modes)
+        self.modes = np.array(m, np.uint8)
+        self.keep_results = keep_results
+        allb = concat_batches(batches).normalized()
+        h = C.c_void_p()
+        lib = engine._lib
+        engine._check("ltpmg_stream_create",
+                      lib.ltpmg_stream_create(engine._h, len(batches), _ptr(self.batch_txns),
+                                              _ptr(self.modes), C.byref(allb.c_struct()),
+                                              STREAM_KEEP_RESULTS if keep_results else 0,
+                                              C.byref(h)))
+        self._h = h
+
+    @property
+    def n_batches(self) -> int:
+        return len(self.batch_txns)
+
+    def run(self, plan: StreamPlan | None = None) -> float:
+        """Runs every batch once; returns the device milliseconds."""
+        ms = C.c_double(0.0)
+        p = plan.c_struct() if plan is not None else None
+        self.engine._check("ltpmg_stream_run",
+                           self.engine._lib.ltpmg_stream_run(self.engine._h, self._h,
+                                                             C.byref(p) if p is not None else None,
+                                                             C.byref(ms)))
+        return ms.value
+
+    def tune(self) -> tuple[StreamPlan, int]:
+        p = _StreamPlan()
+        n = C.c_uint32(0)
+        self.engine._check("ltpmg_stream_tune",
+                           self.engine._lib.ltpmg_stream_tune(self.engine._h, self._h, C.byref(p),
+                                                              C.byref(n)))
+        return StreamPlan.from_c(p), n.value
+
+    def groups(self, plan: StreamPlan | None = None) -> list[int]:
+        p = plan.c_struct() if plan is not None else None
+        first = np.zeros(self.n_batches, np.uint32)
+        n = self.engine._lib.ltpmg_stream_groups(self.engine._h, self._h,
+                                                 C.byref(p) if p is not None else None,
+                                                 _ptr(first), self.n_batches)
+        if n < 0:
+            self.engine._check("ltpmg_stream_groups", n)
+        return first[:n].tolist()
+
+    def fetch(self, batch: int, results: bool = True) -> tuple[np.ndarray, np.ndarray | None]:
+        commit = np.empty(int(self.batch_txns[batch]), np.uint8)
+        res = (np.empty((int(self.batch_ops[batch]), self.engine.result_stride), np.uint8)
+               if results else None)
+        self.engine._check("ltpmg_stream_fetch",
+                           self.engine._lib.ltpmg_stream_fetch(
+                               self.engine._h, self._h, batch, _ptr(commit),
+                               _ptr(res) if res is not None else None,
+                               res.nbytes if res is not None else 0))
+        return commit, res
+
+    def committed(self) -> np.ndarray:
+        out = np.zeros(self.n_batches, np.uint32)
+        self.engine._check("ltpmg_stream_committed",
+                           self.engine._lib.ltpmg_stream_committed(self.engine._h, self._h,
+                                                                   _ptr(out)))
+        return out
+
+    def close(self) -> None:
+        if self._h and self.engine._h:
+            self.engine._lib.ltpmg_stream_destroy(self.engine._h, self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def take_txns(pool: Batch, ids: np.ndarray) -> Batch:

@@ -33,6 +33,30 @@ def test_engine_exports_every_declared_symbol():
     assert decl == set(eng.ENGINE_API)
 
 
+def test_stream_exports_every_declared_symbol():
+    decl = _declared(ROOT / "include" / "ltpmg_stream.h")
+    decl |= set(re.findall(r"^void (ltpmg_[a-z_0-9]+)\(", (ROOT / "include" / "ltpmg_stream.h").read_text(), re.M))
+    assert len(decl) >= 8
+    assert decl <= _exported(eng.LIB_DIR / "libltpmg_hip.so")
+    assert decl == set(eng.STREAM_API)
+
+
+def test_serving_and_order_entry_export_every_declared_symbol():
+    exported = _exported(eng.LIB_DIR / "libltpmg_host.so")
+    for header, api in (("ltpmg_serve.h", eng.SERVE_API), ("ltpmg_oe.h", eng.OE_API)):
+        decl = _declared(ROOT / "include" / header)
+        assert decl and decl <= exported
+        assert decl == set(api)
+
+
+def test_concat_batches_keeps_each_batch():
+    a = eng.Batch.from_txns([[(0, 0, 1, 5, 0)], [(1, 0, 2, 6, 9), (0, 0, 0, 7, 0)]])
+    b = eng.Batch.from_txns([[(2, 1, 3, 8, 4)]])
+    c = eng.concat_batches([a, b])
+    assert c.txn_off.tolist() == [0, 1, 3, 4]
+    assert c.rows.tolist() == [5, 6, 7, 8] and c.meta.tolist() == a.meta.tolist() + b.meta.tolist()
+
+
 def test_workload_exports_every_declared_symbol():
     decl = _declared(ROOT / "include" / "ltpmg_workload.h")
     assert decl <= _exported(eng.LIB_DIR / "libltpmg_host.so")
