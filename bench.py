@@ -75,6 +75,8 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
     n, t, w = c["n_ops"], c["n_txns"], c["width"]
     per = {
         "mo_keys": 14 * n,                     # rows 4 + meta 2 in, key 4 + val 4 out
+        "rs_hist_ops": 6 * n,                  # digit 0 from the ops: rows 4 + meta 2 in
+        "rs_scatter_ops": 14 * n,              # rows + meta in, key + val out
         "rs_hist": 4 * n,                      # keys in
         "rs_scatter": 16 * n,                  # key+val in and out
         "mo_seg_scan": 8 * n,                  # sorted key+val in
@@ -143,11 +145,13 @@ def run_mode(e, batch, mode: int, steps: int, warmup: int) -> dict:
             "exec_ms_last": st["exec_ms"], "upload_ms": st["upload_ms"]}
 
 
-def profile_mode(e, mode: int) -> dict:
-    """One extra step with per-kernel HIP events; stats() sums repeated stages."""
+def profile_mode(e, mode: int) -> tuple[dict, dict]:
+    """One extra step with per-kernel HIP events: total ms and launches per
+    stage (the radix-sort stages repeat once per digit)."""
     e.execute(mode, profile=True)
     e.wait()
-    return e.stats()["kernels"]
+    st = e.stats()
+    return st["kernels"], st["kernel_launches"]
 
 
 def serve_compare(e, pool, args) -> dict:
@@ -220,6 +224,8 @@ def batching_compare(args) -> dict:
         else:
             batches.append(oe.next(int(n)))
             kinds.append("oe")
+        if (i + 1) % 25 == 0:
+            log(f"batching stream: {i + 1}/{nb} batches generated")
     txns = int(sizes.sum())
     ops = int(sum(b.n_ops for b in batches))
     log(f"batching stream: {nb} batches, {txns} txns, {ops} ops, generated in "
@@ -347,7 +353,7 @@ def main() -> int:
         warm = args.warmup if mode == MODE_O else 1
         r = run_mode(e, batch, mode, steps, warm)
         log(f"{name}: {steps} steps in {r['wall_s']:.3f} s, committed {r['committed']}")
-        r["kernels_ms"] = profile_mode(e, mode)
+        r["kernels_ms"], r["launches"] = profile_mode(e, mode)
         commit, _ = e.fetch(results=False)
         r["commit"] = commit
         res[name] = r
@@ -357,9 +363,7 @@ def main() -> int:
     ms_per_step = 1000.0 * head["wall_s"] / args.steps
 
     # roofline of the dominant kernel of the headline mode
-    ker = head["kernels_ms"]
-    passes = (max(1, int(np.ceil(np.log2(args.rows)))) + 4 + 7) // 8
-    launches = {k: (passes if k.startswith("rs_") else 1) for k in ker}
+    ker, launches = head["kernels_ms"], head["launches"]
     avg = {k: ker[k] / launches[k] for k in ker}
     dom = max(avg, key=lambda k: avg[k])
     alg = algorithmic_bytes(dom, counts)
@@ -377,7 +381,7 @@ def main() -> int:
     for name, r in res.items():
         for k, ms in r["kernels_ms"].items():
             a = algorithmic_bytes(k, counts, r["commit"], batch)
-            l = passes if k.startswith("rs_") else 1
+            l = r["launches"][k]
             per_kernel[f"{name}/{k}"] = {
                 "ms": ms, "launches": l,
                 "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None,

@@ -45,8 +45,16 @@
  *     orderline rows.
  *   Stocklevel (read-only): d_next_o_id, 10 orderlines' ol_i_id and those
  *     items' s_quantity.
- * The customer-by-last-name path (the C_LAST index, gpuquery.cu:2641-2650)
- * is not built: Payment and Orderstatus always address the customer by id.
+ * Customer by last name (the C_LAST index of cuda/src/gpudatabase.cu:201-230
+ * and its scan in parse<Payment_Query>, gpuquery.cu:2641-2650): a Payment
+ * or Orderstatus with isName set addresses the customer with the lowest C_ID
+ * in its district whose last-name code equals C_LAST, or C_ID 0 when no
+ * customer has it, as the reference's first-match scan does.  A customer's
+ * code is the first little-endian u64 of its c_last cell, as
+ * ltpmg_fill_table(table_seed) writes it, mod LTPMG_OE_LAST_NAMES (the
+ * reference's 15-name vocabulary, src/database.cpp:312-316).  Names never
+ * change, so the lookup is resolved when the query is decomposed into ops,
+ * the step that does it in the reference.
  */
 #ifndef LTPMG_OE_H
 #define LTPMG_OE_H
@@ -60,6 +68,7 @@ extern "C" {
 #endif
 
 #define LTPMG_OE_TABLES 9
+#define LTPMG_OE_LAST_NAMES 15
 
 typedef struct {
     uint32_t n_txns;
@@ -72,11 +81,18 @@ typedef struct {
     uint32_t append_cap;      /* capacity of the append tables, in transactions */
     uint32_t append_next[4];  /* in/out: next row of neworder, history, order, orderline */
     uint64_t seed;
+    uint32_t byname_pct;      /* Payment / Orderstatus: percent by last name (the reference draws 15) */
+    uint64_t table_seed;      /* the seed the tables were filled with (last-name lookups) */
 } ltpmg_oe_spec;
 
 /* Fills out[9] with the table descriptors for W partitions and an append
  * capacity of append_cap transactions. */
 int ltpmg_oe_schema(uint32_t warehouses, uint32_t append_cap, ltpmg_table_desc out[LTPMG_OE_TABLES]);
+
+/* first[(w * 10 + d) * LTPMG_OE_LAST_NAMES + k] = the C_ID a lookup of
+ * last-name code k in district (w, d) finds, for tables filled from
+ * table_seed. */
+int ltpmg_oe_last_name_index(uint32_t warehouses, uint64_t table_seed, uint32_t *first);
 
 /* Upper bound on device ops of a batch. */
 uint64_t ltpmg_gen_oe_max_ops(const ltpmg_oe_spec *spec);

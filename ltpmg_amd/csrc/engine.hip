@@ -401,12 +401,20 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     if (rc != LTPMG_OK) return rc;
     DevTables dt{e->d_cols, e->d_tabs};
     const uint32_t n_tabs = n_tabs_span(e);
-    rec.begin("mo_keys");
-    launch_mo_keys(st, dt, ov, col_bits, ob.sort.keys[0], ob.sort.vals[0]);
-    rec.end();
     int sorted = 0;
-    if (ov.n_ops > 1) {
-        for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    if (ov.n_ops > 1 && key_bits > 0) {
+        // digit 0 straight from the ops: the unsorted keys are never stored
+        rec.begin("rs_hist_ops");
+        launch_rs_hist_ops(st, dt, ov, col_bits, ob.sort);
+        rec.end();
+        rec.begin("rs_scan");
+        launch_rs_scan(st, ob.sort, ov.n_ops);
+        rec.end();
+        rec.begin("rs_scatter_ops");
+        launch_rs_scatter_ops(st, dt, ov, col_bits, ob.sort);
+        rec.end();
+        sorted = 1;
+        for (uint32_t shift = 8; shift < key_bits; shift += 8) {
             rec.begin("rs_hist");
             launch_rs_hist(st, ob.sort, sorted, ov.n_ops, shift);
             rec.end();
@@ -418,6 +426,10 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
             rec.end();
             sorted ^= 1;
         }
+    } else {
+        rec.begin("mo_keys");
+        launch_mo_keys(st, dt, ov, col_bits, ob.sort.keys[0], ob.sort.vals[0]);
+        rec.end();
     }
     rec.begin("mo_seg_scan");
     launch_mo_seg_scan(st, ov, ob, sorted);

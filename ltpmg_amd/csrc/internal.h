@@ -24,6 +24,12 @@ constexpr uint32_t kTileOps = 2048;    // nominal ops per CC tile
 constexpr uint32_t kTileOpsMax = kTileOps + LTPMG_MAX_TXN_OPS;
 constexpr uint32_t kSortItems = 16;    // keys per thread in a radix-sort tile
 constexpr uint32_t kSortTile = kBlock * kSortItems;
+// Cell moves in flight per lane group (independent loads issued before the
+// stores of a round); a build-time knob for tuning sweeps.
+#ifndef LTPMG_MOVE_DEPTH
+#define LTPMG_MOVE_DEPTH 4
+#endif
+constexpr int kMoveDepth = LTPMG_MOVE_DEPTH;
 constexpr uint32_t kNoSet = 0xFFFFFFFFu;     // ordered-mode state: no write seen
 constexpr uint32_t kRmwOnly = 0xFFFFFFFEu;   // ordered-mode state: only RMW_ADDs seen
 
@@ -182,6 +188,10 @@ int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits)
 void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
 void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n);
 void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
+// the ordered mode's first pass (digit 0) straight from the ops: histogram,
+// then (after launch_rs_scan) the scatter into keys[1] / vals[1]
+void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b);
+void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b);
 void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
                            uint32_t *partials);
 size_t scan_partials_needed(uint32_t n);

@@ -71,7 +71,7 @@ __device__ inline void wave_append(bool pred, uint32_t v, uint32_t *list, uint32
 template <int G, bool VEC16, bool READS>
 __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &ov, DevTables dt,
                                 uint8_t *results, uint32_t res_stride) {
-    constexpr int R = 4;
+    constexpr int R = kMoveDepth;
     constexpr uint32_t groups = kBlock / G;
     const uint32_t g = threadIdx.x / G, lane = threadIdx.x % G, b0 = lane * 16;
     for (uint32_t base = g; base < n; base += groups * R) {

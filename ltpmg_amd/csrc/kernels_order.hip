@@ -107,9 +107,61 @@ __device__ inline Seg thread_aggregate(const Src &src, uint32_t first) {
     return a;
 }
 
-template <class Src>
-__global__ __launch_bounds__(kBlock) void k_seg_reduce(Src src, uint32_t *tflag, uint32_t *tset,
-                                                       uint32_t *tdelta) {
+// The thread's 8 consecutive sorted elements (two 16-byte loads each for
+// keys and vals when in range), and the key before them.  Positions past n
+// read as all-ones keys / invalid vals; keys are < 2^31, so all-ones never
+// equals a real key.
+__device__ inline void load8(const OrderSrc &src, uint32_t first, uint32_t *key, uint32_t *val,
+                             uint32_t &prev) {
+    if (first + kSegItems <= src.n) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(src.keys + first);
+        const uint4 *v4 = reinterpret_cast<const uint4 *>(src.vals + first);
+        uint4 a = k4[0], b = k4[1], c = v4[0], d = v4[1];
+        key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
+        key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
+        val[0] = c.x; val[1] = c.y; val[2] = c.z; val[3] = c.w;
+        val[4] = d.x; val[5] = d.y; val[6] = d.z; val[7] = d.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kSegItems; ++j) {
+            key[j] = first + j < src.n ? src.keys[first + j] : 0xFFFFFFFFu;
+            val[j] = first + j < src.n ? src.vals[first + j] : kInvalid;
+        }
+    }
+    prev = (first > 0 && first - 1 < src.n) ? src.keys[first - 1] : 0xFFFFFFFFu;
+}
+
+__device__ inline Seg agg8(const OrderSrc &src, uint32_t first, const uint32_t *key,
+                           const uint32_t *val, uint32_t prev) {
+    Seg agg{0u, kNoSet, 0u};
+#pragma unroll
+    for (uint32_t j = 0; j < kSegItems; ++j) {
+        if (first + j >= src.n) break;
+        St s = elem_state(val[j], src.args);
+        bool h = key[j] != (j ? key[j - 1] : prev);
+        agg = seg_op(agg, Seg{h ? 1u : 0u, s.set, s.d});
+    }
+    return agg;
+}
+
+// Tile aggregates of the sorted (cell, op) list.
+__global__ __launch_bounds__(kBlock) void k_seg_reduce(OrderSrc src, uint32_t *tflag,
+                                                       uint32_t *tset, uint32_t *tdelta) {
+    const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
+    uint32_t key[kSegItems], val[kSegItems], prev;
+    load8(src, first, key, val, prev);
+    Seg tot;
+    block_seg_exclusive(agg8(src, first, key, val, prev), &tot);
+    if (threadIdx.x == 0) {
+        tflag[blockIdx.x] = tot.f;
+        tset[blockIdx.x] = tot.set;
+        tdelta[blockIdx.x] = tot.d;
+    }
+}
+
+// The self-test's tile aggregates (explicit head flags).
+__global__ __launch_bounds__(kBlock) void k_selftest_reduce(SelftestSrc src, uint32_t *tflag,
+                                                            uint32_t *tset, uint32_t *tdelta) {
     uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
     Seg tot;
     block_seg_exclusive(thread_aggregate(src, first), &tot);
@@ -120,22 +172,43 @@ __global__ __launch_bounds__(kBlock) void k_seg_reduce(Src src, uint32_t *tflag,
     }
 }
 
-// One block: carry[k] = segmented fold of tile aggregates 0..k-1.
+// One block: carry[k] = segmented fold of tile aggregates 0..k-1.  Rounds of
+// 2048 tiles are staged through LDS with coalesced loads; each thread folds
+// 8 contiguous tiles, one block scan per round, and the fold of all earlier
+// rounds is carried in.
 __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
                                                       const uint32_t *tdelta, uint32_t n_tiles,
                                                       uint32_t *cset, uint32_t *cdelta) {
-    // thread t folds the contiguous tiles [lo, hi), one block-wide scan of
-    // the folds, then each thread writes its tiles' carries
-    const uint32_t per = (n_tiles + kBlock - 1) / kBlock;
-    const uint32_t lo = min(n_tiles, threadIdx.x * per), hi = min(n_tiles, lo + per);
-    Seg agg{0u, kNoSet, 0u};
-    for (uint32_t k = lo; k < hi; ++k) agg = seg_op(agg, Seg{tflag[k], tset[k], tdelta[k]});
-    Seg tot;
-    Seg run = block_seg_exclusive(agg, &tot);
-    for (uint32_t k = lo; k < hi; ++k) {
-        cset[k] = run.set;
-        cdelta[k] = run.d;
-        run = seg_op(run, Seg{tflag[k], tset[k], tdelta[k]});
+    constexpr uint32_t kPer = 8, kRound = kBlock * kPer;
+    __shared__ uint32_t sf[kRound], ss[kRound], sd[kRound];
+    Seg carry{0u, kNoSet, 0u};
+    for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
+        const uint32_t n = min(kRound, n_tiles - c0);
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+            sf[i] = tflag[c0 + i];
+            ss[i] = tset[c0 + i];
+            sd[i] = tdelta[c0 + i];
+        }
+        __syncthreads();
+        const uint32_t lo = min(n, threadIdx.x * kPer), hi = min(n, lo + kPer);
+        Seg agg{0u, kNoSet, 0u};
+        for (uint32_t k = lo; k < hi; ++k) agg = seg_op(agg, Seg{sf[k], ss[k], sd[k]});
+        Seg tot;
+        Seg run = seg_op(carry, block_seg_exclusive(agg, &tot));
+        // each thread overwrites only its own tiles' entries with their carries
+        for (uint32_t k = lo; k < hi; ++k) {
+            const Seg x{sf[k], ss[k], sd[k]};
+            ss[k] = run.set;
+            sd[k] = run.d;
+            run = seg_op(run, x);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+            cset[c0 + i] = ss[i];
+            cdelta[c0 + i] = sd[i];
+        }
+        carry = seg_op(carry, tot);
+        __syncthreads();  // the next round overwrites sf / ss / sd
     }
 }
 
@@ -193,7 +266,7 @@ template <int G, bool VEC16, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
                                   const uint64_t *args, uint32_t res_span, bool to_result,
                                   uint32_t g, uint32_t groups) {
-    constexpr int R = 4;
+    constexpr int R = kMoveDepth;
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
     for (uint32_t base = g; base < n; base += groups * R) {
         uint32_t key[R];
@@ -277,25 +350,8 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     __shared__ uint32_t n_rd, n_wr, cont_key;
     __shared__ uint4 s_def;
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
-    // the thread's 8 sorted elements, read once (two 16-byte loads when in range)
-    uint32_t key[kSegItems], val[kSegItems];
-    if (first + kSegItems <= src.n) {
-        const uint4 *k4 = reinterpret_cast<const uint4 *>(src.keys + first);
-        const uint4 *v4 = reinterpret_cast<const uint4 *>(src.vals + first);
-        uint4 a = k4[0], b = k4[1], c = v4[0], d = v4[1];
-        key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
-        key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
-        val[0] = c.x; val[1] = c.y; val[2] = c.z; val[3] = c.w;
-        val[4] = d.x; val[5] = d.y; val[6] = d.z; val[7] = d.w;
-    } else {
-#pragma unroll
-        for (uint32_t j = 0; j < kSegItems; ++j) {
-            key[j] = first + j < src.n ? src.keys[first + j] : 0xFFFFFFFFu;
-            val[j] = first + j < src.n ? src.vals[first + j] : kInvalid;
-        }
-    }
-    // keys are < 2^31, so all-ones never equals a real key
-    const uint32_t prev = (first > 0 && first - 1 < src.n) ? src.keys[first - 1] : 0xFFFFFFFFu;
+    uint32_t key[kSegItems], val[kSegItems], prev;
+    load8(src, first, key, val, prev);
     const uint32_t next = first + kSegItems < src.n ? src.keys[first + kSegItems] : 0xFFFFFFFFu;
     if (threadIdx.x == 0) {
         n_rd = 0;
@@ -303,14 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         cont_key = prev == key[0] ? key[0] : 0xFFFFFFFFu;  // first segment began earlier
         s_def = make_uint4(0xFFFFFFFFu, kNoSet, 0u, 0u);
     }
-    Seg agg{0u, kNoSet, 0u};
-#pragma unroll
-    for (uint32_t j = 0; j < kSegItems; ++j) {
-        if (first + j >= src.n) break;
-        St s = elem_state(val[j], src.args);
-        bool h = key[j] != (j ? key[j - 1] : prev);
-        agg = seg_op(agg, Seg{h ? 1u : 0u, s.set, s.d});
-    }
+    const Seg agg = agg8(src, first, key, val, prev);
     __syncthreads();  // n_rd, n_wr, cont_key, s_def set before any lane uses them
     Seg tot;
     Seg pre = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, block_seg_exclusive(agg, &tot));
@@ -387,9 +436,36 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *deferre
 
 // ---------------- radix sort ----------------
 
-__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *keys, uint32_t n,
-                                                    uint32_t shift, uint32_t n_tiles,
-                                                    uint32_t *hist) {
+// Sort inputs.  SortedSrc: the key and value arrays of the previous pass.
+// OpSrc: the first pass of the ordered mode reads the ops themselves and
+// builds key = (global row << col_bits) | col and val = op | invalid << 29 |
+// kind << 30 on the fly, so the keys are never written out unsorted.
+struct SortedSrc {
+    const uint32_t *k, *v;
+    __device__ void kv(uint32_t i, uint32_t &key, uint32_t &val) const {
+        key = k[i];
+        val = v[i];
+    }
+};
+struct OpSrc {
+    const TabDesc *tabs;
+    const uint32_t *rows;
+    const uint16_t *meta;
+    uint32_t col_bits;
+    __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val) const {
+        const uint16_t m = meta[o];
+        const TabDesc &td = tabs[op_tab(m)];
+        const uint32_t row = rows[o];
+        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
+                        op_col(m) < td.n_cols;
+        key = ok ? ((td.row_base + row) << col_bits) | op_col(m) : 0u;
+        val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
+                                                    uint32_t n_tiles, uint32_t *hist) {
     __shared__ uint32_t h[kBlock / 64][256];
     for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&h[0][0])[i] = 0;
     __syncthreads();
@@ -397,7 +473,11 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *keys, uint32
     const uint32_t base = blockIdx.x * kSortTile;
     for (uint32_t it = 0; it < kSortItems; ++it) {
         uint32_t idx = base + it * kBlock + threadIdx.x;
-        if (idx < n) atomicAdd(&h[w][(keys[idx] >> shift) & 255u], 1u);
+        if (idx < n) {
+            uint32_t key, val;
+            src.kv(idx, key, val);
+            atomicAdd(&h[w][(key >> shift) & 255u], 1u);
+        }
     }
     __syncthreads();
     uint32_t d = threadIdx.x;  // kBlock == 256 digits
@@ -410,8 +490,8 @@ __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total);
 
 // Stable scatter: wave w of tile k ranks its 16 chunks of 64 keys in order;
 // lanes holding the same digit are found with 8 ballots (wave64 multisplit).
-__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin,
-                                                       uint32_t *kout, uint32_t *vout,
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, uint32_t *vout,
                                                        uint32_t n, uint32_t shift,
                                                        uint32_t n_tiles, const uint32_t *offs) {
     __shared__ uint32_t cnt[kBlock / 64][256];
@@ -428,7 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, cons
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
         bool valid = idx < n;
-        uint32_t key = valid ? kin[idx] : 0u;
+        uint32_t key = 0u, val = 0u;
+        if (valid) src.kv(idx, key, val);
         uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -445,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *kin, cons
         prior = __shfl(prior, (int)leader);
         r[c] = prior + lanemask_lt_popc(peers);
         k[c] = key;
-        v[c] = valid ? vin[idx] : 0u;
+        v[c] = val;
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -578,8 +659,8 @@ void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uin
 
 void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_hist, dim3(n_tiles), dim3(kBlock), 0, s, b.keys[cur], n, shift,
-                       n_tiles, b.hist);
+    hipLaunchKernelGGL(k_rs_hist<SortedSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
+                       SortedSrc{b.keys[cur], b.vals[cur]}, n, shift, n_tiles, b.hist);
 }
 
 void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n) {
@@ -589,8 +670,22 @@ void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n) {
 
 void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_scatter, dim3(n_tiles), dim3(kBlock), 0, s, b.keys[cur], b.vals[cur],
-                       b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, n_tiles, b.hist);
+    hipLaunchKernelGGL(k_rs_scatter<SortedSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
+                       SortedSrc{b.keys[cur], b.vals[cur]}, b.keys[cur ^ 1], b.vals[cur ^ 1], n,
+                       shift, n_tiles, b.hist);
+}
+
+void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b) {
+    const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(k_rs_hist<OpSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
+                       OpSrc{dt.tabs, ov.rows, ov.meta, col_bits}, n, 0u, n_tiles, b.hist);
+}
+
+void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b) {
+    const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(k_rs_scatter<OpSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
+                       OpSrc{dt.tabs, ov.rows, ov.meta, col_bits}, b.keys[1], b.vals[1], n, 0u,
+                       n_tiles, b.hist);
 }
 
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits) {
@@ -632,8 +727,8 @@ void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
-    hipLaunchKernelGGL(k_seg_reduce<OrderSrc>, dim3(n_tiles), dim3(kBlock), 0, s, src,
-                       ob.tile_flag, ob.tile_set, ob.tile_delta);
+    hipLaunchKernelGGL(k_seg_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, ob.tile_flag,
+                       ob.tile_set, ob.tile_delta);
     hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, ob.tile_flag, ob.tile_set,
                        ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta);
 }
@@ -667,8 +762,7 @@ void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t 
     uint32_t *tf = scratch, *ts = scratch + n_tiles, *td = scratch + 2 * n_tiles;
     uint32_t *cs = scratch + 3 * n_tiles, *cd = scratch + 4 * n_tiles;
     SelftestSrc src{vals, head, n};
-    hipLaunchKernelGGL(k_seg_reduce<SelftestSrc>, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts,
-                       td);
+    hipLaunchKernelGGL(k_selftest_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts, td);
     hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, tf, ts, td, n_tiles, cs, cd);
     hipLaunchKernelGGL(k_selftest_apply, dim3(n_tiles), dim3(kBlock), 0, s, src, cs, cd, out);
 }

@@ -164,11 +164,13 @@ class _OeSpec(C.Structure):
     _fields_ = [("n_txns", C.c_uint32), ("warehouses", C.c_uint32), ("neworder_pct", C.c_uint32),
                 ("payment_pct", C.c_uint32), ("lines_min", C.c_uint32), ("lines_max", C.c_uint32),
                 ("remote_bp", C.c_uint32), ("append_cap", C.c_uint32),
-                ("append_next", C.c_uint32 * 4), ("seed", C.c_uint64)]
+                ("append_next", C.c_uint32 * 4), ("seed", C.c_uint64),
+                ("byname_pct", C.c_uint32), ("table_seed", C.c_uint64)]
 
 
 OE_API = {
     "ltpmg_oe_schema": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ltpmg_oe_last_name_index": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p]),
     "ltpmg_gen_oe_max_ops": (C.c_uint64, [C.POINTER(_OeSpec)]),
     "ltpmg_gen_oe": (C.c_int, [C.POINTER(_OeSpec), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32]),
@@ -291,9 +293,12 @@ class OeStream:
 
     def __init__(self, warehouses: int, append_cap: int, *, neworder_pct: int = 45,
                  payment_pct: int = 43, lines: tuple[int, int] = (5, 15), remote_bp: int = 100,
-                 seed: int = 1):
+                 seed: int = 1, byname_pct: int = 0, table_seed: int = 0):
+        """byname_pct of Payments and Orderstatus select the customer by last
+        name, resolved against tables filled from table_seed."""
         self.spec = _OeSpec(0, warehouses, neworder_pct, payment_pct, lines[0], lines[1], remote_bp,
-                            append_cap, (C.c_uint32 * 4)(0, 0, 0, 0), seed & 0xFFFFFFFFFFFFFFFF)
+                            append_cap, (C.c_uint32 * 4)(0, 0, 0, 0), seed & 0xFFFFFFFFFFFFFFFF,
+                            byname_pct, table_seed & 0xFFFFFFFFFFFFFFFF)
         self.batches = 0
 
     def next(self, n_txns: int, n_threads: int = 0) -> Batch:
@@ -313,6 +318,15 @@ class OeStream:
         self.batches += 1
         k = n.value
         return Batch(off, rows[:k].copy(), metas[:k].copy(), args[:k].copy())
+
+
+def oe_last_name_index(warehouses: int, table_seed: int) -> np.ndarray:
+    """[W * 10, 15] C_IDs a last-name lookup finds (include/ltpmg_oe.h)."""
+    out = np.zeros(warehouses * 10 * 15, np.uint32)
+    rc = host_lib().ltpmg_oe_last_name_index(warehouses, table_seed & 0xFFFFFFFFFFFFFFFF, _ptr(out))
+    if rc != 0:
+        raise LtpmgError("ltpmg_oe_last_name_index", rc, "bad arguments")
+    return out.reshape(warehouses * 10, 15)
 
 
 def zeta(n: int, theta: float) -> float:
@@ -435,14 +449,15 @@ class Engine:
     def stats(self, slot: int = 0) -> dict:
         s = _Stats()
         self._check("ltpmg_stats", self._lib.ltpmg_stats(self._h, slot, C.byref(s)))
-        kernels = {}
+        kernels, launches = {}, {}
         for i in range(s.n_kernels):
             name = bytes(s.kernel_name[i]).split(b"\0", 1)[0].decode()
             kernels[name] = kernels.get(name, 0.0) + float(s.kernel_ms[i])
+            launches[name] = launches.get(name, 0) + 1
         return {"n_txns": s.n_txns, "n_ops": s.n_ops, "committed": s.committed,
                 "aborted": s.aborted, "mode": s.mode, "exec_ms": float(s.exec_ms),
                 "upload_ms": float(s.upload_ms), "kernels": kernels,
-                "hbm_allocated": int(s.hbm_allocated)}
+                "kernel_launches": launches, "hbm_allocated": int(s.hbm_allocated)}
 
     def run(self, batch: Batch, mode: int, slot: int = 0, results: bool = True):
         self.submit(batch, slot)

@@ -118,7 +118,7 @@ void pack(const std::vector<std::any> &batch, KV_Batch &out) {
     }
     if (g.info.empty()) LOG(FATAL) << "transfer_query_to_GPU before transfer_database_to_GPU";
     Packer p{out, n_cols(g.info[0]), g.seed ^ 0x5943534Full};
-    oe::Sink sink{out.rows, out.meta, out.args, g.seed ^ 0x4F45ull};
+    oe::Sink sink{out.rows, out.meta, out.args, g.seed ^ 0x4F45ull, g.seed};
     out.txn_off.push_back(0);
     for (const std::any &a : batch) {
         if (pack_ycsb<YCSB_A_Query>(a, p, 'A') || pack_ycsb<YCSB_B_Query>(a, p, 'B') ||

@@ -34,17 +34,27 @@ namespace i { enum : uint32_t { ID, IM_ID, PRICE, NAME, DATA }; }
 
 constexpr uint32_t kDistricts = 10, kCustomers = 3000, kStock = 100000, kItems = 100000;
 constexpr uint32_t kMaxLines = 15;
+constexpr uint32_t kLastNames = LTPMG_OE_LAST_NAMES;
 
 inline uint32_t district_row(uint32_t w, uint32_t dd) { return w * kDistricts + dd; }
 inline uint32_t customer_row(uint32_t w, uint32_t dd, uint32_t cc) { return district_row(w, dd) * kCustomers + cc; }
 inline uint32_t stock_row(uint32_t w, uint32_t item) { return w * kStock + item; }
+
+// Last-name code of a customer row: the first little-endian u64 of its
+// c_last cell, as ltpmg_fill_table(table_seed) writes it, mod kLastNames.
+uint32_t last_name_code(uint64_t table_seed, uint32_t customer_row);
+// The customer of district (w, dd) that a lookup by last-name code k finds:
+// the lowest C_ID with that code, or C_ID 0 when none has it (the
+// reference's index scan, cuda/src/gpuquery.cu:2641-2650).
+uint32_t customer_by_last_name(uint64_t table_seed, uint32_t w, uint32_t dd, uint32_t k);
 
 // Collects ops of one batch in SoA form.
 struct Sink {
     std::vector<uint32_t> &rows;
     std::vector<uint16_t> &meta;
     std::vector<uint64_t> &args;
-    uint64_t seed;  // payload seeds of inserted cells
+    uint64_t seed;        // payload seeds of inserted cells
+    uint64_t table_seed;  // the tables' fill seed (last-name lookups)
     void put(uint32_t kind, uint32_t table, uint32_t col, uint32_t row, uint64_t arg) {
         rows.push_back(row);
         meta.push_back(LTPMG_META(kind, table, col));
@@ -84,6 +94,7 @@ void set_slots(Payment_Query &q, const Slots &s);
 // Query builders drawing from a transaction's own random stream.
 struct Mix {
     uint32_t W, lines_min, lines_max, remote_bp, cap;
+    uint32_t byname_pct;  // Payment / Orderstatus: percent that select the customer by last name
 };
 Neworder_Query make_neworder(ltpmg_host::Rng &r, const Mix &m);
 Payment_Query make_payment(ltpmg_host::Rng &r, const Mix &m);

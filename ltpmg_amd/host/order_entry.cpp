@@ -12,6 +12,18 @@ using ltpmg_host::Rng;
 
 namespace oe {
 
+uint32_t last_name_code(uint64_t table_seed, uint32_t row) {
+    const uint64_t id = ((uint64_t)row << 10) | ((uint64_t)CUSTOMER << 6) | c::LAST;
+    const uint64_t cell = ltpmg_host::mix64(table_seed + ltpmg_host::kGolden * (id + 1));
+    return (uint32_t)(ltpmg_host::mix64(cell + ltpmg_host::kGolden) % kLastNames);
+}
+
+uint32_t customer_by_last_name(uint64_t table_seed, uint32_t w, uint32_t dd, uint32_t k) {
+    for (uint32_t cc = 0; cc < kCustomers; ++cc)
+        if (last_name_code(table_seed, customer_row(w, dd, cc)) == k) return cc;
+    return 0;
+}
+
 void decompose(const Neworder_Query &q, Sink &out) {
     const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.W_ID, q.D_ID, q.C_ID);
     out.read(WAREHOUSE, w::TAX, q.W_ID);
@@ -38,7 +50,8 @@ void decompose(const Neworder_Query &q, Sink &out) {
 }
 
 void decompose(const Payment_Query &q, Sink &out) {
-    const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.C_W_ID, q.C_D_ID, q.C_ID);
+    const uint32_t cid = q.isName ? customer_by_last_name(out.table_seed, q.C_W_ID, q.C_D_ID, q.C_LAST) : q.C_ID;
+    const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.C_W_ID, q.C_D_ID, cid);
     out.add(WAREHOUSE, w::YTD, q.W_ID, (int32_t)q.H_AMOUNT);
     out.read(WAREHOUSE, w::NAME, q.W_ID);
     out.add(DISTRICT, d::YTD, dr, (int32_t)q.H_AMOUNT);
@@ -53,7 +66,8 @@ void decompose(const Payment_Query &q, Sink &out) {
 }
 
 void decompose(const Orderstatus_Query &q, Sink &out) {
-    const uint32_t cr = customer_row(q.W_ID, q.D_ID, q.C_ID);
+    const uint32_t cid = q.isName ? customer_by_last_name(out.table_seed, q.W_ID, q.D_ID, q.C_LAST) : q.C_ID;
+    const uint32_t cr = customer_row(q.W_ID, q.D_ID, cid);
     out.read(CUSTOMER, c::BALANCE, cr);
     out.read(CUSTOMER, c::FIRST, cr);
     out.read(CUSTOMER, c::LAST, cr);
@@ -120,6 +134,10 @@ Payment_Query make_payment(Rng &r, const Mix &m) {
     q.C_W_ID = q.W_ID;
     q.C_D_ID = q.D_ID;
     q.H_AMOUNT = (uint32_t)r.range(100, 500000);
+    if (m.byname_pct) {  // drawn only when asked, so by-id streams stay as they were
+        q.isName = r.below(100) < m.byname_pct ? 1u : 0u;
+        q.C_LAST = (uint32_t)r.below(kLastNames);
+    }
     return q;
 }
 
@@ -130,6 +148,10 @@ Orderstatus_Query make_orderstatus(Rng &r, const Mix &m) {
     q.C_ID = (uint32_t)r.nurand(1023, 0, kCustomers - 1);
     q.O_ID = (uint32_t)r.below(m.cap);
     q.OL_ID = (uint32_t)r.below((uint64_t)m.cap * kMaxLines - 5);
+    if (m.byname_pct) {
+        q.isName = r.below(100) < m.byname_pct ? 1u : 0u;
+        q.C_LAST = (uint32_t)r.below(kLastNames);
+    }
     return q;
 }
 
@@ -176,6 +198,15 @@ extern "C" int ltpmg_oe_schema(uint32_t W, uint32_t cap, ltpmg_table_desc out[LT
     return LTPMG_OK;
 }
 
+extern "C" int ltpmg_oe_last_name_index(uint32_t W, uint64_t table_seed, uint32_t *first) {
+    if (!first || W == 0 || (uint64_t)W * kDistricts * kCustomers > 0xFFFFFFFFull) return LTPMG_EINVAL;
+    for (uint32_t w = 0; w < W; ++w)
+        for (uint32_t dd = 0; dd < kDistricts; ++dd)
+            for (uint32_t k = 0; k < kLastNames; ++k)
+                first[(w * kDistricts + dd) * kLastNames + k] = customer_by_last_name(table_seed, w, dd, k);
+    return LTPMG_OK;
+}
+
 extern "C" uint64_t ltpmg_gen_oe_max_ops(const ltpmg_oe_spec *sp) {
     // a Neworder of 15 remote lines is the longest: 14 + 15 * 11 ops
     return sp ? (uint64_t)sp->n_txns * (14 + kMaxLines * 11) : 0;
@@ -185,7 +216,7 @@ extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows
                             uint64_t *args, uint64_t max_ops, uint32_t *n_ops_out, uint32_t n_threads) {
     if (!sp || !txn_off || !n_ops_out || sp->warehouses == 0 || sp->append_cap == 0 ||
         sp->neworder_pct + sp->payment_pct > 100 || sp->lines_min < 1 || sp->lines_max > kMaxLines ||
-        sp->lines_min > sp->lines_max || sp->remote_bp > 10000)
+        sp->lines_min > sp->lines_max || sp->remote_bp > 10000 || sp->byname_pct > 100)
         return LTPMG_EINVAL;
     if (n_threads == 0) n_threads = std::max(1u, std::thread::hardware_concurrency());
     const uint32_t W = sp->warehouses, cap = sp->append_cap;
@@ -199,9 +230,9 @@ extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows
     const ltpmg_oe_spec spec = *sp;
     auto gen = [&](uint32_t ci) {
         Chunk &ch = chunks[ci];
-        Sink sink{ch.rows, ch.meta, ch.args, spec.seed ^ 0x4F45ull};
+        Sink sink{ch.rows, ch.meta, ch.args, spec.seed ^ 0x4F45ull, spec.table_seed};
         const uint64_t t0 = (uint64_t)spec.n_txns * ci / n_chunks, t1 = (uint64_t)spec.n_txns * (ci + 1) / n_chunks;
-        const Mix mix{W, spec.lines_min, spec.lines_max, spec.remote_bp, cap};
+        const Mix mix{W, spec.lines_min, spec.lines_max, spec.remote_bp, cap, spec.byname_pct};
         for (uint64_t t = t0; t < t1; ++t) {
             Rng r(spec.seed, t);
             const size_t before = ch.rows.size();

@@ -64,6 +64,20 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
     }
     auto pool_txn = [&](uint32_t id) { return id % pool->n_txns; };
 
+    // size every slot for the largest batch before the clock starts, so no
+    // batch pays for device or pinned allocations while requests wait
+    {
+        uint32_t max_len = 1;
+        for (uint32_t t = 0; t < pool->n_txns; ++t)
+            max_len = std::max(max_len, pool->txn_off[t + 1] - pool->txn_off[t]);
+        const uint64_t cap_ops = std::min<uint64_t>((uint64_t)cfg->max_batch * max_len, (1u << 29) - 1);
+        for (uint32_t s = 0; s < cfg->n_inflight; ++s) {
+            ltpmg_staging st;
+            rc = ltpmg_stage_batch(e, s, cfg->max_batch, (uint32_t)cap_ops, &st);
+            if (rc != LTPMG_OK) return rc;
+        }
+    }
+
     std::deque<uint32_t> queue;
     std::vector<uint32_t> entries;  // arrival ids of all cut batches, in order
     std::vector<uint8_t> commit_buf, res_buf;

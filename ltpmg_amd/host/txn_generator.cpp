@@ -107,7 +107,8 @@ namespace {
 template <typename Q, typename Make>
 void gen_oe_queries(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, uint64_t tag,
                     Make make) {
-    const oe::Mix mix{param->warehouse_size, 5, 15, 100, param->oe_append_cap};
+    // 15 % of payments by last name, as src/txn_generator.cpp:346-351 draws them
+    const oe::Mix mix{param->warehouse_size, 5, 15, 100, param->oe_append_cap, 15};
     out.resize(size);
     for (size_t i = 0; i < size; ++i) {
         ltpmg_host::Rng r(param->seed ^ (tag << 48), i);
@@ -151,6 +152,8 @@ void Txn_Generator<KV_Batch>::generate_txn(std::shared_ptr<Param> param, size_t 
         sp.remote_bp = 100;
         sp.append_cap = param->oe_append_cap;
         sp.seed = param->seed ^ 0x4F45ull;
+        sp.byname_pct = 15;  // src/txn_generator.cpp:346-351
+        sp.table_seed = param->seed;  // the tables are filled from the run's seed
         KV_Batch b;
         uint64_t cap = ltpmg_gen_oe_max_ops(&sp);
         b.txn_off.resize(size + 1);

@@ -131,6 +131,21 @@ def test_segscan_random(n, p):
     assert (out == (expect & 0xFFFFFFFF).astype(np.uint32)).all()
 
 
+def test_segscan_many_tiles():
+    # more than 2048 tiles of 2048 elements: the carry kernel takes several rounds
+    n = 9_000_000
+    rng = np.random.default_rng(9)
+    vals = rng.integers(0, 1 << 20, n).astype(np.uint32)
+    heads = (rng.random(n) < 1e-4).astype(np.uint8)
+    heads[4_194_304] = 0  # a segment across the first round boundary of tiles
+    with Engine() as e:
+        out = e.selftest_segscan(vals, heads)
+    excl = np.cumsum(vals, dtype=np.uint64) - vals
+    start = np.maximum.accumulate(np.where((heads != 0) | (np.arange(n) == 0), np.arange(n), 0))
+    expect = (excl - excl[start]) & np.uint64(0xFFFFFFFF)
+    assert (out == expect.astype(np.uint32)).all()
+
+
 # ---------------- closed forms on the GPU ----------------
 def test_gpu_mode_a_closed_forms():
     schema = {0: (64, [4, 4])}
@@ -235,6 +250,15 @@ def test_c2_shape_reduced(mode):
     with Engine() as e:
         im = _setup(e, {0: (1_000_000, [100] * 10)}, 0x4C54504D47000004)
         _compare(e, im, b, mode)
+
+
+def test_ordered_mode_over_many_tiles():
+    # 4.8M ops: more than 2048 scan tiles, so segment carries cross rounds
+    b = gen_kv(300_000, 1_000_000, 4, ops=16, zipf_theta=0.99, read_pct=50, write_kind=OP_UPDATE,
+               seed=0x4C54504D47000005)
+    with Engine() as e:
+        im = _setup(e, {0: (1_000_000, [100] * 4)}, 0x4C54504D47000006)
+        _compare(e, im, b, MODE_O)
 
 
 def test_batch_sequence_and_determinism():

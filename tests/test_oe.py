@@ -7,9 +7,51 @@ import pytest
 
 import oracle_ffi as orc
 from ltpmg_amd import MODE_A, MODE_O, Engine
-from ltpmg_amd.engine import OP_READ, OP_RMW_ADD, OP_UPDATE, OeStream, oe_schema
+from ltpmg_amd.engine import OP_READ, OP_RMW_ADD, OP_UPDATE, OeStream, oe_last_name_index, oe_schema
 
 WAREHOUSE, DISTRICT, CUSTOMER, NEWORDER, HISTORY, ORDER, ORDERLINE, STOCK, ITEM = range(9)
+C_BALANCE, C_LAST = 7, 11
+N_NAMES = 15
+
+
+def _name_codes(seed: int, rows: np.ndarray) -> np.ndarray:
+    """Last-name code of customer rows from the oracle's fill of the c_last
+    column: first little-endian u64 of the cell mod 15."""
+    im = orc.Image({CUSTOMER: (int(rows.max()) + 1, [4] * 11 + [32])})
+    im.fill_cells(seed, CUSTOMER, C_LAST, rows)
+    cells = im.cols[CUSTOMER][C_LAST][rows]
+    return (cells[:, :8].copy().view("<u8")[:, 0] % N_NAMES).astype(np.int64)
+
+
+def test_last_name_index_matches_the_filled_cells():
+    seed, W = 0x4C54504D47000007, 2
+    rows = np.arange(W * 10 * 3000, dtype=np.uint32)
+    codes = _name_codes(seed, rows).reshape(W * 10, 3000)
+    expect = np.zeros((W * 10, N_NAMES), np.uint32)
+    for dist in range(W * 10):
+        for k in range(N_NAMES):
+            hit = np.nonzero(codes[dist] == k)[0]
+            expect[dist, k] = hit[0] if hit.size else 0  # first match, C_ID 0 when none
+    assert (oe_last_name_index(W, seed) == expect).all()
+
+
+def test_payments_by_last_name_address_the_first_match():
+    seed, W = 0x4C54504D47000008, 3
+    first = oe_last_name_index(W, seed)
+    by_name = OeStream(W, 5000, neworder_pct=0, payment_pct=100, seed=5, byname_pct=100,
+                       table_seed=seed).next(600)
+    by_id = OeStream(W, 5000, neworder_pct=0, payment_pct=100, seed=5).next(600)
+
+    def customers(b):
+        sel = (((b.meta >> 8) & 15) == CUSTOMER) & (((b.meta >> 2) & 63) == C_BALANCE)
+        return b.rows[sel].astype(np.int64)
+
+    for b, want in ((by_name, True), (by_id, False)):
+        r = customers(b)
+        assert r.size == b.n_txns
+        dist, cid = r // 3000, r % 3000
+        hit = first[dist, _name_codes(seed, r.astype(np.uint32))] == cid
+        assert hit.all() if want else not hit.all()
 
 
 def test_schema_shapes():
@@ -81,7 +123,7 @@ def test_oracle_mode_a_matches_pairwise_on_small_batches():
 def test_gpu_order_entry_parity(mode):
     W, cap = 4, 30_000
     sch = oe_schema(W, cap)
-    st = OeStream(W, cap, seed=7)
+    st = OeStream(W, cap, seed=7, byname_pct=60, table_seed=11)
     with Engine() as e:
         for t, (n, widths) in sch.items():
             e.create_table(t, n, widths)
