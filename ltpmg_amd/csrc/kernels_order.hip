@@ -442,7 +442,9 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *deferre
 // kind << 30 on the fly, so the keys are never written out unsorted.
 struct SortedSrc {
     const uint32_t *k, *v;
-    __device__ void kv(uint32_t i, uint32_t &key, uint32_t &val) const {
+    struct Cache {};
+    __device__ void prepare(Cache &) const {}
+    __device__ void kv(uint32_t i, uint32_t &key, uint32_t &val, const Cache &) const {
         key = k[i];
         val = v[i];
     }
@@ -452,13 +454,24 @@ struct OpSrc {
     const uint32_t *rows;
     const uint16_t *meta;
     uint32_t col_bits;
-    __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val) const {
+    // the table descriptors in LDS, so a key costs no dependent global load
+    struct Cache {
+        uint32_t base[LTPMG_MAX_TABLES], n_rows[LTPMG_MAX_TABLES], n_cols[LTPMG_MAX_TABLES];
+    };
+    __device__ void prepare(Cache &c) const {
+        if (threadIdx.x < LTPMG_MAX_TABLES) {
+            const TabDesc &td = tabs[threadIdx.x];
+            c.base[threadIdx.x] = td.row_base;
+            c.n_rows[threadIdx.x] = td.live ? td.n_rows : 0u;
+            c.n_cols[threadIdx.x] = td.n_cols;
+        }
+    }
+    __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
         const uint16_t m = meta[o];
-        const TabDesc &td = tabs[op_tab(m)];
-        const uint32_t row = rows[o];
-        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
-                        op_col(m) < td.n_cols;
-        key = ok ? ((td.row_base + row) << col_bits) | op_col(m) : 0u;
+        const uint32_t t = op_tab(m), row = rows[o];
+        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
+                        op_col(m) < c.n_cols[t];
+        key = ok ? ((c.base[t] + row) << col_bits) | op_col(m) : 0u;
         val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 };
@@ -467,7 +480,9 @@ template <class Src>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
                                                     uint32_t n_tiles, uint32_t *hist) {
     __shared__ uint32_t h[kBlock / 64][256];
+    __shared__ typename Src::Cache cache;
     for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&h[0][0])[i] = 0;
+    src.prepare(cache);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t base = blockIdx.x * kSortTile;
@@ -475,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
         uint32_t idx = base + it * kBlock + threadIdx.x;
         if (idx < n) {
             uint32_t key, val;
-            src.kv(idx, key, val);
+            src.kv(idx, key, val, cache);
             atomicAdd(&h[w][(key >> shift) & 255u], 1u);
         }
     }
@@ -498,9 +513,11 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     __shared__ uint32_t gbase[256];
     __shared__ uint32_t tile_off[256];
     __shared__ uint32_t sk[kSortTile], sv[kSortTile];
+    __shared__ typename Src::Cache cache;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
     gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+    src.prepare(cache);
     __syncthreads();
     const uint32_t wbase = blockIdx.x * kSortTile + w * (kSortTile / (kBlock / 64));
     uint32_t k[kSortItems], v[kSortItems], r[kSortItems];
@@ -509,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
         uint32_t idx = wbase + c * 64 + lane;
         bool valid = idx < n;
         uint32_t key = 0u, val = 0u;
-        if (valid) src.kv(idx, key, val);
+        if (valid) src.kv(idx, key, val, cache);
         uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
