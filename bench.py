@@ -11,6 +11,17 @@ the device-resident table.  The headline mode is the ordered mode (every
 txn commits, each cell sees its ops in TID order); the lowest-TID-wins mode
 is reported beside it.
 
+Beside the headline line's fields it reports, from the same run:
+  serving              served vs plain batches on the same table (ordered
+                       mode, open-loop Poisson arrivals of the batch's txns)
+  batching             a stream of 200 uneven batches (1k-1M txns, 1-64 ops,
+                       key-value and order-entry) through the batching form
+                       vs plain per-batch launches
+  serving_large_schema served vs plain on W=1024 order entry + a 100M-row
+                       key-value table, arrivals from the order-entry mix
+  cpu_baseline         the threaded CPU restatement of the reference's
+                       pipeline on a sample of the same batch
+
 Prints one JSON line (see the field list at the end of main()).
 """
 from __future__ import annotations
@@ -261,6 +272,48 @@ def batching_compare(args) -> dict:
     return out
 
 
+def serve_c5(args) -> dict:
+    """Serving on the large schema: open-loop Poisson arrivals of single
+    transactions drawn from the order-entry mix (45% NewOrder 5-15 lines,
+    43% Payment, 12% read-only; 15% of Payments by last name) over W=1024
+    partitions, with a 100M-row x 10 x 100 B key-value table also resident
+    (about 190 GB of tables with 128-byte slots).  Ordered mode.  The
+    engine's HBM budget bounds the slots, which are sized before the clock
+    starts.  Forms: plain (fixed batches, one in flight) at two batch sizes,
+    served (continuous batching, two in flight)."""
+    from ltpmg_amd import MODE_O, Engine
+    from ltpmg_amd.engine import OeStream, oe_schema
+    W, cap, kv_rows = 1024, 1 << 20, args.c5_kv_rows
+    budget = int(args.c5_hbm_budget_gb * (1 << 30))
+    e = Engine(n_slots=2, hbm_budget=budget)
+    schema = dict(oe_schema(W, cap))
+    schema[9] = (kv_rows, [100] * 10)
+    t0 = time.perf_counter()
+    for t, (n, widths) in sorted(schema.items()):
+        e.create_table(t, n, widths)
+        e.fill_table(t, TABLE_SEED)
+    tables_gb = e.stats_hbm() / (1 << 30)
+    log(f"serving (large schema): {tables_gb:.1f} GiB of tables filled in {time.perf_counter() - t0:.1f} s")
+    pool = OeStream(W, cap, seed=BATCH_SEED + 9, byname_pct=15, table_seed=TABLE_SEED).next(args.c5_pool)
+    out = {"partitions": W, "kv_rows": kv_rows, "tables_gib": tables_gb, "hbm_budget_gib": args.c5_hbm_budget_gb,
+           "pool_txns": pool.n_txns, "pool_ops": pool.n_ops, "mode": "ordered", "rates": []}
+    for rate in args.c5_rates:
+        n = int(max(1 << 17, min(1 << 20, rate * args.c5_seconds)))
+        row = {"arrival_rate": rate, "arrivals": n}
+        forms = [(f"plain_{b}", dict(max_batch=b, min_batch=b, n_inflight=1)) for b in args.c5_plain_batches]
+        forms.append(("served", dict(max_batch=args.c5_served_max, max_wait_us=args.c5_wait_us, n_inflight=2)))
+        for name, kw in forms:
+            r = e.serve(pool, arrival_rate=rate, n_arrivals=n, mode=MODE_O, seed=11, **kw)
+            row[name] = {k: r[k] for k in ("committed_per_s", "p50_ms", "p99_ms", "p999_ms",
+                                           "mean_batch_txns", "batches", "host_pack_s", "device_ms")}
+            log(f"serve[large] {name} @ {rate:.3g}/s: {r['committed_per_s']:.3g} txn/s, "
+                f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, p999 {r['p999_ms']:.2f} ms")
+        out["rates"].append(row)
+    out["hbm_allocated_gib"] = e.stats_hbm() / (1 << 30)
+    e.close()
+    return out
+
+
 def pct(xs, q):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
@@ -308,6 +361,16 @@ def main() -> int:
     ap.add_argument("--plain-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-wait-us", type=float, default=2000.0)
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip serving on the large order-entry + key-value schema")
+    ap.add_argument("--c5-kv-rows", type=int, default=100_000_000)
+    ap.add_argument("--c5-hbm-budget-gb", type=float, default=240.0)
+    ap.add_argument("--c5-pool", type=int, default=1 << 20)
+    ap.add_argument("--c5-rates", type=float, nargs="*", default=[1e6, 5e6, 2e7, 5e7])
+    ap.add_argument("--c5-seconds", type=float, default=0.5, help="arrival window per rate")
+    ap.add_argument("--c5-plain-batches", type=int, nargs="*", default=[1 << 15, 1 << 17])
+    ap.add_argument("--c5-served-max", type=int, default=1 << 17)
+    ap.add_argument("--c5-wait-us", type=float, default=1000.0)
     ap.add_argument("--no-batching", action="store_true",
                     help="skip the batching-form stream comparison")
     ap.add_argument("--batching-batches", type=int, default=200)
@@ -395,6 +458,9 @@ def main() -> int:
     batching = None
     if not args.no_batching:
         batching = batching_compare(args)
+    serving_large = None
+    if not args.no_c5:
+        serving_large = serve_c5(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -428,7 +494,7 @@ def main() -> int:
         "modes": modes,
         "upload_ms": head["upload_ms"], "upload_wall_s": upload_wall,
         "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
-        "batching": batching, "batch_counts": counts,
+        "batching": batching, "serving_large_schema": serving_large, "batch_counts": counts,
     }
     print(json.dumps(out), flush=True)
     return 0

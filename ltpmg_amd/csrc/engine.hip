@@ -157,16 +157,15 @@ uint32_t n_tabs_span(const ltpmg_engine *e) {
 int sync_descriptors(ltpmg_engine *e) {
     ColDesc cols[LTPMG_MAX_TABLES * LTPMG_MAX_COLS] = {};
     TabDesc tabs[LTPMG_MAX_TABLES] = {};
-    uint64_t row_base = 0;
+    uint64_t cell_base = 0;  // past 2^32 - 1 cells only the lowest-TID-wins mode runs
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
         const Table &tb = e->tables[t];
         if (!tb.live) continue;
-        tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)row_base, 1u};
-        row_base += tb.n_rows;
+        tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u};
+        cell_base += (uint64_t)tb.n_rows * tb.n_cols;
         for (uint32_t c = 0; c < tb.n_cols; ++c)
             cols[t * LTPMG_MAX_COLS + c] = ColDesc{tb.col[c], tb.slot[c], tb.width[c]};
     }
-    if (row_base > 0xFFFFFFFFull) return fail(e, LTPMG_EINVAL, "more than 2^32 rows over all tables");
     HIP_TRY(e, hipMemcpy(e->d_cols, cols, sizeof cols, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->d_tabs, tabs, sizeof tabs, hipMemcpyHostToDevice));
     return LTPMG_OK;
@@ -180,8 +179,8 @@ void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
         dev_free(e, ob.sort.keys[i], ops * 4);
         dev_free(e, ob.sort.vals[i], ops * 4);
     }
-    dev_free(e, ob.sort.hist, 256 * st * 4);
-    dev_free(e, ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)) * 4);
+    dev_free(e, ob.sort.hist, (st << kMaxDigitBits) * 4);
+    dev_free(e, ob.sort.partials, scan_partials_needed((uint32_t)(st << kMaxDigitBits)) * 4);
     dev_free(e, ob.tile_flag, sg * 4 * 5);
     dev_free(e, ob.deferred, sg * 16);
     ob = OrderBufs{};
@@ -195,8 +194,8 @@ int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
         ALLOC(ob.sort.keys[i], ops);
         ALLOC(ob.sort.vals[i], ops);
     }
-    ALLOC(ob.sort.hist, 256 * st);
-    ALLOC(ob.sort.partials, scan_partials_needed((uint32_t)(256 * st)));
+    ALLOC(ob.sort.hist, st << kMaxDigitBits);
+    ALLOC(ob.sort.partials, scan_partials_needed((uint32_t)(st << kMaxDigitBits)));
     ALLOC(ob.tile_flag, sg * 5);
     ALLOC(ob.deferred, sg);
 #undef ALLOC
@@ -374,19 +373,17 @@ int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *t
     return LTPMG_OK;
 }
 
-// Key geometry of ordered mode: cell key = (global row << col_bits) | col.
-int order_key_bits(ltpmg_engine *e, uint32_t *col_bits, uint32_t *key_bits) {
-    uint64_t total_rows = 0;
-    uint32_t max_cols = 1;
+// Key geometry of ordered mode: cells are numbered table by table, row
+// major, so a key needs ceil(log2(cells)) bits; all-ones stays free as the
+// kernels' end-of-list sentinel.
+int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
+    uint64_t cells = 0;
     for (const Table &t : e->tables)
-        if (t.live) {
-            total_rows += t.n_rows;
-            max_cols = std::max(max_cols, t.n_cols);
-        }
-    *col_bits = ceil_log2(max_cols);
-    *key_bits = ceil_log2(total_rows) + *col_bits;
-    if (*key_bits > 31)
-        return fail(e, LTPMG_EINVAL, "ordered mode needs <= 31 key bits (rows x columns), got %u", *key_bits);
+        if (t.live) cells += (uint64_t)t.n_rows * t.n_cols;
+    if (cells > 0xFFFFFFFFull)
+        return fail(e, LTPMG_EINVAL, "ordered mode numbers at most 2^32 - 1 cells; the tables hold %llu",
+                    (unsigned long long)cells);
+    *key_bits = ceil_log2(cells);
     return LTPMG_OK;
 }
 
@@ -396,39 +393,40 @@ int order_key_bits(ltpmg_engine *e, uint32_t *col_bits, uint32_t *key_bits) {
 template <class Rec, class Wait>
 int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &ob,
                   uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables) {
-    uint32_t col_bits = 0, key_bits = 0;
-    int rc = order_key_bits(e, &col_bits, &key_bits);
+    uint32_t key_bits = 0;
+    int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
     DevTables dt{e->d_cols, e->d_tabs};
     const uint32_t n_tabs = n_tabs_span(e);
     int sorted = 0;
     if (ov.n_ops > 1 && key_bits > 0) {
         // digit 0 straight from the ops: the unsorted keys are never stored
+        const uint32_t bits = radix_digit_bits(key_bits);
         rec.begin("rs_hist_ops");
-        launch_rs_hist_ops(st, dt, ov, col_bits, ob.sort);
+        launch_rs_hist_ops(st, dt, ov, ob.sort, bits);
         rec.end();
         rec.begin("rs_scan");
-        launch_rs_scan(st, ob.sort, ov.n_ops);
+        launch_rs_scan(st, ob.sort, ov.n_ops, bits);
         rec.end();
         rec.begin("rs_scatter_ops");
-        launch_rs_scatter_ops(st, dt, ov, col_bits, ob.sort);
+        launch_rs_scatter_ops(st, dt, ov, ob.sort, bits);
         rec.end();
         sorted = 1;
-        for (uint32_t shift = 8; shift < key_bits; shift += 8) {
+        for (uint32_t shift = bits; shift < key_bits; shift += bits) {
             rec.begin("rs_hist");
-            launch_rs_hist(st, ob.sort, sorted, ov.n_ops, shift);
+            launch_rs_hist(st, ob.sort, sorted, ov.n_ops, shift, bits);
             rec.end();
             rec.begin("rs_scan");
-            launch_rs_scan(st, ob.sort, ov.n_ops);
+            launch_rs_scan(st, ob.sort, ov.n_ops, bits);
             rec.end();
             rec.begin("rs_scatter");
-            launch_rs_scatter(st, ob.sort, sorted, ov.n_ops, shift);
+            launch_rs_scatter(st, ob.sort, sorted, ov.n_ops, shift, bits);
             rec.end();
             sorted ^= 1;
         }
     } else {
         rec.begin("mo_keys");
-        launch_mo_keys(st, dt, ov, col_bits, ob.sort.keys[0], ob.sort.vals[0]);
+        launch_mo_keys(st, dt, ov, ob.sort.keys[0], ob.sort.vals[0]);
         rec.end();
     }
     rec.begin("mo_seg_scan");
@@ -437,10 +435,10 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     rc = wait_tables();
     if (rc != LTPMG_OK) return rc;
     rec.begin("mo_apply_reads");
-    launch_mo_apply_reads(st, dt, ov, ob, sorted, col_bits, n_tabs, results, rs);
+    launch_mo_apply_reads(st, dt, ov, ob, sorted, n_tabs, results, rs);
     rec.end();
     rec.begin("mo_final_writes");
-    launch_mo_final_writes(st, dt, ov, ob, col_bits, n_tabs, rs);
+    launch_mo_final_writes(st, dt, ov, ob, n_tabs, rs);
     launch_fill_u8(st, commit, ov.n_txns, 1);
     rec.end();
     return LTPMG_OK;
@@ -871,8 +869,9 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
         rc = dev_alloc_n(e, &b.keys[i], std::max<size_t>(n, 1));
         if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.vals[i], std::max<size_t>(n, 1));
     }
-    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.hist, 256 * tiles);
-    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.partials, scan_partials_needed((uint32_t)(256 * tiles)));
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.hist, tiles << kMaxDigitBits);
+    if (rc == LTPMG_OK)
+        rc = dev_alloc_n(e, &b.partials, scan_partials_needed((uint32_t)(tiles << kMaxDigitBits)));
     if (rc == LTPMG_OK && n) {
         int cur = 0;
         if (hipMemcpy(b.keys[0], keys, 4ull * n, hipMemcpyHostToDevice) != hipSuccess ||
@@ -890,8 +889,8 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
         dev_free(e, b.keys[i], 4ull * std::max<size_t>(n, 1));
         dev_free(e, b.vals[i], 4ull * std::max<size_t>(n, 1));
     }
-    dev_free(e, b.hist, 1024 * tiles);
-    dev_free(e, b.partials, 4 * scan_partials_needed((uint32_t)(256 * tiles)));
+    dev_free(e, b.hist, (tiles << kMaxDigitBits) * 4);
+    dev_free(e, b.partials, 4 * scan_partials_needed((uint32_t)(tiles << kMaxDigitBits)));
     return rc;
 }
 

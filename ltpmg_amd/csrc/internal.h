@@ -65,7 +65,8 @@ struct TabDesc {
     unsigned long long *lock;
     uint32_t n_rows;
     uint32_t n_cols;
-    uint32_t row_base;  // first global row of this table in ordered-mode cell keys
+    uint32_t cell_base;  // ordered-mode key of (row 0, col 0): cells are numbered
+                         // table by table, row-major, key = cell_base + row * n_cols + col
     uint32_t live;
 };
 
@@ -178,20 +179,23 @@ hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile
 struct SortBufs {
     uint32_t *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;      // 256 * n_tiles
+    uint32_t *hist;      // 2^kMaxDigitBits * n_tiles
     uint32_t *partials;  // scan partials
 };
+constexpr uint32_t kMaxDigitBits = 9;  // radix digits of 8 or 9 bits
 // Sorts (keys[0], vals[0]) by the low key_bits bits, stably; returns which
 // buffer index holds the result.
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits);
+// digit width for key_bits: 9 when that saves a pass, else 8
+uint32_t radix_digit_bits(uint32_t key_bits);
 // one LSD pass, split so each kernel can be timed: hist, scan, scatter
-void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
-void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n);
-void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift);
+void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits);
+void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n, uint32_t bits);
+void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits);
 // the ordered mode's first pass (digit 0) straight from the ops: histogram,
 // then (after launch_rs_scan) the scatter into keys[1] / vals[1]
-void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b);
-void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b);
+void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits);
+void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits);
 void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
                            uint32_t *partials);
 size_t scan_partials_needed(uint32_t n);
@@ -205,15 +209,15 @@ struct OrderBufs {
     uint32_t *carry_delta;
     uint4 *deferred;        // per seg-scan tile: a final write that waits for every tile's reads
 };
-void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, uint32_t *keys,
+void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys,
                     uint32_t *vals);
 // ordered-mode stages, run in this order after launch_mo_keys + launch_radix_sort
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted);
 void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint32_t col_bits, uint32_t n_tabs, uint8_t *results,
+                           uint32_t n_tabs, uint8_t *results,
                            uint32_t res_stride);
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t col_bits, uint32_t n_tabs, uint32_t res_stride);
+                            uint32_t n_tabs, uint32_t res_stride);
 uint32_t seg_tiles(uint32_t n);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,

@@ -7,7 +7,7 @@
 // applies the list serially (2525-2700).  Under zipf keys the hottest row's
 // serial list is the tail.  Here the order is made data-parallel instead:
 //
-//   mo_keys       key = (global row << col_bits) | col, val = op | kind<<30
+//   mo_keys       key = cell number (table by table, row-major), val = op | kind<<30
 //   radix sort    stable LSD sort on key, 8-bit digits (ops are already in
 //                 TID order, so a stable sort yields per-cell TID order)
 //   seg reduce /  segmented scan of the per-cell state monoid
@@ -239,17 +239,17 @@ __global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, cons
 
 struct KeyDecode {
     const TabDesc *tabs;
-    uint32_t col_bits, n_tabs;
-    // global row -> (table, row); returns false when out of range
+    uint32_t n_tabs;
+    // cell key -> (table, row, col); returns false when out of range
     __device__ bool operator()(uint32_t key, uint32_t &tab, uint32_t &row, uint32_t &col) const {
-        uint32_t g = key >> col_bits;
-        col = key & ((1u << col_bits) - 1u);
         for (uint32_t t = 0; t < n_tabs; ++t) {
             const TabDesc &td = tabs[t];
-            if (td.live && g >= td.row_base && g - td.row_base < td.n_rows) {
+            const uint32_t off = key - td.cell_base;
+            if (td.live && key >= td.cell_base && (uint64_t)off < (uint64_t)td.n_rows * td.n_cols) {
                 tab = t;
-                row = g - td.row_base;
-                return col < td.n_cols;
+                row = off / td.n_cols;
+                col = off - row * td.n_cols;
+                return true;
             }
         }
         return false;
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *deferre
 
 // Sort inputs.  SortedSrc: the key and value arrays of the previous pass.
 // OpSrc: the first pass of the ordered mode reads the ops themselves and
-// builds key = (global row << col_bits) | col and val = op | invalid << 29 |
+// builds key = cell_base + row * n_cols + col and val = op | invalid << 29 |
 // kind << 30 on the fly, so the keys are never written out unsorted.
 struct SortedSrc {
     const uint32_t *k, *v;
@@ -453,7 +453,6 @@ struct OpSrc {
     const TabDesc *tabs;
     const uint32_t *rows;
     const uint16_t *meta;
-    uint32_t col_bits;
     // the table descriptors in LDS, so a key costs no dependent global load
     struct Cache {
         uint32_t base[LTPMG_MAX_TABLES], n_rows[LTPMG_MAX_TABLES], n_cols[LTPMG_MAX_TABLES];
@@ -461,7 +460,7 @@ struct OpSrc {
     __device__ void prepare(Cache &c) const {
         if (threadIdx.x < LTPMG_MAX_TABLES) {
             const TabDesc &td = tabs[threadIdx.x];
-            c.base[threadIdx.x] = td.row_base;
+            c.base[threadIdx.x] = td.cell_base;
             c.n_rows[threadIdx.x] = td.live ? td.n_rows : 0u;
             c.n_cols[threadIdx.x] = td.n_cols;
         }
@@ -471,17 +470,20 @@ struct OpSrc {
         const uint32_t t = op_tab(m), row = rows[o];
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
                         op_col(m) < c.n_cols[t];
-        key = ok ? ((c.base[t] + row) << col_bits) | op_col(m) : 0u;
+        key = ok ? c.base[t] + row * c.n_cols[t] + op_col(m) : 0u;
         val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 };
 
-template <class Src>
+// Digit histogram of one 4096-key tile: per-wave LDS counters, then one
+// column hist[d * n_tiles + tile] per digit (the scan runs digit-major).
+template <class Src, uint32_t BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
                                                     uint32_t n_tiles, uint32_t *hist) {
-    __shared__ uint32_t h[kBlock / 64][256];
+    constexpr uint32_t D = 1u << BITS;
+    __shared__ uint32_t h[kBlock / 64][D];
     __shared__ typename Src::Cache cache;
-    for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&h[0][0])[i] = 0;
+    for (uint32_t i = threadIdx.x; i < D * (kBlock / 64); i += kBlock) (&h[0][0])[i] = 0;
     src.prepare(cache);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6;
@@ -491,32 +493,37 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
         if (idx < n) {
             uint32_t key, val;
             src.kv(idx, key, val, cache);
-            atomicAdd(&h[w][(key >> shift) & 255u], 1u);
+            atomicAdd(&h[w][(key >> shift) & (D - 1u)], 1u);
         }
     }
     __syncthreads();
-    uint32_t d = threadIdx.x;  // kBlock == 256 digits
-    uint32_t c = 0;
-    for (uint32_t i = 0; i < kBlock / 64; ++i) c += h[i][d];
-    hist[(uint64_t)d * n_tiles + blockIdx.x] = c;
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < kBlock / 64; ++i) c += h[i][d];
+        hist[(uint64_t)d * n_tiles + blockIdx.x] = c;
+    }
 }
 
 __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total);
 
 // Stable scatter: wave w of tile k ranks its 16 chunks of 64 keys in order;
-// lanes holding the same digit are found with 8 ballots (wave64 multisplit).
-template <class Src>
+// lanes holding the same digit are found with BITS ballots (wave64
+// multisplit).  The tile is then staged in LDS in (digit, input) order and
+// each digit's run is written contiguously.
+template <class Src, uint32_t BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, uint32_t *vout,
                                                        uint32_t n, uint32_t shift,
                                                        uint32_t n_tiles, const uint32_t *offs) {
-    __shared__ uint32_t cnt[kBlock / 64][256];
-    __shared__ uint32_t gbase[256];
-    __shared__ uint32_t tile_off[256];
+    constexpr uint32_t D = 1u << BITS, PER = D / kBlock;  // digits per thread
+    static_assert(D % kBlock == 0, "whole digits per thread");
+    __shared__ uint32_t cnt[kBlock / 64][D];
+    __shared__ uint32_t gbase[D];
+    __shared__ uint32_t tile_off[D];
     __shared__ uint32_t sk[kSortTile], sv[kSortTile];
     __shared__ typename Src::Cache cache;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    for (uint32_t i = threadIdx.x; i < 256 * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
-    gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * n_tiles + blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < D * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) gbase[d] = offs[(uint64_t)d * n_tiles + blockIdx.x];
     src.prepare(cache);
     __syncthreads();
     const uint32_t wbase = blockIdx.x * kSortTile + w * (kSortTile / (kBlock / 64));
@@ -527,10 +534,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
         bool valid = idx < n;
         uint32_t key = 0u, val = 0u;
         if (valid) src.kv(idx, key, val, cache);
-        uint32_t d = (key >> shift) & 255u;
+        uint32_t d = (key >> shift) & (D - 1u);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (uint32_t b = 0; b < 8; ++b) {
+        for (uint32_t b = 0; b < BITS; ++b) {
             uint64_t bb = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
@@ -548,15 +555,28 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     }
     __syncthreads();
     {  // per digit: offsets of each wave inside the digit's run, and the run's
-       // start inside the tile (exclusive scan over digits)
-        uint32_t d = threadIdx.x, run = 0;
-        for (uint32_t i = 0; i < kBlock / 64; ++i) {
-            uint32_t t = cnt[i][d];
-            cnt[i][d] = run;
-            run += t;
+       // start inside the tile (exclusive scan over digits, PER consecutive
+       // digits per thread so thread order is digit order)
+        uint32_t runs[PER], mine = 0;
+#pragma unroll
+        for (uint32_t p = 0; p < PER; ++p) {
+            const uint32_t d = threadIdx.x * PER + p;
+            uint32_t run = 0;
+            for (uint32_t i = 0; i < kBlock / 64; ++i) {
+                uint32_t t = cnt[i][d];
+                cnt[i][d] = run;
+                run += t;
+            }
+            runs[p] = run;
+            mine += run;
         }
         uint32_t total;
-        tile_off[d] = block_exclusive_sum(run, &total);
+        uint32_t ex = block_exclusive_sum(mine, &total);
+#pragma unroll
+        for (uint32_t p = 0; p < PER; ++p) {
+            tile_off[threadIdx.x * PER + p] = ex;
+            ex += runs[p];
+        }
     }
     __syncthreads();  // every tile_off[d] written before any lane reads it
     // stage the tile in LDS in (digit, input) order ...
@@ -564,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
         if (idx < n) {
-            uint32_t d = (k[c] >> shift) & 255u;
+            uint32_t d = (k[c] >> shift) & (D - 1u);
             uint32_t lp = tile_off[d] + cnt[w][d] + r[c];
             sk[lp] = k[c];
             sv[lp] = v[c];
@@ -576,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     const uint32_t tile0 = blockIdx.x * kSortTile;
     const uint32_t tile_n = n - tile0 < kSortTile ? n - tile0 : kSortTile;
     for (uint32_t j = threadIdx.x; j < tile_n; j += kBlock) {
-        uint32_t key = sk[j], d = (key >> shift) & 255u;
+        uint32_t key = sk[j], d = (key >> shift) & (D - 1u);
         uint32_t pos = gbase[d] + (j - tile_off[d]);
         kout[pos] = key;
         vout[pos] = sv[j];
@@ -647,8 +667,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32
 
 // ---------------- mode O keys ----------------
 
-__global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uint32_t col_bits,
-                                                    uint32_t *keys, uint32_t *vals) {
+__global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uint32_t *keys,
+                                                    uint32_t *vals) {
     for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < ov.n_ops;
          o += gridDim.x * blockDim.x) {
         uint16_t m = ov.meta[o];
@@ -656,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
         uint32_t row = ov.rows[o];
         bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
                   op_col(m) < td.n_cols;
-        keys[o] = ok ? ((td.row_base + row) << col_bits) | op_col(m) : 0u;
+        keys[o] = ok ? td.cell_base + row * td.n_cols + op_col(m) : 0u;
         vals[o] = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 }
@@ -674,55 +694,65 @@ void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uin
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n, partials);
 }
 
-void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
+#define LTPMG_RS_BITS(BITS_VAR, KERNEL, SRC, ...)                                            \
+    do {                                                                                   \
+        if (BITS_VAR == 9)                                                                 \
+            hipLaunchKernelGGL((KERNEL<SRC, 9>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else                                                                               \
+            hipLaunchKernelGGL((KERNEL<SRC, 8>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+    } while (0)
+
+void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_hist<SortedSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
-                       SortedSrc{b.keys[cur], b.vals[cur]}, n, shift, n_tiles, b.hist);
+    LTPMG_RS_BITS(bits, k_rs_hist, SortedSrc, SortedSrc{b.keys[cur], b.vals[cur]}, n, shift, n_tiles,
+                  b.hist);
 }
 
-void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n) {
+void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n, uint32_t bits) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    launch_scan_exclusive(s, b.hist, b.hist, 256u * n_tiles, b.partials);
+    launch_scan_exclusive(s, b.hist, b.hist, (1u << bits) * n_tiles, b.partials);
 }
 
-void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift) {
+void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_scatter<SortedSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
-                       SortedSrc{b.keys[cur], b.vals[cur]}, b.keys[cur ^ 1], b.vals[cur ^ 1], n,
-                       shift, n_tiles, b.hist);
+    LTPMG_RS_BITS(bits, k_rs_scatter, SortedSrc, SortedSrc{b.keys[cur], b.vals[cur]}, b.keys[cur ^ 1],
+                  b.vals[cur ^ 1], n, shift, n_tiles, b.hist);
 }
 
-void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b) {
+void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
     const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_hist<OpSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
-                       OpSrc{dt.tabs, ov.rows, ov.meta, col_bits}, n, 0u, n_tiles, b.hist);
+    LTPMG_RS_BITS(bits, k_rs_hist, OpSrc, OpSrc{dt.tabs, ov.rows, ov.meta}, n, 0u, n_tiles, b.hist);
 }
 
-void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, SortBufs &b) {
+void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
     const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
-    hipLaunchKernelGGL(k_rs_scatter<OpSrc>, dim3(n_tiles), dim3(kBlock), 0, s,
-                       OpSrc{dt.tabs, ov.rows, ov.meta, col_bits}, b.keys[1], b.vals[1], n, 0u,
-                       n_tiles, b.hist);
+    LTPMG_RS_BITS(bits, k_rs_scatter, OpSrc, OpSrc{dt.tabs, ov.rows, ov.meta}, b.keys[1], b.vals[1], n,
+                  0u, n_tiles, b.hist);
+}
+
+uint32_t radix_digit_bits(uint32_t key_bits) {
+    // 9-bit digits when they save a pass (e.g. 27 key bits: 3 passes, not 4)
+    return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9u : 8u;
 }
 
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits) {
     int cur = 0;
     if (n <= 1) return cur;
-    for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-        launch_rs_hist(s, b, cur, n, shift);
-        launch_rs_scan(s, b, n);
-        launch_rs_scatter(s, b, cur, n, shift);
+    const uint32_t bits = radix_digit_bits(key_bits);
+    for (uint32_t shift = 0; shift < key_bits; shift += bits) {
+        launch_rs_hist(s, b, cur, n, shift, bits);
+        launch_rs_scan(s, b, n, bits);
+        launch_rs_scatter(s, b, cur, n, shift, bits);
         cur ^= 1;
     }
     return cur;
 }
 
-void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t col_bits, uint32_t *keys,
-                    uint32_t *vals) {
+void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint32_t *vals) {
     uint32_t blocks = (ov.n_ops + kBlock - 1) / kBlock;
     if (blocks > 16384) blocks = 16384;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, col_bits, keys, vals);
+    hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, keys, vals);
 }
 
 #define LTPMG_MO_DISPATCH(KERNEL, GRID, ...)                                               \
@@ -751,22 +781,21 @@ void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
 }
 
 void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint32_t col_bits, uint32_t n_tabs, uint8_t *results,
-                           uint32_t res_stride) {
+                           uint32_t n_tabs, uint8_t *results, uint32_t res_stride) {
     const uint32_t n = ov.n_ops;
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
-    KeyDecode dec{dt.tabs, col_bits, n_tabs};
+    KeyDecode dec{dt.tabs, n_tabs};
     LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.carry_set, ob.carry_delta, dt, dec,
                       results, res_stride, ob.deferred);
 }
 
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t col_bits, uint32_t n_tabs, uint32_t res_stride) {
+                            uint32_t n_tabs, uint32_t res_stride) {
     if (ov.n_ops == 0) return;
     const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
-    KeyDecode dec{dt.tabs, col_bits, n_tabs};
+    KeyDecode dec{dt.tabs, n_tabs};
     // one lane group per entry; res_stride >= every slot, so G lanes cover a cell
     const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
     LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.deferred, n_tiles, dt, dec, ov.args);

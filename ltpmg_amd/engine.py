@@ -394,6 +394,12 @@ class Engine:
                     self._lib.ltpmg_read_column(self._h, table_id, col, row0, n, _ptr(out)))
         return out
 
+    def stats_hbm(self) -> int:
+        """Bytes of HBM the engine holds (tables, lock words, slots)."""
+        s = _Stats()
+        self._lib.ltpmg_stats(self._h, 0, C.byref(s))  # fills hbm_allocated even before any batch
+        return int(s.hbm_allocated)
+
     def digest(self, table_id: int) -> int:
         v = C.c_uint64(0)
         self._check("ltpmg_table_digest", self._lib.ltpmg_table_digest(self._h, table_id, C.byref(v)))

@@ -2,8 +2,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -26,6 +29,64 @@ struct Flight {
     size_t first;  // index into `entries`
     uint32_t n;
     uint32_t n_ops;
+};
+
+// Packing workers that live for the whole run: a batch is split into one
+// range per worker, and run() returns when every range is copied.
+class PackPool {
+  public:
+    explicit PackPool(unsigned n) {
+        for (unsigned w = 1; w < n; ++w) th_.emplace_back([this, w] { loop(w); });
+        n_ = n;
+    }
+    ~PackPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    // fn(w, n) for every worker w in [0, n); the caller runs w = 0
+    void run(const std::function<void(unsigned, unsigned)> &fn) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &fn;
+            pending_ = n_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0, n_);
+        std::unique_lock<std::mutex> l(mu_);
+        done_cv_.wait(l, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+    unsigned size() const { return n_; }
+
+  private:
+    void loop(unsigned w) {
+        uint64_t seen = 0;
+        while (true) {
+            const std::function<void(unsigned, unsigned)> *fn;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                fn = job_;
+            }
+            (*fn)(w, n_);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--pending_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(unsigned, unsigned)> *job_ = nullptr;
+    uint64_t gen_ = 0;
+    unsigned pending_ = 0, n_ = 1;
+    bool stop_ = false;
 };
 
 double pct(std::vector<double> &v, double q) {
@@ -90,6 +151,7 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
     double pack_s = 0.0, device_ms = 0.0;
     uint64_t batches = 0, retries = 0, batch_txn_sum = 0;
     const unsigned pack_threads = cfg->pack_threads ? cfg->pack_threads : 8;
+    PackPool pool_threads(pack_threads);
     if (rec) {
         rec->n_batches = rec->n_entries = rec->overflow = 0;
         if (rec->batch_off && rec->cap_batches) rec->batch_off[0] = 0;
@@ -185,14 +247,13 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
                         std::memcpy(st.args + dst, pool->args + src, (size_t)len * 8);
                     }
                 };
-                const unsigned nt = k >= 65536 ? pack_threads : 1;
-                if (nt == 1) {
+                // the workers pay off once a batch carries a few MB of ops
+                if (n_ops < (1u << 17) || pool_threads.size() == 1) {
                     copy(0, k);
                 } else {
-                    std::vector<std::thread> th;
-                    for (unsigned w = 0; w < nt; ++w)
-                        th.emplace_back(copy, (uint32_t)((uint64_t)k * w / nt), (uint32_t)((uint64_t)k * (w + 1) / nt));
-                    for (auto &t : th) t.join();
+                    pool_threads.run([&](unsigned w, unsigned nw) {
+                        copy((uint32_t)((uint64_t)k * w / nw), (uint32_t)((uint64_t)k * (w + 1) / nw));
+                    });
                 }
                 rc = ltpmg_submit_staged(e, slot, LTPMG_SUBMIT_TRUSTED);  // the pool was checked
                 if (rc == LTPMG_OK) rc = ltpmg_execute(e, slot, cfg->mode);

@@ -86,8 +86,10 @@ def test_sort_reference_kat_descending_million():
     assert (v == np.arange(n - 1, -1, -1, dtype=np.uint32)).all()
 
 
+# 9, 17, 18 and 27 key bits take 9-bit digits (one pass fewer than 8-bit ones)
 @pytest.mark.parametrize("n,bits", [(1, 8), (100, 7), (4097, 8), (70000, 13), (300001, 24),
-                                    (1 << 20, 28), (123457, 32)])
+                                    (1 << 20, 28), (123457, 32), (100000, 9), (65537, 17),
+                                    (250001, 18), (1 << 21, 27)])
 def test_sort_is_stable_and_matches_numpy(n, bits):
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
@@ -259,6 +261,72 @@ def test_ordered_mode_over_many_tiles():
     with Engine() as e:
         im = _setup(e, {0: (1_000_000, [100] * 4)}, 0x4C54504D47000006)
         _compare(e, im, b, MODE_O)
+
+
+def _sparse_image(schema: dict, seed: int, batches) -> orc.Image:
+    """An oracle image that fills only the cells the batches touch."""
+    im = orc.Image(schema)
+    for t, (n, widths) in schema.items():
+        for c in range(len(widths)):
+            rows = np.unique(np.concatenate([b.rows[(((b.meta >> 8) & 15) == t) & (((b.meta >> 2) & 63) == c)]
+                                             for b in batches]))
+            if rows.size:
+                im.fill_cells(seed, t, c, rows)
+    return im
+
+
+def test_ordered_mode_past_2e31_cells():
+    # 34M rows x 64 columns = 2.18e9 cells: keys above 2^31 reach the sort and decode
+    rows, cols = 34_000_000, 64
+    rng = np.random.default_rng(12)
+    n = 60_000
+    hi = rng.integers(rows - 2_000_000, rows, n // 2)
+    lo = rng.integers(0, rows, n // 2)
+    keys = np.concatenate([hi, lo])
+    txns = []
+    for t in range(n // 4):
+        ops = []
+        for k in range(4):
+            r = int(keys[4 * t + k])
+            ops.append((OP_UPDATE if k % 2 else OP_READ, 0, int(rng.integers(0, cols)), r,
+                        int(rng.integers(0, 2**62))))
+        # distinct (row, col) within the txn
+        seen, uniq = set(), []
+        for op in ops:
+            if (op[3], op[2]) not in seen:
+                seen.add((op[3], op[2]))
+                uniq.append(op)
+        txns.append(uniq)
+    b1 = Batch.from_txns(txns)
+    # the second batch reads every cell the first one touched, then some
+    cells = sorted({(int(r), int(c)) for r, c in zip(b1.rows, (b1.meta >> 2) & 63)})
+    b2 = Batch.from_txns([[(OP_READ, 0, c, r, 0)] for r, c in cells])
+    schema = {0: (rows, [4] * cols)}
+    seed = 0x4C54504D47000009
+    with Engine() as e:
+        e.create_table(0, rows, [4] * cols)
+        e.fill_table(0, seed)
+        im = _sparse_image(schema, seed, [b1, b2])
+        rs = e.result_stride
+        for b in (b1, b2):
+            commit, res = e.run(b, MODE_O)
+            oc, ores = orc.replay(im, b, MODE_O, rs)
+            assert (commit == oc).all()
+            m = orc.read_mask(b, oc)
+            assert (res[m] == ores[m]).all()
+
+
+def test_ordered_mode_refuses_more_than_2e32_cells():
+    rows, cols = 67_200_000, 64  # 4.3e9 cells
+    b = Batch.from_txns([[(OP_UPDATE, 0, 63, rows - 1, 5)], [(OP_READ, 0, 63, rows - 1, 0)]])
+    with Engine() as e:
+        e.create_table(0, rows, [4] * cols)
+        e.submit(b)
+        with pytest.raises(LtpmgError) as ei:
+            e.execute(MODE_O)
+        assert ei.value.code == -1 and "cells" in str(ei.value)
+        commit, res = e.run(b, MODE_A)
+        assert commit.tolist() == [1, 0]
 
 
 def test_batch_sequence_and_determinism():
