@@ -2,12 +2,13 @@
 // same class, public methods and the two extern entry points at
 // database.h:66-70).
 //
-// The reference reads the schema from data/<benchmark>.json with a vendored
-// JSON parser (src/row_parser.cpp); this build fixes the schemas it runs
-// (YCSB's 3 int + 3 double + 4 string columns, data/YCSB.json; the KV tables)
-// and keeps the filepath argument for source compatibility.  Cells are filled
-// from Param::seed with the engine's seeded cell definition (include/ltpmg.h),
-// so the host image equals what ltpmg_fill_table would write.
+// As in the reference, the constructor parses the benchmark's schema file
+// (Param's path, else the same name under --schema_dir, default the
+// schemas/ directory beside the library) and the YCSB table's int / double /
+// string column counts come from it; the order-entry schema's table list
+// must match the adapter's nine tables; the KV benchmarks have no file.
+// Cells are filled from Param::seed with the engine's seeded cell definition
+// (include/ltpmg.h), so the host image equals what ltpmg_fill_table writes.
 #pragma once
 
 #ifndef LTPMG_DATABASE
@@ -16,10 +17,15 @@
 #include "db_structure.h"
 #include "define.h"
 #include "param.h"
+#include "row_parser.h"
 
 class Database {
 private:
     std::string filepath;
+
+    Row_Parser parser;
+
+    std::vector<Row> rows;
 
     Global_Table_Info *tables_for_GPU_info = nullptr;
 
@@ -55,6 +61,7 @@ public:
     void transfer_to_CPU(std::shared_ptr<Param> param);
 
     const Global_Table_Info *get_info() const { return tables_for_GPU_info; }
+    const std::vector<Row> &get_rows() const { return rows; }
     const Global_Table *get_tables() const { return tables_for_GPU; }
 };
 

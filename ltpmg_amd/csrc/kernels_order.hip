@@ -730,8 +730,12 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
                   0u, n_tiles, b.hist);
 }
 
+#ifndef LTPMG_RADIX_MAX_BITS
+#define LTPMG_RADIX_MAX_BITS 9
+#endif
 uint32_t radix_digit_bits(uint32_t key_bits) {
     // 9-bit digits when they save a pass (e.g. 27 key bits: 3 passes, not 4)
+    if (LTPMG_RADIX_MAX_BITS < 9) return 8u;
     return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9u : 8u;
 }
 

@@ -176,6 +176,20 @@ OE_API = {
                                C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32]),
 }
 
+class _FixtureInfo(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("mode", C.c_uint32), ("n_tables", C.c_uint32),
+                ("tables", _TableDesc * MAX_TABLES)]
+
+
+FIXTURE_NO_MODE = 0xFFFFFFFF
+
+FIXTURE_API = {
+    "ltpmg_fixture_write": (C.c_int, [C.c_char_p, C.POINTER(_FixtureInfo), C.POINTER(_Batch)]),
+    "ltpmg_fixture_probe": (C.c_int, [C.c_char_p, C.POINTER(_FixtureInfo), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32)]),
+    "ltpmg_fixture_read": (C.c_int, [C.c_char_p, C.POINTER(_Batch)]),
+}
+
 SERVE_API = {
     "ltpmg_serve_run": (C.c_int, [C.c_void_p, C.POINTER(_ServeCfg), C.POINTER(_Batch),
                                   C.POINTER(_ServeStats), C.POINTER(_ServeRecord)]),
@@ -206,7 +220,7 @@ def hip_lib() -> C.CDLL:
 
 def host_lib() -> C.CDLL:
     hip_lib()  # the host library links against it
-    return _load("libltpmg_host.so", {**WORKLOAD_API, **SERVE_API, **OE_API})
+    return _load("libltpmg_host.so", {**WORKLOAD_API, **SERVE_API, **OE_API, **FIXTURE_API})
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
@@ -327,6 +341,43 @@ def oe_last_name_index(warehouses: int, table_seed: int) -> np.ndarray:
     if rc != 0:
         raise LtpmgError("ltpmg_oe_last_name_index", rc, "bad arguments")
     return out.reshape(warehouses * 10, 15)
+
+
+def write_fixture(path: str, batch: Batch, *, seed: int, schema: dict[int, tuple[int, list[int]]],
+                  mode: int | None = None) -> None:
+    """Writes a versioned binary batch fixture (include/ltpmg_fixture.h)."""
+    info = _FixtureInfo()
+    info.seed = seed & 0xFFFFFFFFFFFFFFFF
+    info.mode = FIXTURE_NO_MODE if mode is None else mode
+    info.n_tables = (max(schema) + 1) if schema else 0
+    for t, (rows, widths) in schema.items():
+        info.tables[t].n_rows = rows
+        info.tables[t].n_cols = len(widths)
+        for c, w in enumerate(widths):
+            info.tables[t].col_bytes[c] = w
+    b = batch.normalized()
+    rc = host_lib().ltpmg_fixture_write(str(path).encode(), C.byref(info), C.byref(b.c_struct()))
+    if rc != 0:
+        raise LtpmgError("ltpmg_fixture_write", rc, f"cannot write {path}")
+
+
+def read_fixture(path: str) -> tuple[Batch, dict]:
+    """(batch, {"seed", "mode", "schema"}) from a binary batch fixture."""
+    lib = host_lib()
+    info = _FixtureInfo()
+    nt, no = C.c_uint32(0), C.c_uint32(0)
+    rc = lib.ltpmg_fixture_probe(str(path).encode(), C.byref(info), C.byref(nt), C.byref(no))
+    if rc != 0:
+        raise LtpmgError("ltpmg_fixture_probe", rc, f"{path} is not a readable batch fixture")
+    b = Batch(np.zeros(nt.value + 1, np.uint32), np.zeros(no.value, np.uint32),
+              np.zeros(no.value, np.uint16), np.zeros(no.value, np.uint64))
+    rc = lib.ltpmg_fixture_read(str(path).encode(), C.byref(b.c_struct()))
+    if rc != 0:
+        raise LtpmgError("ltpmg_fixture_read", rc, f"{path}: bad arrays or checksum")
+    schema = {t: (info.tables[t].n_rows, list(info.tables[t].col_bytes[:info.tables[t].n_cols]))
+              for t in range(info.n_tables) if info.tables[t].n_cols}
+    mode = None if info.mode == FIXTURE_NO_MODE else info.mode
+    return b, {"seed": info.seed, "mode": mode, "schema": schema}
 
 
 def zeta(n: int, theta: float) -> float:

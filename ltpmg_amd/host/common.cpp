@@ -6,7 +6,7 @@
 #include <ctime>
 #include <map>
 
-DEFINE_string(benchmark, "YCSB_E", "YCSB_A/B/C/D/E, KV, KV_C1 (TPCC_PART/TPCC_ALL/TEST sizes only).");
+DEFINE_string(benchmark, "YCSB_E", "YCSB_A/B/C/E, KV, KV_C1, TPCC_PART, OE (TPCC_ALL/TEST: sizes only).");
 DEFINE_uint32(table_size, 1, "YCSB: table_size * 1,000,000 rows; TPC-C: warehouses.");
 DEFINE_uint32(batch_size, 8192, "transactions per batch.");
 DEFINE_uint32(epoch_tp, 10, "batches executed (the same batch each epoch).");
@@ -25,6 +25,8 @@ DEFINE_uint32(kv_ops, 16, "KV / KV_C1: logical ops per txn.");
 DEFINE_uint32(read_percent, 50, "KV / KV_C1: percent of ops that READ.");
 DEFINE_string(dump_path, "", "write epoch 0's batch, results and image digest under this prefix.");
 DEFINE_uint32(append_cap, 1u << 20, "TPCC_PART / OE: capacity of the append-only tables, in transactions.");
+DEFINE_string(schema_dir, "", "directory of <benchmark>.json table schemas when Param's path is absent "
+                             "(default: the schemas/ directory beside the library).");
 DEFINE_uint32(minloglevel, 0, "drop log lines below this severity (0 INFO .. 3 FATAL).");
 
 namespace ltpmg_log {

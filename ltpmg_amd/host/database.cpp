@@ -9,10 +9,31 @@
 // the same seed (ltpmg_fill_table), which gives the same bytes.
 #include "database.h"
 
+#include <dlfcn.h>
+
 #include "ltpmg.h"
 #include "ltpmg_oe.h"
 
 namespace {
+// Param's schema path ("../data/<benchmark>.json", relative as in the
+// reference) if it exists, else that file name under --schema_dir or the
+// schemas/ directory beside this library; "" for benchmarks without one.
+std::string resolve_schema(const std::string &path) {
+    if (path.empty() || std::ifstream(path).good()) return path;
+    std::string dir = FLAGS_schema_dir;
+    if (dir.empty()) {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void *>(&resolve_schema), &info) && info.dli_fname) {
+            const std::string lib = info.dli_fname;
+            dir = lib.substr(0, lib.find_last_of('/') + 1) + "../schemas";
+        }
+    }
+    const std::string alt = dir + "/" + path.substr(path.find_last_of('/') + 1);
+    if (std::ifstream(alt).good()) return alt;
+    LOG(WARNING) << "schema " << path << " not found (nor " << alt << ")";
+    return "";
+}
+
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -31,7 +52,9 @@ void cell_bytes(uint64_t seed, uint32_t table, uint32_t row, uint32_t col, uint3
 }
 }  // namespace
 
-Database::Database(const std::string &filepath) : filepath(filepath) {}
+Database::Database(const std::string &path) : filepath(resolve_schema(path)) {
+    if (!filepath.empty()) rows = parser.parse(filepath);
+}
 
 Database::~Database() {}
 
@@ -62,7 +85,17 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
     if (param->is_oe()) {
         // nine order-entry tables with the widths of include/ltpmg_oe.h; the
         // per-type counts of Global_Table_Info cannot express them, so only
-        // the row counts are recorded and the engine fills the tables
+        // the row counts are recorded and the engine fills the tables.  The
+        // schema file must list the same tables in the same order.
+        static const char *const kNames[LTPMG_OE_TABLES] = {"Warehouse", "District", "Customer",
+                                                           "Neworder", "History", "Order",
+                                                           "Orderline", "Stock", "Item"};
+        if (rows.empty()) LOG(FATAL) << "order-entry benchmarks need their schema file (TPCC.json)";
+        if (rows.size() != LTPMG_OE_TABLES) LOG(FATAL) << filepath << ": " << rows.size() << " tables, 9 expected";
+        for (uint32_t i = 0; i < LTPMG_OE_TABLES; ++i)
+            if (rows[i].get_table_name() != kNames[i])
+                LOG(FATAL) << filepath << ": table " << i << " is " << rows[i].get_table_name() << ", expected "
+                           << kNames[i];
         ltpmg_table_desc d[LTPMG_OE_TABLES];
         if (n_tables != LTPMG_OE_TABLES || ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, d))
             LOG(FATAL) << "order-entry schema: " << param->warehouse_size << " partitions, append capacity "
@@ -78,9 +111,15 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
         LOG(FATAL) << "benchmark " << param->benchmark << ": schema not built in this engine";
     Global_Table_Info &t = tables_for_GPU_info[0];
     if (param->is_ycsb()) {
-        t.int_size = 3;
-        t.double_size = 3;
-        t.string_size = 4;
+        // column counts by type from the schema (src/database.cpp:86-95)
+        if (rows.size() != 1) LOG(FATAL) << "YCSB needs a one-table schema file (YCSB.json)";
+        for (const Attribute &a : rows[0].getAttributes()) {
+            if (a.getType() == "int") ++t.int_size;
+            else if (a.getType() == "double") ++t.double_size;
+            else ++t.string_size;
+        }
+        if (t.int_size + t.double_size + t.string_size > LTPMG_MAX_COLS)
+            LOG(FATAL) << filepath << ": more than " << LTPMG_MAX_COLS << " columns";
         t.string_length = 8;
         t.table_size = param->ycsb_size;
     } else if (param->benchmark == "KV_C1") {

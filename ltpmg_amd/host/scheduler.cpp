@@ -5,6 +5,7 @@
 #include "scheduler.h"
 
 #include "ltpmg.h"
+#include "ltpmg_fixture.h"
 #include "ltpmg_oe.h"
 
 namespace {
@@ -16,37 +17,39 @@ void write_vec(const std::string &path, const std::vector<T> &v) {
     if (!f) LOG(FATAL) << "cannot write " << path;
 }
 
-// Epoch 0's batch and outputs, so a checker can replay the batch on the
-// seeded initial image.
+// Epoch 0's batch as a binary fixture (include/ltpmg_fixture.h: the batch,
+// the fill seed, the mode and the table shapes) plus its outputs, so a
+// checker can replay the batch on the seeded initial image.
 void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info) {
     Batch_Results r;
     if (!fetch_results_from_GPU(param, &r)) LOG(FATAL) << "no results to dump";
     const std::string &p = FLAGS_dump_path;
-    write_vec(p + ".txn_off.u32", r.batch.txn_off);
-    write_vec(p + ".rows.u32", r.batch.rows);
-    write_vec(p + ".meta.u16", r.batch.meta);
-    write_vec(p + ".args.u64", r.batch.args);
+    ltpmg_fixture_info fx;
+    std::memset(&fx, 0, sizeof fx);
+    fx.seed = param->seed;
+    fx.mode = param->cc_mode;
+    fx.n_tables = param->table_cnt;
+    if (param->is_oe()) {
+        ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, fx.tables);
+    } else {
+        for (uint32_t t = 0; t < param->table_cnt; ++t) {
+            const Global_Table_Info &ti = info[t];
+            ltpmg_table_desc &d = fx.tables[t];
+            d.n_rows = ti.table_size;
+            for (uint32_t c = 0; c < ti.int_size; ++c) d.col_bytes[d.n_cols++] = 4;
+            for (uint32_t c = 0; c < ti.double_size; ++c) d.col_bytes[d.n_cols++] = 8;
+            for (uint32_t c = 0; c < ti.string_size; ++c) d.col_bytes[d.n_cols++] = 4 * ti.string_length;
+        }
+    }
+    ltpmg_batch b{(uint32_t)(r.batch.txn_off.size() - 1), (uint32_t)r.batch.rows.size(), r.batch.txn_off.data(),
+                  r.batch.rows.data(), r.batch.meta.data(), r.batch.args.data()};
+    if (ltpmg_fixture_write((p + ".batch").c_str(), &fx, &b) != LTPMG_OK) LOG(FATAL) << "cannot write " << p << ".batch";
     write_vec(p + ".commit.u8", r.commit);
     write_vec(p + ".results.u8", r.results);
     std::ofstream j(p + ".json");
-    j << "{\"benchmark\": \"" << param->benchmark << "\", \"seed\": " << param->seed
-      << ", \"mode\": " << param->cc_mode << ", \"result_stride\": " << r.result_stride
-      << ", \"committed\": " << param->result.committed << ", \"tables\": [";
-    ltpmg_table_desc oe_desc[LTPMG_OE_TABLES];
-    if (param->is_oe()) ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, oe_desc);
-    for (uint32_t t = 0; t < param->table_cnt; ++t) {
-        const Global_Table_Info &ti = info[t];
-        j << (t ? ", " : "") << "{\"rows\": " << ti.table_size << ", \"col_bytes\": [";
-        uint32_t k = 0;
-        if (param->is_oe()) {
-            for (uint32_t c = 0; c < oe_desc[t].n_cols; ++c) j << (k++ ? ", " : "") << oe_desc[t].col_bytes[c];
-        } else {
-            for (uint32_t c = 0; c < ti.int_size; ++c) j << (k++ ? ", " : "") << 4;
-            for (uint32_t c = 0; c < ti.double_size; ++c) j << (k++ ? ", " : "") << 8;
-            for (uint32_t c = 0; c < ti.string_size; ++c) j << (k++ ? ", " : "") << 4 * ti.string_length;
-        }
-        j << "], \"digest\": " << table_digest_on_GPU(param, t) << "}";
-    }
+    j << "{\"benchmark\": \"" << param->benchmark << "\", \"result_stride\": " << r.result_stride
+      << ", \"committed\": " << param->result.committed << ", \"digests\": [";
+    for (uint32_t t = 0; t < param->table_cnt; ++t) j << (t ? ", " : "") << table_digest_on_GPU(param, t);
     j << "]}\n";
     LOG(INFO) << "dumped epoch 0 to " << p << ".*";
 }

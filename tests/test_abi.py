@@ -43,7 +43,8 @@ def test_stream_exports_every_declared_symbol():
 
 def test_serving_and_order_entry_export_every_declared_symbol():
     exported = _exported(eng.LIB_DIR / "libltpmg_host.so")
-    for header, api in (("ltpmg_serve.h", eng.SERVE_API), ("ltpmg_oe.h", eng.OE_API)):
+    for header, api in (("ltpmg_serve.h", eng.SERVE_API), ("ltpmg_oe.h", eng.OE_API),
+                        ("ltpmg_fixture.h", eng.FIXTURE_API)):
         decl = _declared(ROOT / "include" / header)
         assert decl and decl <= exported
         assert decl == set(api)

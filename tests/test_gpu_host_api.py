@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_ffi as orc
-from ltpmg_amd.engine import LIB_DIR, Batch
+from ltpmg_amd.engine import LIB_DIR, read_fixture
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -22,27 +22,23 @@ def run_driver(tmp_path: Path, *flags: str) -> dict:
     proc = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert proc.returncode == 0, proc.stderr[-2000:]
     meta = json.loads((tmp_path / "dump.json").read_text())
-
-    def load(ext, dt):
-        return np.fromfile(f"{prefix}.{ext}", dtype=dt)
-
-    meta["batch"] = Batch(load("txn_off.u32", np.uint32), load("rows.u32", np.uint32),
-                          load("meta.u16", np.uint16), load("args.u64", np.uint64))
-    meta["commit"] = load("commit.u8", np.uint8)
-    meta["results"] = load("results.u8", np.uint8).reshape(-1, meta["result_stride"])
+    batch, info = read_fixture(f"{prefix}.batch")
+    meta.update(info)
+    meta["batch"] = batch
+    meta["commit"] = np.fromfile(f"{prefix}.commit.u8", dtype=np.uint8)
+    meta["results"] = np.fromfile(f"{prefix}.results.u8", dtype=np.uint8).reshape(-1, meta["result_stride"])
     return meta
 
 
 def check_against_oracle(meta: dict):
-    schema = {t: (tb["rows"], tb["col_bytes"]) for t, tb in enumerate(meta["tables"])}
-    im = orc.Image(schema).fill(meta["seed"])
+    im = orc.Image(meta["schema"]).fill(meta["seed"])
     b = meta["batch"]
     oc, ores = orc.replay(im, b, meta["mode"], meta["result_stride"])
     assert (meta["commit"] == oc).all()
     m = orc.read_mask(b, oc)
     assert (meta["results"][m] == ores[m]).all()
-    for t, tb in enumerate(meta["tables"]):
-        assert tb["digest"] == orc.digest_columns(t, im.cols[t])
+    for t in sorted(meta["schema"]):
+        assert meta["digests"][t] == orc.digest_columns(t, im.cols[t])
     assert meta["committed"] == int(oc.sum())
 
 
@@ -72,7 +68,7 @@ def test_driver_ycsb(tmp_path, bench):
 def test_driver_order_entry(tmp_path, bench, mode):
     meta = run_driver(tmp_path, f"--benchmark={bench}", "--table_size=2", "--append_cap=20000",
                       "--batch_size=4096", "--epoch_tp=1", f"--cc_mode={mode}")
-    assert len(meta["tables"]) == 9
+    assert len(meta["schema"]) == 9
     check_against_oracle(meta)
 
 
