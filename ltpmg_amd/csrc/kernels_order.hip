@@ -730,11 +730,15 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
                   0u, n_tiles, b.hist);
 }
 
+// 8-bit digits by default.  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are
+// taken when they save a pass; on MI355X that measured slower for the 27-bit
+// keys of a 10M x 10 table (three 9-bit passes 1.624 ms per batch, four
+// 8-bit passes 1.610 ms): the 9-bit scatter's LDS drops it to 3 workgroups
+// per CU.
 #ifndef LTPMG_RADIX_MAX_BITS
-#define LTPMG_RADIX_MAX_BITS 9
+#define LTPMG_RADIX_MAX_BITS 8
 #endif
 uint32_t radix_digit_bits(uint32_t key_bits) {
-    // 9-bit digits when they save a pass (e.g. 27 key bits: 3 passes, not 4)
     if (LTPMG_RADIX_MAX_BITS < 9) return 8u;
     return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9u : 8u;
 }

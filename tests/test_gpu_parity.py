@@ -86,7 +86,6 @@ def test_sort_reference_kat_descending_million():
     assert (v == np.arange(n - 1, -1, -1, dtype=np.uint32)).all()
 
 
-# 9, 17, 18 and 27 key bits take 9-bit digits (one pass fewer than 8-bit ones)
 @pytest.mark.parametrize("n,bits", [(1, 8), (100, 7), (4097, 8), (70000, 13), (300001, 24),
                                     (1 << 20, 28), (123457, 32), (100000, 9), (65537, 17),
                                     (250001, 18), (1 << 21, 27)])
