@@ -320,7 +320,8 @@ def pct(xs, q):
     return xs[k]
 
 
-def cpu_baseline(batch, rows: int, width: int, n_cols: int, sample_txns: int, threads: int) -> dict:
+def cpu_baseline(batch, rows: int, width: int, n_cols: int, sample_txns: int, threads: int,
+                 runs: int) -> dict:
     import oracle_ffi as orc
     from ltpmg_amd.engine import Batch
     s = min(sample_txns, batch.n_txns)
@@ -333,14 +334,17 @@ def cpu_baseline(batch, rows: int, width: int, n_cols: int, sample_txns: int, th
         r = np.unique(sb.rows[col == c])
         im.fill_cells(TABLE_SEED, 0, c, r)
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         _, _, secs = orc.cpu_pipeline(im, sb, orc.MODE_O, 128, n_threads=threads)
         times.append(secs)
     secs = statistics.median(times)
+    what = "the whole batch" if s == batch.n_txns else f"the first {s} txns of the batch"
     return {"value": s / secs, "unit": "txn/s", "cores": threads, "kind": "port",
-            "sample": f"first {s} txns ({n_ops} ops) of the same batch, ordered mode, "
+            "sample": f"{what} ({s} txns, {n_ops} ops), ordered mode, the same table seed; "
                       f"C++ restatement of the reference pipeline (histogram, scan, scatter, "
-                      f"per-row TID sort, per-row apply), median of 3, {platform.processor() or 'host'}",
+                      f"per-row TID sort, per-row apply) timed around CC + execute only; "
+                      f"median of {runs} runs ({sum(times):.1f} s of CPU pipeline time), "
+                      f"{platform.processor() or 'host'}",
             "seconds": secs}
 
 
@@ -353,7 +357,8 @@ def main() -> int:
     ap.add_argument("--txns", type=int, default=1 << 20)
     ap.add_argument("--ops", type=int, default=16)
     ap.add_argument("--theta", type=float, default=0.99)
-    ap.add_argument("--cpu-sample-txns", type=int, default=1 << 18)
+    ap.add_argument("--cpu-sample-txns", type=int, default=1 << 20)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8],
                     help="arrival rates (txn/s) for the served-vs-plain comparison; none to skip")
@@ -466,7 +471,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(batch, args.rows, width, n_cols, args.cpu_sample_txns, threads)
+        cpu = cpu_baseline(batch, args.rows, width, n_cols, args.cpu_sample_txns, threads, args.cpu_runs)
 
     modes = {}
     for name, r in res.items():

@@ -180,7 +180,7 @@ void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
         dev_free(e, ob.sort.vals[i], ops * 4);
     }
     dev_free(e, ob.sort.hist, (st << kMaxDigitBits) * 4);
-    dev_free(e, ob.sort.partials, scan_partials_needed((uint32_t)(st << kMaxDigitBits)) * 4);
+    dev_free(e, ob.sort.partials, sort_partials_needed() * 4);
     dev_free(e, ob.tile_flag, sg * 4 * 5);
     dev_free(e, ob.deferred, sg * 16);
     ob = OrderBufs{};
@@ -195,7 +195,7 @@ int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
         ALLOC(ob.sort.vals[i], ops);
     }
     ALLOC(ob.sort.hist, st << kMaxDigitBits);
-    ALLOC(ob.sort.partials, scan_partials_needed((uint32_t)(st << kMaxDigitBits)));
+    ALLOC(ob.sort.partials, sort_partials_needed());
     ALLOC(ob.tile_flag, sg * 5);
     ALLOC(ob.deferred, sg);
 #undef ALLOC
@@ -871,7 +871,7 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
     }
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.hist, tiles << kMaxDigitBits);
     if (rc == LTPMG_OK)
-        rc = dev_alloc_n(e, &b.partials, scan_partials_needed((uint32_t)(tiles << kMaxDigitBits)));
+        rc = dev_alloc_n(e, &b.partials, sort_partials_needed());
     if (rc == LTPMG_OK && n) {
         int cur = 0;
         if (hipMemcpy(b.keys[0], keys, 4ull * n, hipMemcpyHostToDevice) != hipSuccess ||
@@ -890,7 +890,7 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
         dev_free(e, b.vals[i], 4ull * std::max<size_t>(n, 1));
     }
     dev_free(e, b.hist, (tiles << kMaxDigitBits) * 4);
-    dev_free(e, b.partials, 4 * scan_partials_needed((uint32_t)(tiles << kMaxDigitBits)));
+    dev_free(e, b.partials, 4 * sort_partials_needed());
     return rc;
 }
 

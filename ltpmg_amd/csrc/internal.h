@@ -22,7 +22,10 @@ constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 constexpr uint32_t kBlock = 256;       // threads per workgroup (4 waves of 64)
 constexpr uint32_t kTileOps = 2048;    // nominal ops per CC tile
 constexpr uint32_t kTileOpsMax = kTileOps + LTPMG_MAX_TXN_OPS;
-constexpr uint32_t kSortItems = 16;    // keys per thread in a radix-sort tile
+#ifndef LTPMG_SORT_ITEMS
+#define LTPMG_SORT_ITEMS 16
+#endif
+constexpr uint32_t kSortItems = LTPMG_SORT_ITEMS;  // keys per thread in a radix-sort tile
 constexpr uint32_t kSortTile = kBlock * kSortItems;
 // Cell moves in flight per lane group (independent loads issued before the
 // stores of a round); a build-time knob for tuning sweeps.
@@ -179,8 +182,8 @@ hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile
 struct SortBufs {
     uint32_t *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;      // 2^kMaxDigitBits * n_tiles
-    uint32_t *partials;  // scan partials
+    uint32_t *hist;      // 2^kMaxDigitBits * n_tiles: per tile, one row of digit counts
+    uint32_t *partials;  // sort_partials_needed() words
 };
 constexpr uint32_t kMaxDigitBits = 9;  // radix digits of 8 or 9 bits
 // Sorts (keys[0], vals[0]) by the low key_bits bits, stably; returns which
@@ -196,9 +199,8 @@ void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t
 // then (after launch_rs_scan) the scatter into keys[1] / vals[1]
 void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits);
 void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits);
-void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
-                           uint32_t *partials);
-size_t scan_partials_needed(uint32_t n);
+// words of SortBufs::partials (per-chunk digit sums of one pass)
+size_t sort_partials_needed();
 
 struct OrderBufs {
     SortBufs sort;

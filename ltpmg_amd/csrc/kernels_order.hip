@@ -475,8 +475,8 @@ struct OpSrc {
     }
 };
 
-// Digit histogram of one 4096-key tile: per-wave LDS counters, then one
-// column hist[d * n_tiles + tile] per digit (the scan runs digit-major).
+// Digit histogram of one 4096-key tile: per-wave LDS counters, then the
+// tile's row of D counts, written contiguously.
 template <class Src, uint32_t BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
                                                     uint32_t n_tiles, uint32_t *hist) {
@@ -488,19 +488,25 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t base = blockIdx.x * kSortTile;
+    // every key of the thread first (independent loads in flight), then the
+    // LDS counts
+    uint32_t digit[kSortItems];
+#pragma unroll
     for (uint32_t it = 0; it < kSortItems; ++it) {
-        uint32_t idx = base + it * kBlock + threadIdx.x;
-        if (idx < n) {
-            uint32_t key, val;
-            src.kv(idx, key, val, cache);
-            atomicAdd(&h[w][(key >> shift) & (D - 1u)], 1u);
-        }
+        const uint32_t idx = base + it * kBlock + threadIdx.x;
+        uint32_t key = 0u, val = 0u;
+        if (idx < n) src.kv(idx, key, val, cache);
+        digit[it] = idx < n ? (key >> shift) & (D - 1u) : D;
     }
+#pragma unroll
+    for (uint32_t it = 0; it < kSortItems; ++it)
+        if (digit[it] < D) atomicAdd(&h[w][digit[it]], 1u);
     __syncthreads();
+    // one contiguous row per tile: hist[tile * D + d]
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
         uint32_t c = 0;
         for (uint32_t i = 0; i < kBlock / 64; ++i) c += h[i][d];
-        hist[(uint64_t)d * n_tiles + blockIdx.x] = c;
+        hist[(uint64_t)blockIdx.x * D + d] = c;
     }
 }
 
@@ -523,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     __shared__ typename Src::Cache cache;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < D * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
-    for (uint32_t d = threadIdx.x; d < D; d += kBlock) gbase[d] = offs[(uint64_t)d * n_tiles + blockIdx.x];
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) gbase[d] = offs[(uint64_t)blockIdx.x * D + d];
     src.prepare(cache);
     __syncthreads();
     const uint32_t wbase = blockIdx.x * kSortTile + w * (kSortTile / (kBlock / 64));
@@ -603,10 +609,15 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
     }
 }
 
-// ---------------- exclusive scan of u32 ----------------
-
-constexpr uint32_t kScanItems = 16;
-constexpr uint32_t kScanTile = kBlock * kScanItems;
+// ---------------- digit offsets of a radix pass ----------------
+//
+// hist holds one row of D digit counts per tile.  The stable scatter needs,
+// for tile t and digit d, offs = (keys of smaller digits) + (keys of digit d
+// in tiles before t).  Tiles are grouped in chunks of up to kOffsetRows rows
+// per pass of the walk: k_rs_chunk_sum sums each chunk's column per digit;
+// k_rs_offsets gives each chunk its column prefix and the digit bases, then
+// walks its tiles and overwrites the counts with offsets.  Every access
+// reads or writes whole rows.
 
 __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total) {
     __shared__ uint32_t ws[kBlock / 64];
@@ -628,40 +639,80 @@ __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total) {
     return pre + inc - x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *in, uint32_t n,
-                                                        uint32_t *partials) {
-    uint32_t first = blockIdx.x * kScanTile + threadIdx.x * kScanItems, s = 0;
-    for (uint32_t j = 0; j < kScanItems; ++j)
-        if (first + j < n) s += in[first + j];
-    uint32_t tot;
-    block_exclusive_sum(s, &tot);
-    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+constexpr uint32_t kMaxSortChunks = 256;
+constexpr uint32_t kOffsetRows = 16;  // rows loaded together in the walk
+
+__host__ __device__ inline uint32_t chunk_tiles(uint32_t n_tiles) {
+    const uint32_t per = (n_tiles + kMaxSortChunks - 1) / kMaxSortChunks;
+    return per < 32u ? 32u : per;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t *partials, uint32_t np) {
-    uint32_t run = 0;
-    for (uint32_t base = 0; base < np; base += kBlock) {
-        uint32_t k = base + threadIdx.x;
-        uint32_t x = k < np ? partials[k] : 0u, tot;
-        uint32_t ex = block_exclusive_sum(x, &tot);
-        if (k < np) partials[k] = run + ex;
-        run += tot;
+template <uint32_t BITS>
+__global__ __launch_bounds__(kBlock) void k_rs_chunk_sum(const uint32_t *hist, uint32_t n_tiles,
+                                                         uint32_t *chunks) {
+    constexpr uint32_t D = 1u << BITS;
+    const uint32_t ct = chunk_tiles(n_tiles), t0 = blockIdx.x * ct;
+    const uint32_t t1 = t0 + ct < n_tiles ? t0 + ct : n_tiles;
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+        uint32_t sum = 0;
+        for (uint32_t t = t0; t < t1; t += kOffsetRows) {  // kOffsetRows loads in flight
+            uint32_t x[kOffsetRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) x[r] = t + r < t1 ? hist[(uint64_t)(t + r) * D + d] : 0u;
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) sum += x[r];
+        }
+        chunks[blockIdx.x * D + d] = sum;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32_t *out,
-                                                      uint32_t n, const uint32_t *partials) {
-    uint32_t first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems], s = 0;
-    for (uint32_t j = 0; j < kScanItems; ++j) {
-        v[j] = first + j < n ? in[first + j] : 0u;
-        s += v[j];
+template <uint32_t BITS>
+__global__ __launch_bounds__(kBlock) void k_rs_offsets(uint32_t *hist, uint32_t n_tiles,
+                                                       const uint32_t *chunks, uint32_t n_chunks) {
+    constexpr uint32_t D = 1u << BITS, PER = D / kBlock;
+    static_assert(D % kBlock == 0, "whole digits per thread");
+    const uint32_t ct = chunk_tiles(n_tiles), t0 = blockIdx.x * ct;
+    const uint32_t t1 = t0 + ct < n_tiles ? t0 + ct : n_tiles;
+    // thread owns digits threadIdx.x * PER + p, so thread order is digit order
+    uint32_t before[PER], total[PER], mine = 0;
+#pragma unroll
+    for (uint32_t p = 0; p < PER; ++p) {
+        const uint32_t d = threadIdx.x * PER + p;
+        uint32_t b = 0, a = 0;
+        for (uint32_t c = 0; c < n_chunks; c += kOffsetRows) {  // kOffsetRows loads in flight
+            uint32_t x[kOffsetRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) x[r] = c + r < n_chunks ? chunks[(c + r) * D + d] : 0u;
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) (c + r < blockIdx.x ? b : a) += x[r];
+        }
+        before[p] = b;
+        total[p] = a + b;
+        mine += a + b;
     }
-    uint32_t tot;
-    uint32_t run = block_exclusive_sum(s, &tot) + partials[blockIdx.x];
-    for (uint32_t j = 0; j < kScanItems; ++j) {
-        if (first + j < n) out[first + j] = run;
-        run += v[j];
+    uint32_t all;
+    uint32_t base = block_exclusive_sum(mine, &all);
+    uint32_t run[PER];
+#pragma unroll
+    for (uint32_t p = 0; p < PER; ++p) {
+        run[p] = base + before[p];
+        base += total[p];
+    }
+    for (uint32_t t = t0; t < t1; t += kOffsetRows) {
+        const uint32_t nr = t1 - t < kOffsetRows ? t1 - t : kOffsetRows;
+#pragma unroll
+        for (uint32_t p = 0; p < PER; ++p) {
+            const uint32_t d = threadIdx.x * PER + p;
+            uint32_t cnt[kOffsetRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r)
+                cnt[r] = r < nr ? hist[(uint64_t)(t + r) * D + d] : 0u;
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) {
+                if (r < nr) hist[(uint64_t)(t + r) * D + d] = run[p];
+                run[p] += cnt[r];
+            }
+        }
     }
 }
 
@@ -683,16 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
 
 }  // namespace
 
-size_t scan_partials_needed(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
-
-void launch_scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n,
-                           uint32_t *partials) {
-    if (n == 0) return;
-    uint32_t nb = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, partials);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, nb);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n, partials);
-}
+size_t sort_partials_needed() { return (size_t)kMaxSortChunks << kMaxDigitBits; }
 
 #define LTPMG_RS_BITS(BITS_VAR, KERNEL, SRC, ...)                                            \
     do {                                                                                   \
@@ -710,7 +752,16 @@ void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t sh
 
 void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n, uint32_t bits) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    launch_scan_exclusive(s, b.hist, b.hist, (1u << bits) * n_tiles, b.partials);
+    const uint32_t n_chunks = (n_tiles + chunk_tiles(n_tiles) - 1) / chunk_tiles(n_tiles);
+    if (bits == 9) {
+        hipLaunchKernelGGL(k_rs_chunk_sum<9>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials);
+        hipLaunchKernelGGL(k_rs_offsets<9>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials,
+                           n_chunks);
+    } else {
+        hipLaunchKernelGGL(k_rs_chunk_sum<8>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials);
+        hipLaunchKernelGGL(k_rs_offsets<8>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials,
+                           n_chunks);
+    }
 }
 
 void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits) {
