@@ -298,7 +298,7 @@ def serve_c5(args) -> dict:
     out = {"partitions": W, "kv_rows": kv_rows, "tables_gib": tables_gb, "hbm_budget_gib": args.c5_hbm_budget_gb,
            "pool_txns": pool.n_txns, "pool_ops": pool.n_ops, "mode": "ordered", "rates": []}
     for rate in args.c5_rates:
-        n = int(max(1 << 17, min(1 << 20, rate * args.c5_seconds)))
+        n = int(max(1 << 17, min(1 << 23, rate * args.c5_seconds)))
         row = {"arrival_rate": rate, "arrivals": n}
         forms = [(f"plain_{b}", dict(max_batch=b, min_batch=b, n_inflight=1)) for b in args.c5_plain_batches]
         forms.append(("served", dict(max_batch=args.c5_served_max, max_wait_us=args.c5_wait_us, n_inflight=2)))
@@ -362,7 +362,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8],
                     help="arrival rates (txn/s) for the served-vs-plain comparison; none to skip")
-    ap.add_argument("--serve-arrivals", type=int, default=1 << 20)
+    ap.add_argument("--serve-arrivals", type=int, default=1 << 22)
     ap.add_argument("--plain-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-wait-us", type=float, default=2000.0)
@@ -453,7 +453,7 @@ def main() -> int:
             per_kernel[f"{name}/{k}"] = {
                 "ms": ms, "launches": l,
                 "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None,
-                "traffic_per_launch": traffic.get(k) if name == "ordered" else None}
+                "traffic_per_launch": traffic.get(k)}
 
     serving = None
     if args.serve_rates:
