@@ -68,16 +68,9 @@ def batch_counts(batch, width: int) -> dict:
     reads = sk == 0
     snapshot_reads = int(np.count_nonzero(reads & (cm == 0)))
     written_cells = int(np.unique(sc[sk != 0]).size)
-    # cells whose final write waits for the deferred pass: the segment spans a
-    # 2048-element tile boundary of the sorted list and holds a write
-    seg_has_write = np.zeros(int(seg[-1]) + 1 if n else 0, bool)
-    seg_has_write[seg[sk != 0]] = True
-    b = np.arange(SEG_TILE, n, SEG_TILE)
-    crossing = np.unique(seg[b][sc[b] == sc[b - 1]]) if n else np.zeros(0, np.int64)
-    deferred = int(np.count_nonzero(seg_has_write[crossing]))
     return {"n_ops": n, "n_txns": batch.n_txns, "reads": int(reads.sum()),
             "writes": int((sk != 0).sum()), "snapshot_reads": snapshot_reads,
-            "written_cells": written_cells, "deferred_writes": deferred,
+            "written_cells": written_cells,
             "seg_tiles": (n + SEG_TILE - 1) // SEG_TILE, "width": width}
 
 
@@ -92,13 +85,12 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
         "rs_scatter": 16 * n,                  # key+val in and out
         "mo_seg_scan": 8 * n,                  # sorted key+val in
         # sorted key+val; per READ its 100-B result, plus the snapshot cell or
-        # the 8-B payload seed of the in-batch UPDATE it sees; per cell whose
-        # segment ends in its tile, payload seed 8 + the cell write
+        # the 8-B payload seed of the in-batch UPDATE it sees; per written
+        # cell its 16-B entry in the final-write list
         "mo_apply_reads": 8 * n + c["reads"] * w + c["snapshot_reads"] * w
-                          + 8 * (c["reads"] - c["snapshot_reads"])
-                          + (c["written_cells"] - c["deferred_writes"]) * (8 + w),
-        # one 16-B entry per tile; per deferred cell payload seed 8 + cell write
-        "mo_final_writes": 16 * c["seg_tiles"] + c["deferred_writes"] * (8 + w),
+                          + 8 * (c["reads"] - c["snapshot_reads"]) + 16 * c["written_cells"],
+        # per written cell: its list entry 16 + payload seed 8 + the cell write
+        "mo_final_writes": c["written_cells"] * (16 + 8 + w),
     }
     if commit is not None and batch is not None:
         kind = batch.meta & 3

@@ -172,38 +172,39 @@ int sync_descriptors(ltpmg_engine *e) {
 }
 
 // Ordered-mode scratch for up to ops ops: sort buffers, per-tile scan state
-// and the deferred final writes.
+// and the final-write list.
 void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
-    const size_t st = ops / kSortTile + 1, sg = seg_tiles((uint32_t)ops) + 1;
+    const size_t sg = seg_tiles((uint32_t)ops) + 1;
     for (int i = 0; i < 2; ++i) {
         dev_free(e, ob.sort.keys[i], ops * 4);
         dev_free(e, ob.sort.vals[i], ops * 4);
     }
-    dev_free(e, ob.sort.hist, (st << kMaxDigitBits) * 4);
+    dev_free(e, ob.sort.hist, sort_hist_words(ops) * 4);
     dev_free(e, ob.sort.partials, sort_partials_needed() * 4);
-    dev_free(e, ob.tile_flag, sg * 4 * 5);
-    dev_free(e, ob.deferred, sg * 16);
+    dev_free(e, ob.tile_flag, (sg * 5 + 1) * 4);
+    dev_free(e, ob.wlist, sg * 16);
     ob = OrderBufs{};
 }
 
 int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
-    const size_t st = ops / kSortTile + 1, sg = seg_tiles((uint32_t)ops) + 1;
+    const size_t sg = seg_tiles((uint32_t)ops) + 1;
     int rc = LTPMG_OK;
 #define ALLOC(ptr, n) if (rc == LTPMG_OK) rc = dev_alloc_n(e, &(ptr), (n))
     for (int i = 0; i < 2; ++i) {
         ALLOC(ob.sort.keys[i], ops);
         ALLOC(ob.sort.vals[i], ops);
     }
-    ALLOC(ob.sort.hist, st << kMaxDigitBits);
+    ALLOC(ob.sort.hist, sort_hist_words(ops));
     ALLOC(ob.sort.partials, sort_partials_needed());
-    ALLOC(ob.tile_flag, sg * 5);
-    ALLOC(ob.deferred, sg);
+    ALLOC(ob.tile_flag, sg * 5 + 1);
+    ALLOC(ob.wlist, sg);
 #undef ALLOC
     if (rc != LTPMG_OK) return rc;
     ob.tile_set = ob.tile_flag + sg;
     ob.tile_delta = ob.tile_flag + 2 * sg;
     ob.carry_set = ob.tile_flag + 3 * sg;
     ob.carry_delta = ob.tile_flag + 4 * sg;
+    ob.wtotal = ob.tile_flag + 5 * sg;
     return LTPMG_OK;
 }
 
@@ -863,13 +864,12 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
     HIP_TRY(e, hipSetDevice(e->device));
     hipStream_t st = e->slots[0].stream;
     SortBufs b{};
-    const size_t tiles = n / kSortTile + 1;
     int rc = LTPMG_OK;
     for (int i = 0; i < 2 && rc == LTPMG_OK; ++i) {
         rc = dev_alloc_n(e, &b.keys[i], std::max<size_t>(n, 1));
         if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.vals[i], std::max<size_t>(n, 1));
     }
-    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.hist, tiles << kMaxDigitBits);
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &b.hist, sort_hist_words(n));
     if (rc == LTPMG_OK)
         rc = dev_alloc_n(e, &b.partials, sort_partials_needed());
     if (rc == LTPMG_OK && n) {
@@ -889,7 +889,7 @@ int ltpmg_selftest_sort(ltpmg_engine *e, uint32_t *keys, uint32_t *vals, uint32_
         dev_free(e, b.keys[i], 4ull * std::max<size_t>(n, 1));
         dev_free(e, b.vals[i], 4ull * std::max<size_t>(n, 1));
     }
-    dev_free(e, b.hist, (tiles << kMaxDigitBits) * 4);
+    dev_free(e, b.hist, sort_hist_words(n) * 4);
     dev_free(e, b.partials, 4 * sort_partials_needed());
     return rc;
 }

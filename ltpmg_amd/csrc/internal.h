@@ -182,10 +182,12 @@ hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile
 struct SortBufs {
     uint32_t *keys[2];
     uint32_t *vals[2];
-    uint32_t *hist;      // 2^kMaxDigitBits * n_tiles: per tile, one row of digit counts
+    uint32_t *hist;      // sort_hist_words(n): per tile, one row of digit counts
     uint32_t *partials;  // sort_partials_needed() words
 };
 constexpr uint32_t kMaxDigitBits = 9;  // radix digits of 8 or 9 bits
+// words of SortBufs::hist for a sort of n keys
+size_t sort_hist_words(size_t n);
 // Sorts (keys[0], vals[0]) by the low key_bits bits, stably; returns which
 // buffer index holds the result.
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits);
@@ -209,7 +211,8 @@ struct OrderBufs {
     uint32_t *tile_delta;
     uint32_t *carry_set;    // per tile carry-in
     uint32_t *carry_delta;
-    uint4 *deferred;        // per seg-scan tile: a final write that waits for every tile's reads
+    uint4 *wlist;           // final writes {key, set, d, 0} of segments that span tiles, one per tile at most
+    uint32_t *wtotal;       // entries in wlist (cleared by the seg scan)
 };
 void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys,
                     uint32_t *vals);

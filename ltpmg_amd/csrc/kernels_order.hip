@@ -55,9 +55,9 @@ __device__ inline St elem_state(uint32_t val, const uint64_t *args) {
     return {kNoSet, 0u};
 }
 
-// Exclusive segmented scan over the block's 256 thread aggregates.
-__device__ inline Seg block_seg_exclusive(Seg x, Seg *total) {
-    __shared__ Seg ws[kBlock / 64];
+// Exclusive segmented scan over the block's 256 thread aggregates; ws is
+// kBlock / 64 entries of LDS, free again when the scan returns.
+__device__ inline Seg block_seg_exclusive(Seg x, Seg *total, Seg *ws) {
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     Seg inc = x;
     for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -77,6 +77,34 @@ __device__ inline Seg block_seg_exclusive(Seg x, Seg *total) {
     __syncthreads();
     *total = tot;
     return seg_op(pre, ex);
+}
+__device__ inline Seg block_seg_exclusive(Seg x, Seg *total) {
+    __shared__ Seg ws[kBlock / 64];
+    return block_seg_exclusive(x, total, ws);
+}
+
+// Exclusive sum over the block's threads; ws as for block_seg_exclusive.
+__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total, uint32_t *ws) {
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = x;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t i = 0; i < kBlock / 64; ++i) {
+        if (i < w) pre += ws[i];
+        tot += ws[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - x;
+}
+__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total) {
+    __shared__ uint32_t ws[kBlock / 64];
+    return block_exclusive_sum(x, total, ws);
 }
 
 // Element sources for the segmented scan.
@@ -178,9 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_selftest_reduce(SelftestSrc src, uin
 // rounds is carried in.
 __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
                                                       const uint32_t *tdelta, uint32_t n_tiles,
-                                                      uint32_t *cset, uint32_t *cdelta) {
+                                                      uint32_t *cset, uint32_t *cdelta,
+                                                      uint32_t *zero_word) {
     constexpr uint32_t kPer = 8, kRound = kBlock * kPer;
     __shared__ uint32_t sf[kRound], ss[kRound], sd[kRound];
+    if (zero_word && threadIdx.x == 0) *zero_word = 0u;  // the next stage's counter
     Seg carry{0u, kNoSet, 0u};
     for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
         const uint32_t n = min(kRound, n_tiles - c0);
@@ -334,75 +364,69 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
 // One tile of the sorted (cell, op) list.  Every READ gets its value: the
 // payload of the last UPDATE before it on the cell, else the snapshot, plus
 // the deltas of the RMW_ADDs in between.  A segment (one cell) that starts
-// and ends in this tile has its final value written here, after the block's
-// reads: no other tile reads that cell.  Only the tile's first segment can
-// have started in an earlier tile; if it ends here with a write, that write
-// waits for the next kernel (one deferred entry per tile, no global atomics).
+// and ends in this tile with a write has its final value written here,
+// after the block's reads: no other tile reads that cell.  Only the tile's
+// first segment can have begun earlier (its reads then span tiles); if it
+// ends here with a write, its entry {key, set, d} goes to the batch's
+// final-write list (at most one per tile, one global atomic each) for
+// k_mo_final_writes, which runs after every tile's reads.
 template <int G, bool VEC16>
 __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *cset,
                                                            const uint32_t *cdelta, DevTables dt,
                                                            KeyDecode dec, uint8_t *results,
-                                                           uint32_t res_stride, uint4 *deferred) {
+                                                           uint32_t res_stride, uint4 *wlist,
+                                                           uint32_t *wtotal) {
     // READs from the front {op, set, d, key}; in-tile final writes from the
-    // back {0, set, d, key}.  A write entry needs a write op in its segment,
-    // so reads + write entries <= tile elements and the two ends never meet.
+    // back {key, set, d, 0}.  Such a write entry needs a write op of this
+    // tile in its segment, so reads + write entries <= tile elements and the
+    // two ends never meet.  The list is the kernel's only LDS (exactly
+    // 32 KiB, so five workgroups fit a CU); the two block scans borrow its
+    // first bytes before any entry is written.
     __shared__ uint4 list[kSegTile];
-    __shared__ uint32_t n_rd, n_wr, cont_key;
-    __shared__ uint4 s_def;
+    Seg *seg_ws = reinterpret_cast<Seg *>(list);                   // bytes 0..47
+    uint32_t *sum_ws = reinterpret_cast<uint32_t *>(list) + 16;    // bytes 64..79
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
     uint32_t key[kSegItems], val[kSegItems], prev;
     load8(src, first, key, val, prev);
     const uint32_t next = first + kSegItems < src.n ? src.keys[first + kSegItems] : 0xFFFFFFFFu;
-    if (threadIdx.x == 0) {
-        n_rd = 0;
-        n_wr = 0;
-        cont_key = prev == key[0] ? key[0] : 0xFFFFFFFFu;  // first segment began earlier
-        s_def = make_uint4(0xFFFFFFFFu, kNoSet, 0u, 0u);
-    }
-    const Seg agg = agg8(src, first, key, val, prev);
-    __syncthreads();  // n_rd, n_wr, cont_key, s_def set before any lane uses them
+    // the key of a segment continuing from the previous tile (all-ones: none)
+    const uint32_t t0 = blockIdx.x * kSegTile;
+    const uint32_t cont_key = t0 > 0 && src.keys[t0 - 1] == src.keys[t0] ? src.keys[t0] : 0xFFFFFFFFu;
     Seg tot;
-    Seg pre = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, block_seg_exclusive(agg, &tot));
-    St run{pre.set, pre.d};
-    const uint32_t ck = cont_key;
+    const Seg pre = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]},
+                           block_seg_exclusive(agg8(src, first, key, val, prev), &tot, seg_ws));
+    // item j of the thread: its state before (ex) and after (run) it, and
+    // whether it is a READ (rd) or the end of a written segment (wr)
+    auto walk = [&](auto &&visit) {
+        St run{pre.set, pre.d};
 #pragma unroll
-    for (uint32_t j = 0; j < kSegItems; ++j) {
-        uint32_t i = first + j;
-        bool rd = false, wr = false;
-        uint32_t op = 0;
-        St ex{kNoSet, 0u};
-        if (i < src.n) {
-            op = val[j] & kOpMask;
-            St s = elem_state(val[j], src.args);
-            bool h = key[j] != (j ? key[j - 1] : prev);
-            ex = h ? St{kNoSet, 0u} : run;
+        for (uint32_t j = 0; j < kSegItems; ++j) {
+            if (first + j >= src.n) break;
+            const St s = elem_state(val[j], src.args);
+            const bool h = key[j] != (j ? key[j - 1] : prev);
+            const St ex = h ? St{kNoSet, 0u} : run;
             run = h ? s : combine(run, s);
-            rd = (val[j] >> 30) == LTPMG_OP_READ && !(val[j] & kInvalid);
-            bool last = key[j] != (j + 1 < kSegItems ? key[j + 1] : next);
-            wr = last && run.set != kNoSet;
+            const bool rd = (val[j] >> 30) == LTPMG_OP_READ && !(val[j] & kInvalid);
+            const bool last = key[j] != (j + 1 < kSegItems ? key[j + 1] : next);
+            visit(j, ex, run, rd, last && run.set != kNoSet);
         }
-        const uint32_t k_j = key[j];
-        if (wr && k_j == ck) {  // end of a segment that began in an earlier tile
-            s_def = make_uint4(k_j, run.set, run.d, 0u);
-            wr = false;
-        }
-        uint64_t m = __ballot(rd);
-        if (m) {
-            uint32_t leader = __builtin_ctzll(m), base = 0;
-            if (__lane_id() == leader) base = atomicAdd(&n_rd, (uint32_t)__popcll(m));
-            base = __shfl(base, (int)leader);
-            if (rd) list[base + lanemask_lt_popc(m)] = make_uint4(op, ex.set, ex.d, k_j);
-        }
-        uint64_t mw = __ballot(wr);
-        if (mw) {
-            uint32_t leader = __builtin_ctzll(mw), base = 0;
-            if (__lane_id() == leader) base = atomicAdd(&n_wr, (uint32_t)__popcll(mw));
-            base = __shfl(base, (int)leader);
-            if (wr) list[kSegTile - 1 - (base + lanemask_lt_popc(mw))] = make_uint4(0u, run.set, run.d, k_j);
-        }
-    }
+    };
+    uint32_t n_rd_t = 0, n_wr_t = 0;
+    walk([&](uint32_t j, St, St, bool rd, bool wr) {
+        n_rd_t += rd;
+        n_wr_t += wr && key[j] != cont_key;
+    });
+    // both counts <= kSegTile, so they share one word of the scan
+    uint32_t both;
+    const uint32_t pos = block_exclusive_sum(n_rd_t | (n_wr_t << 16), &both, sum_ws);
+    const uint32_t n_rd = both & 0xFFFFu, n_wr = both >> 16;
+    uint32_t rp = pos & 0xFFFFu, wp = pos >> 16;
+    walk([&](uint32_t j, St ex, St run, bool rd, bool wr) {
+        if (rd) list[rp++] = make_uint4(val[j] & kOpMask, ex.set, ex.d, key[j]);
+        if (wr && key[j] != cont_key) list[kSegTile - 1 - wp++] = make_uint4(key[j], run.set, run.d, 0u);
+        if (wr && key[j] == cont_key) wlist[atomicAdd(wtotal, 1u)] = make_uint4(key[j], run.set, run.d, 0u);
+    });
     __syncthreads();
-    if (threadIdx.x == 0) deferred[blockIdx.x] = s_def;
     move_cells<G, VEC16>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[k];
         key_out = e.w;
@@ -412,22 +436,22 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     __syncthreads();  // every read of this tile's cells is done
     move_cells<G, VEC16>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[kSegTile - 1 - k];
-        key_out = e.w;
+        key_out = e.x;
         st = St{e.y, e.z};
         dst = nullptr;  // the cell itself
     }, dt, dec, src.args, 0u, false, threadIdx.x / G, kBlock / G);
 }
 
-// The deferred final writes, at most one per tile; an entry with an all-ones
-// key is empty and decodes to no cell.
+// The final writes of segments that span tiles, after every tile's reads:
+// entries {key, set, d} from k_mo_apply_reads, *wtotal of them.
 template <int G, bool VEC16>
-__global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *deferred, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, const uint32_t *wtotal,
                                                             DevTables dt, KeyDecode dec,
                                                             const uint64_t *args) {
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    move_cells<G, VEC16>(n, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
-        uint4 e = deferred[k];
+    move_cells<G, VEC16>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
+        uint4 e = wlist[k];
         key = e.x;
         st = St{e.y, e.z};
         dst = nullptr;
@@ -498,9 +522,20 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
         if (idx < n) src.kv(idx, key, val, cache);
         digit[it] = idx < n ? (key >> shift) & (D - 1u) : D;
     }
+    // equal neighbouring digits of the thread are merged first: skewed
+    // digits (Zipf keys put most of them on one high digit) would otherwise
+    // serialise on one LDS counter
+    uint32_t run_d = digit[0], run_n = 0;
 #pragma unroll
-    for (uint32_t it = 0; it < kSortItems; ++it)
-        if (digit[it] < D) atomicAdd(&h[w][digit[it]], 1u);
+    for (uint32_t it = 0; it < kSortItems; ++it) {
+        if (digit[it] != run_d) {
+            if (run_d < D && run_n) atomicAdd(&h[w][run_d], run_n);
+            run_d = digit[it];
+            run_n = 0;
+        }
+        ++run_n;
+    }
+    if (run_d < D && run_n) atomicAdd(&h[w][run_d], run_n);
     __syncthreads();
     // one contiguous row per tile: hist[tile * D + d]
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
@@ -510,30 +545,27 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
     }
 }
 
-__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total);
 
-// Stable scatter: wave w of tile k ranks its 16 chunks of 64 keys in order;
+// LDS of one scatter tile
+template <uint32_t BITS>
+struct ScatterLds {
+    static constexpr uint32_t D = 1u << BITS;
+    uint32_t cnt[kBlock / 64][D];  // per wave and digit: keys ranked so far, then the wave's offset
+    uint32_t gbase[D];             // global position of the tile's first key of each digit
+    uint32_t tile_off[D];          // position of each digit's run inside the tile
+    uint32_t sk[kSortTile], sv[kSortTile];
+};
+
+// Ranks one tile: wave w ranks its kSortItems chunks of 64 keys in order;
 // lanes holding the same digit are found with BITS ballots (wave64
-// multisplit).  The tile is then staged in LDS in (digit, input) order and
-// each digit's run is written contiguously.
+// multisplit).  Leaves per-wave digit counts in L.cnt.
 template <class Src, uint32_t BITS>
-__global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, uint32_t *vout,
-                                                       uint32_t n, uint32_t shift,
-                                                       uint32_t n_tiles, const uint32_t *offs) {
-    constexpr uint32_t D = 1u << BITS, PER = D / kBlock;  // digits per thread
-    static_assert(D % kBlock == 0, "whole digits per thread");
-    __shared__ uint32_t cnt[kBlock / 64][D];
-    __shared__ uint32_t gbase[D];
-    __shared__ uint32_t tile_off[D];
-    __shared__ uint32_t sk[kSortTile], sv[kSortTile];
-    __shared__ typename Src::Cache cache;
+__device__ inline void rank_tile(const Src &src, const typename Src::Cache &cache, uint32_t tile,
+                                 uint32_t n, uint32_t shift, ScatterLds<BITS> &L, uint32_t *k,
+                                 uint32_t *v, uint32_t *r) {
+    constexpr uint32_t D = 1u << BITS;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    for (uint32_t i = threadIdx.x; i < D * (kBlock / 64); i += kBlock) (&cnt[0][0])[i] = 0;
-    for (uint32_t d = threadIdx.x; d < D; d += kBlock) gbase[d] = offs[(uint64_t)blockIdx.x * D + d];
-    src.prepare(cache);
-    __syncthreads();
-    const uint32_t wbase = blockIdx.x * kSortTile + w * (kSortTile / (kBlock / 64));
-    uint32_t k[kSortItems], v[kSortItems], r[kSortItems];
+    const uint32_t wbase = tile * kSortTile + w * (kSortTile / (kBlock / 64));
 #pragma unroll
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
@@ -550,8 +582,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
         uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
         uint32_t prior = 0;
         if (valid && lane == leader) {
-            prior = cnt[w][d];
-            cnt[w][d] = prior + (uint32_t)__popcll(peers);
+            prior = L.cnt[w][d];
+            L.cnt[w][d] = prior + (uint32_t)__popcll(peers);
         }
         prior = __shfl(prior, (int)leader);
         r[c] = prior + lanemask_lt_popc(peers);
@@ -559,54 +591,102 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
         v[c] = val;
         __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
-    {  // per digit: offsets of each wave inside the digit's run, and the run's
-       // start inside the tile (exclusive scan over digits, PER consecutive
-       // digits per thread so thread order is digit order)
-        uint32_t runs[PER], mine = 0;
+}
+
+// Digits per thread when a thread owns PER consecutive digits (thread order
+// is digit order); with fewer digits than threads, threads >= D own none.
+template <uint32_t BITS>
+struct DigitSplit {
+    static constexpr uint32_t D = 1u << BITS;
+    static constexpr uint32_t PER = D >= kBlock ? D / kBlock : 1u;
+    static_assert(D % kBlock == 0 || kBlock % D == 0, "digits split evenly over threads");
+    __device__ static bool owns() { return threadIdx.x * PER < D; }
+};
+
+// After rank_tile and a barrier: per digit, each wave's offset inside the
+// digit's run (in L.cnt) and the run's start inside the tile (L.tile_off,
+// an exclusive scan over digits in thread order).  runs[p] = the tile's
+// count of digit threadIdx.x * PER + p (0 for a thread that owns none).
+template <uint32_t BITS>
+__device__ inline void digit_runs(ScatterLds<BITS> &L, uint32_t *runs) {
+    using DS = DigitSplit<BITS>;
+    constexpr uint32_t PER = DS::PER;
+    const bool own = DS::owns();
+    uint32_t mine = 0;
 #pragma unroll
-        for (uint32_t p = 0; p < PER; ++p) {
-            const uint32_t d = threadIdx.x * PER + p;
-            uint32_t run = 0;
+    for (uint32_t p = 0; p < PER; ++p) {
+        const uint32_t d = threadIdx.x * PER + p;
+        uint32_t run = 0;
+        if (own) {
             for (uint32_t i = 0; i < kBlock / 64; ++i) {
-                uint32_t t = cnt[i][d];
-                cnt[i][d] = run;
+                uint32_t t = L.cnt[i][d];
+                L.cnt[i][d] = run;
                 run += t;
             }
-            runs[p] = run;
-            mine += run;
         }
-        uint32_t total;
-        uint32_t ex = block_exclusive_sum(mine, &total);
-#pragma unroll
-        for (uint32_t p = 0; p < PER; ++p) {
-            tile_off[threadIdx.x * PER + p] = ex;
-            ex += runs[p];
-        }
+        runs[p] = run;
+        mine += run;
     }
-    __syncthreads();  // every tile_off[d] written before any lane reads it
-    // stage the tile in LDS in (digit, input) order ...
+    uint32_t total;
+    uint32_t ex = block_exclusive_sum(mine, &total);
+#pragma unroll
+    for (uint32_t p = 0; p < PER; ++p) {
+        if (own) L.tile_off[threadIdx.x * PER + p] = ex;
+        ex += runs[p];
+    }
+}
+
+// After L.gbase and L.tile_off are set and a barrier: stages the tile in
+// LDS in (digit, input) order, then writes each digit's run contiguously
+// (neighbouring lanes store neighbouring addresses).
+template <uint32_t BITS>
+__device__ inline void stage_and_write(ScatterLds<BITS> &L, uint32_t tile, uint32_t n, uint32_t shift,
+                                       const uint32_t *k, const uint32_t *v, const uint32_t *r,
+                                       uint32_t *kout, uint32_t *vout) {
+    constexpr uint32_t D = 1u << BITS;
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t wbase = tile * kSortTile + w * (kSortTile / (kBlock / 64));
 #pragma unroll
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
         if (idx < n) {
             uint32_t d = (k[c] >> shift) & (D - 1u);
-            uint32_t lp = tile_off[d] + cnt[w][d] + r[c];
-            sk[lp] = k[c];
-            sv[lp] = v[c];
+            uint32_t lp = L.tile_off[d] + L.cnt[w][d] + r[c];
+            L.sk[lp] = k[c];
+            L.sv[lp] = v[c];
         }
     }
     __syncthreads();
-    // ... then write each digit's run contiguously: neighbouring lanes store
-    // neighbouring addresses
-    const uint32_t tile0 = blockIdx.x * kSortTile;
+    const uint32_t tile0 = tile * kSortTile;
     const uint32_t tile_n = n - tile0 < kSortTile ? n - tile0 : kSortTile;
     for (uint32_t j = threadIdx.x; j < tile_n; j += kBlock) {
-        uint32_t key = sk[j], d = (key >> shift) & (D - 1u);
-        uint32_t pos = gbase[d] + (j - tile_off[d]);
+        uint32_t key = L.sk[j], d = (key >> shift) & (D - 1u);
+        uint32_t pos = L.gbase[d] + (j - L.tile_off[d]);
         kout[pos] = key;
-        vout[pos] = sv[j];
+        vout[pos] = L.sv[j];
     }
+}
+
+// Stable scatter of tile blockIdx.x with its digit offsets read from offs
+// (one row of D per tile, from k_rs_offsets).
+template <class Src, uint32_t BITS>
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, uint32_t *vout,
+                                                       uint32_t n, uint32_t shift,
+                                                       uint32_t n_tiles, const uint32_t *offs) {
+    constexpr uint32_t D = 1u << BITS, PER = DigitSplit<BITS>::PER;
+    __shared__ ScatterLds<BITS> L;
+    __shared__ typename Src::Cache cache;
+    for (uint32_t i = threadIdx.x; i < D * (kBlock / 64); i += kBlock) (&L.cnt[0][0])[i] = 0;
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) L.gbase[d] = offs[(uint64_t)blockIdx.x * D + d];
+    src.prepare(cache);
+    __syncthreads();
+    uint32_t k[kSortItems], v[kSortItems], r[kSortItems];
+    rank_tile<Src, BITS>(src, cache, blockIdx.x, n, shift, L, k, v, r);
+    __syncthreads();
+    uint32_t runs[PER];
+    digit_runs<BITS>(L, runs);
+    __syncthreads();  // every tile_off[d] written before any lane reads it
+    stage_and_write<BITS>(L, blockIdx.x, n, shift, k, v, r, kout, vout);
 }
 
 // ---------------- digit offsets of a radix pass ----------------
@@ -619,25 +699,6 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
 // walks its tiles and overwrites the counts with offsets.  Every access
 // reads or writes whole rows.
 
-__device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total) {
-    __shared__ uint32_t ws[kBlock / 64];
-    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint32_t inc = x;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(inc, off);
-        if (lane >= off) inc += y;
-    }
-    if (lane == 63) ws[w] = inc;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-    for (uint32_t i = 0; i < kBlock / 64; ++i) {
-        if (i < w) pre += ws[i];
-        tot += ws[i];
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + inc - x;
-}
 
 constexpr uint32_t kMaxSortChunks = 256;
 constexpr uint32_t kOffsetRows = 16;  // rows loaded together in the walk
@@ -669,8 +730,9 @@ __global__ __launch_bounds__(kBlock) void k_rs_chunk_sum(const uint32_t *hist, u
 template <uint32_t BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_offsets(uint32_t *hist, uint32_t n_tiles,
                                                        const uint32_t *chunks, uint32_t n_chunks) {
-    constexpr uint32_t D = 1u << BITS, PER = D / kBlock;
-    static_assert(D % kBlock == 0, "whole digits per thread");
+    using DS = DigitSplit<BITS>;
+    constexpr uint32_t D = DS::D, PER = DS::PER;
+    const bool own = DS::owns();
     const uint32_t ct = chunk_tiles(n_tiles), t0 = blockIdx.x * ct;
     const uint32_t t1 = t0 + ct < n_tiles ? t0 + ct : n_tiles;
     // thread owns digits threadIdx.x * PER + p, so thread order is digit order
@@ -679,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_offsets(uint32_t *hist, uint32_t 
     for (uint32_t p = 0; p < PER; ++p) {
         const uint32_t d = threadIdx.x * PER + p;
         uint32_t b = 0, a = 0;
-        for (uint32_t c = 0; c < n_chunks; c += kOffsetRows) {  // kOffsetRows loads in flight
+        for (uint32_t c = 0; own && c < n_chunks; c += kOffsetRows) {  // kOffsetRows loads in flight
             uint32_t x[kOffsetRows];
 #pragma unroll
             for (uint32_t r = 0; r < kOffsetRows; ++r) x[r] = c + r < n_chunks ? chunks[(c + r) * D + d] : 0u;
@@ -698,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_offsets(uint32_t *hist, uint32_t 
         run[p] = base + before[p];
         base += total[p];
     }
-    for (uint32_t t = t0; t < t1; t += kOffsetRows) {
+    for (uint32_t t = t0; own && t < t1; t += kOffsetRows) {
         const uint32_t nr = t1 - t < kOffsetRows ? t1 - t : kOffsetRows;
 #pragma unroll
         for (uint32_t p = 0; p < PER; ++p) {
@@ -736,10 +798,14 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
 
 size_t sort_partials_needed() { return (size_t)kMaxSortChunks << kMaxDigitBits; }
 
+size_t sort_hist_words(size_t n) { return (n / kSortTile + 1) << kMaxDigitBits; }
+
 #define LTPMG_RS_BITS(BITS_VAR, KERNEL, SRC, ...)                                            \
     do {                                                                                   \
         if (BITS_VAR == 9)                                                                 \
             hipLaunchKernelGGL((KERNEL<SRC, 9>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else if (BITS_VAR == 7)                                                            \
+            hipLaunchKernelGGL((KERNEL<SRC, 7>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
         else                                                                               \
             hipLaunchKernelGGL((KERNEL<SRC, 8>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
@@ -756,6 +822,10 @@ void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n, uint32_t bits) {
     if (bits == 9) {
         hipLaunchKernelGGL(k_rs_chunk_sum<9>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials);
         hipLaunchKernelGGL(k_rs_offsets<9>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials,
+                           n_chunks);
+    } else if (bits == 7) {
+        hipLaunchKernelGGL(k_rs_chunk_sum<7>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials);
+        hipLaunchKernelGGL(k_rs_offsets<7>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials,
                            n_chunks);
     } else {
         hipLaunchKernelGGL(k_rs_chunk_sum<8>, dim3(n_chunks), dim3(kBlock), 0, s, b.hist, n_tiles, b.partials);
@@ -781,17 +851,23 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
                   0u, n_tiles, b.hist);
 }
 
-// 8-bit digits by default.  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are
-// taken when they save a pass; on MI355X that measured slower for the 27-bit
-// keys of a 10M x 10 table (three 9-bit passes 1.624 ms per batch, four
-// 8-bit passes 1.610 ms): the 9-bit scatter's LDS drops it to 3 workgroups
-// per CU.
+// The pass count of 8-bit digits, with the narrowest digit (7 or 8 bits)
+// that keeps it: at 7 bits a 4096-key tile has 128 digits, so each digit's
+// run in the scatter averages 32 keys, one 128-byte line per array instead
+// of half of one.  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are taken when
+// they save a pass; on MI355X that measured slower for the 27-bit keys of a
+// 10M x 10 table (three 9-bit passes 1.624 ms per batch, four 8-bit passes
+// 1.610 ms): the 9-bit scatter's LDS drops it to 3 workgroups per CU.
 #ifndef LTPMG_RADIX_MAX_BITS
 #define LTPMG_RADIX_MAX_BITS 8
 #endif
+#ifndef LTPMG_RADIX_MIN_BITS
+#define LTPMG_RADIX_MIN_BITS 7
+#endif
 uint32_t radix_digit_bits(uint32_t key_bits) {
-    if (LTPMG_RADIX_MAX_BITS < 9) return 8u;
-    return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9u : 8u;
+    if (LTPMG_RADIX_MAX_BITS >= 9 && (key_bits + 8) / 9 < (key_bits + 7) / 8) return 9u;
+    const uint32_t passes = (key_bits + 7) / 8;
+    return LTPMG_RADIX_MIN_BITS <= 7 && passes * 7 >= key_bits ? 7u : 8u;
 }
 
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits) {
@@ -836,7 +912,7 @@ void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
     hipLaunchKernelGGL(k_seg_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, ob.tile_flag,
                        ob.tile_set, ob.tile_delta);
     hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, ob.tile_flag, ob.tile_set,
-                       ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta);
+                       ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta, ob.wtotal);
 }
 
 void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
@@ -847,17 +923,17 @@ void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
     KeyDecode dec{dt.tabs, n_tabs};
     LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.carry_set, ob.carry_delta, dt, dec,
-                      results, res_stride, ob.deferred);
+                      results, res_stride, ob.wlist, ob.wtotal);
 }
 
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
                             uint32_t n_tabs, uint32_t res_stride) {
     if (ov.n_ops == 0) return;
-    const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
     KeyDecode dec{dt.tabs, n_tabs};
-    // one lane group per entry; res_stride >= every slot, so G lanes cover a cell
+    // at most one entry per scan tile; one lane group per entry
+    const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
     const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
-    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.deferred, n_tiles, dt, dec, ov.args);
+    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.wlist, ob.wtotal, dt, dec, ov.args);
 }
 
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
@@ -868,7 +944,7 @@ void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t 
     uint32_t *cs = scratch + 3 * n_tiles, *cd = scratch + 4 * n_tiles;
     SelftestSrc src{vals, head, n};
     hipLaunchKernelGGL(k_selftest_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts, td);
-    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, tf, ts, td, n_tiles, cs, cd);
+    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, tf, ts, td, n_tiles, cs, cd, nullptr);
     hipLaunchKernelGGL(k_selftest_apply, dim3(n_tiles), dim3(kBlock), 0, s, src, cs, cd, out);
 }
 

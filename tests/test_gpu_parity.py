@@ -262,6 +262,24 @@ def test_ordered_mode_over_many_tiles():
         _compare(e, im, b, MODE_O)
 
 
+@pytest.mark.parametrize("first_kind", [OP_UPDATE, OP_RMW_ADD])
+def test_ordered_mode_tile_of_reads_ends_a_written_segment(first_kind):
+    # Sorted list: cell (0,0) = one write then 2100 READs, cell (0,1) = 2500
+    # READs.  The second 2048-op tile is READs only, yet the segment that
+    # ends in it carries a write from the first tile: its final write must
+    # still be applied, and it must not take a READ's place in that tile.
+    txns = [[(first_kind, 0, 0, 0, 0x1234_5678_9ABC)]]
+    txns += [[(OP_READ, 0, 0, 0, 0)] for _ in range(2100)]
+    txns += [[(OP_READ, 0, 1, 0, 0)] for _ in range(2500)]
+    # the same shape one table over, so a tile can hold the end of one and
+    # the start of the other
+    txns += [[(first_kind, 1, 0, 3, 77)]] + [[(OP_READ, 1, 0, 3, 0)] for _ in range(4095)]
+    b = Batch.from_txns(txns)
+    with Engine() as e:
+        im = _setup(e, {0: (100, [100, 100]), 1: (10, [8])}, 0x4C54504D4700000A)
+        _compare(e, im, b, MODE_O)
+
+
 def _sparse_image(schema: dict, seed: int, batches) -> orc.Image:
     """An oracle image that fills only the cells the batches touch."""
     im = orc.Image(schema)
