@@ -33,6 +33,22 @@ constexpr uint32_t kSortTile = kBlock * kSortItems;
 #define LTPMG_MOVE_DEPTH 4
 #endif
 constexpr int kMoveDepth = LTPMG_MOVE_DEPTH;
+// Tile order across the XCDs.  Workgroups are dealt to the 8 XCDs in turn
+// (workgroup b to XCD b mod 8), and each XCD has its own L2.  xcd_tile maps
+// workgroup b to a tile so each XCD takes one contiguous run of tiles:
+// neighbouring tiles, whose outputs share cache lines, then meet in one L2.
+// For kernels whose tiles carry equal work (the radix scatter); a kernel
+// whose neighbouring tiles are all hot would pile them onto one XCD.  A
+// bijection on [0, n) for any n, so correctness never depends on it.
+#ifndef LTPMG_XCD_TILES
+#define LTPMG_XCD_TILES 1
+#endif
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    if (!LTPMG_XCD_TILES) return b;
+    const uint32_t x = b % kXcds, i = b / kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
 constexpr uint32_t kNoSet = 0xFFFFFFFFu;     // ordered-mode state: no write seen
 constexpr uint32_t kRmwOnly = 0xFFFFFFFEu;   // ordered-mode state: only RMW_ADDs seen
 
