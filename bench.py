@@ -68,9 +68,16 @@ def batch_counts(batch, width: int) -> dict:
     reads = sk == 0
     snapshot_reads = int(np.count_nonzero(reads & (cm == 0)))
     written_cells = int(np.unique(sc[sk != 0]).size)
+    # written cells whose segment crosses a 2048-element tile boundary of the
+    # sorted list: their final write goes through the cross-tile list
+    seg_has_write = np.zeros(int(seg[-1]) + 1 if n else 0, bool)
+    seg_has_write[seg[sk != 0]] = True
+    b = np.arange(SEG_TILE, n, SEG_TILE)
+    crossing = np.unique(seg[b][sc[b] == sc[b - 1]]) if n else np.zeros(0, np.int64)
+    cross = int(np.count_nonzero(seg_has_write[crossing]))
     return {"n_ops": n, "n_txns": batch.n_txns, "reads": int(reads.sum()),
             "writes": int((sk != 0).sum()), "snapshot_reads": snapshot_reads,
-            "written_cells": written_cells,
+            "written_cells": written_cells, "cross_tile_writes": cross,
             "seg_tiles": (n + SEG_TILE - 1) // SEG_TILE, "width": width}
 
 
@@ -85,12 +92,15 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
         "rs_scatter": 16 * n,                  # key+val in and out
         "mo_seg_scan": 8 * n,                  # sorted key+val in
         # sorted key+val; per READ its 100-B result, plus the snapshot cell or
-        # the 8-B payload seed of the in-batch UPDATE it sees; per written
-        # cell its 16-B entry in the final-write list
+        # the 8-B payload seed of the in-batch UPDATE it sees; per cell whose
+        # segment lies in one tile, payload seed 8 + the cell write; per cell
+        # whose segment crosses tiles, its 16-B entry in the cross-tile list
         "mo_apply_reads": 8 * n + c["reads"] * w + c["snapshot_reads"] * w
-                          + 8 * (c["reads"] - c["snapshot_reads"]) + 16 * c["written_cells"],
-        # per written cell: its list entry 16 + payload seed 8 + the cell write
-        "mo_final_writes": c["written_cells"] * (16 + 8 + w),
+                          + 8 * (c["reads"] - c["snapshot_reads"])
+                          + (c["written_cells"] - c["cross_tile_writes"]) * (8 + w)
+                          + 16 * c["cross_tile_writes"],
+        # per cross-tile cell: its list entry 16 + payload seed 8 + the cell write
+        "mo_final_writes": c["cross_tile_writes"] * (16 + 8 + w),
     }
     if commit is not None and batch is not None:
         kind = batch.meta & 3
