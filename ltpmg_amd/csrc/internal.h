@@ -134,10 +134,8 @@ __device__ inline void st16_nt(uint8_t *p, u32x4 v) {
     v4u32 x = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(x, reinterpret_cast<v4u32 *>(p));
 }
-// 16-byte chunk starting at byte b of the UPDATE payload of width w
-__device__ inline u32x4 payload16(uint64_t arg, uint32_t b, uint32_t w) {
-    uint64_t lo = stream64(arg, b >> 3), hi = stream64(arg, (b >> 3) + 1);
-    u32x4 v{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+// zero the words of a 16-byte chunk at byte b that lie past width w
+__device__ inline u32x4 clip16(u32x4 v, uint32_t b, uint32_t w) {
     if (b + 16 > w) {
         if (b + 4 > w) v.x = 0;
         if (b + 8 > w) v.y = 0;
@@ -145,6 +143,11 @@ __device__ inline u32x4 payload16(uint64_t arg, uint32_t b, uint32_t w) {
         v.w = 0;
     }
     return v;
+}
+// 16-byte chunk starting at byte b of the UPDATE payload of width w
+__device__ inline u32x4 payload16(uint64_t arg, uint32_t b, uint32_t w) {
+    uint64_t lo = stream64(arg, b >> 3), hi = stream64(arg, (b >> 3) + 1);
+    return clip16(u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)}, b, w);
 }
 __device__ inline u32x4 add16(u32x4 v, uint32_t d, uint32_t b, uint32_t w) {
     if (b < w) v.x += d;

@@ -292,12 +292,17 @@ struct KeyDecode {
 // HBM requests in flight instead of one.  fetch(k, key, st, dst) gives entry
 // k's cell key, state and destination: a result slot (to_result; span bytes,
 // zero past the slot) or the cell itself (final writes).
-template <int G, bool VEC16, class Fetch>
+// ONE: every slot fits G lanes x 16 bytes (the result stride is exactly
+// G * 16), so each lane moves one 16-byte chunk and the payload words it
+// needs are fixed per lane.
+template <int G, bool VEC16, bool ONE, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
                                   const uint64_t *args, uint32_t res_span, bool to_result,
                                   uint32_t g, uint32_t groups) {
     constexpr int R = kMoveDepth;
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
+    // stream64(s, j) = mix64(s + (j + 1) * golden): the lane's two words
+    const uint64_t gold0 = (uint64_t)((b0 >> 3) + 1) * kGolden, gold1 = gold0 + kGolden;
     for (uint32_t base = g; base < n; base += groups * R) {
         uint32_t key[R];
         St st[R];
@@ -323,10 +328,15 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
                 }
             }
         }
+        // the payload seeds depend only on the entries: issued after every
+        // column-descriptor load (loads complete in order, so waiting for a
+        // descriptor never waits for a seed) and before any descriptor is
+        // used, so a seed's latency overlaps its descriptor's
+#pragma unroll
+        for (int r = 0; r < R; ++r) arg[r] = st[r].set < kRmwOnly ? args[st[r].set] : 0ull;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const bool has_set = st[r].set < kRmwOnly;
-            arg[r] = (cell[r] && has_set) ? args[st[r].set] : 0ull;
             v[r] = u32x4{0u, 0u, 0u, 0u};
             if (VEC16 && cell[r] && !has_set && cd[r].slot >= 16 && b0 < cd[r].slot)
                 v[r] = ld16(cell[r] + b0);
@@ -336,7 +346,21 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
             if (!cell[r]) continue;
             const uint32_t span = to_result ? res_span : cd[r].slot;
             const bool has_set = st[r].set < kRmwOnly;
-            if (VEC16 && cd[r].slot >= 16) {
+            if (ONE && cd[r].slot >= 16) {
+                u32x4 x{0u, 0u, 0u, 0u};
+                if (b0 < cd[r].slot) {
+                    if (has_set) {
+                        const uint64_t lo = mix64(arg[r] + gold0), hi = mix64(arg[r] + gold1);
+                        x = u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+                        x = clip16(x, b0, cd[r].width);
+                    } else {
+                        x = v[r];
+                    }
+                    x = add16(x, st[r].d, b0, cd[r].width);
+                }
+                if (to_result) st16_nt(dst[r] + b0, x);
+                else if (b0 < span) st16(dst[r] + b0, x);
+            } else if (VEC16 && cd[r].slot >= 16) {
                 for (uint32_t b = b0; b < span; b += G * 16) {
                     u32x4 x{0u, 0u, 0u, 0u};
                     if (b < cd[r].slot) {
@@ -370,7 +394,7 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
 // ends here with a write, its entry {key, set, d} goes to the batch's
 // final-write list (at most one per tile, one global atomic each) for
 // k_mo_final_writes, which runs after every tile's reads.
-template <int G, bool VEC16>
+template <int G, bool VEC16, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *cset,
                                                            const uint32_t *cdelta, DevTables dt,
                                                            KeyDecode dec, uint8_t *results,
@@ -431,14 +455,14 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         if (wr && key[j] == cont_key) wlist[atomicAdd(wtotal, 1u)] = make_uint4(key[j], run.set, run.d, 0u);
     });
     __syncthreads();
-    move_cells<G, VEC16>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[k];
         key_out = e.w;
         st = St{e.y, e.z};
         dst = results + (uint64_t)e.x * res_stride;
     }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G);
     __syncthreads();  // every read of this tile's cells is done
-    move_cells<G, VEC16>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[kSegTile - 1 - k];
         key_out = e.x;
         st = St{e.y, e.z};
@@ -448,13 +472,13 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
 
 // The final writes of segments that span tiles, after every tile's reads:
 // entries {key, set, d} from k_mo_apply_reads, *wtotal of them.
-template <int G, bool VEC16>
+template <int G, bool VEC16, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, const uint32_t *wtotal,
                                                             DevTables dt, KeyDecode dec,
                                                             const uint64_t *args) {
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    move_cells<G, VEC16>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
         uint4 e = wlist[k];
         key = e.x;
         st = St{e.y, e.z};
@@ -895,18 +919,25 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint
     hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, keys, vals);
 }
 
-#define LTPMG_MO_DISPATCH(KERNEL, GRID, ...)                                               \
-    do {                                                                                   \
-        if (res_stride >= 128)                                                             \
-            hipLaunchKernelGGL((KERNEL<8, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
-        else if (res_stride >= 64)                                                         \
-            hipLaunchKernelGGL((KERNEL<4, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
-        else if (res_stride >= 32)                                                         \
-            hipLaunchKernelGGL((KERNEL<2, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
-        else if (res_stride >= 16)                                                         \
-            hipLaunchKernelGGL((KERNEL<1, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
-        else                                                                               \
-            hipLaunchKernelGGL((KERNEL<1, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+#define LTPMG_MO_LAUNCH(KERNEL, G_, GRID, ...)                                                 \
+    do {                                                                                       \
+        if (res_stride == G_ * 16)                                                             \
+            hipLaunchKernelGGL((KERNEL<G_, true, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else                                                                                   \
+            hipLaunchKernelGGL((KERNEL<G_, true, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+    } while (0)
+#define LTPMG_MO_DISPATCH(KERNEL, GRID, ...)                                                \
+    do {                                                                                    \
+        if (res_stride >= 128)                                                              \
+            LTPMG_MO_LAUNCH(KERNEL, 8, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 64)                                                          \
+            LTPMG_MO_LAUNCH(KERNEL, 4, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 32)                                                          \
+            LTPMG_MO_LAUNCH(KERNEL, 2, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 16)                                                          \
+            LTPMG_MO_LAUNCH(KERNEL, 1, GRID, __VA_ARGS__);                                  \
+        else                                                                                \
+            hipLaunchKernelGGL((KERNEL<1, false, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
 
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
