@@ -527,7 +527,7 @@ struct OpSrc {
     }
 };
 
-// Digit histogram of one 4096-key tile: per-wave LDS counters, then the
+// Digit histogram of one sort tile: per-wave LDS counters, then the
 // tile's row of D counts, written contiguously.
 template <class Src, uint32_t BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
@@ -729,8 +729,17 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(Src src, uint32_t *kout, 
 // reads or writes whole rows.
 
 
-constexpr uint32_t kMaxSortChunks = 256;
-constexpr uint32_t kOffsetRows = 16;  // rows loaded together in the walk
+// Build-time knobs of the offset scan; 128 chunks with 32 row loads in
+// flight measured fastest for 8192 tiles (0.064 ms for four passes, against
+// 0.10 ms with 256 chunks and 16 loads).
+#ifndef LTPMG_SORT_CHUNKS
+#define LTPMG_SORT_CHUNKS 128
+#endif
+#ifndef LTPMG_OFFSET_ROWS
+#define LTPMG_OFFSET_ROWS 32
+#endif
+constexpr uint32_t kMaxSortChunks = LTPMG_SORT_CHUNKS;  // tile groups of the offset scan
+constexpr uint32_t kOffsetRows = LTPMG_OFFSET_ROWS;     // rows loaded together in the walk
 
 __host__ __device__ inline uint32_t chunk_tiles(uint32_t n_tiles) {
     const uint32_t per = (n_tiles + kMaxSortChunks - 1) / kMaxSortChunks;
@@ -881,9 +890,9 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
 }
 
 // The pass count of 8-bit digits, with the narrowest digit (7 or 8 bits)
-// that keeps it: at 7 bits a 4096-key tile has 128 digits, so each digit's
-// run in the scatter averages 32 keys, one 128-byte line per array instead
-// of half of one.  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are taken when
+// that keeps it: half as many digits make each digit's run in the scatter
+// twice as long, so fewer of its lines are written in part (measured 0.05 ms
+// per C2 batch faster than 8 bits).  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are taken when
 // they save a pass; on MI355X that measured slower for the 27-bit keys of a
 // 10M x 10 table (three 9-bit passes 1.624 ms per batch, four 8-bit passes
 // 1.610 ms): the 9-bit scatter's LDS drops it to 3 workgroups per CU.
