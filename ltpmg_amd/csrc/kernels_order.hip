@@ -200,45 +200,41 @@ __global__ __launch_bounds__(kBlock) void k_selftest_reduce(SelftestSrc src, uin
     }
 }
 
-// One block: carry[k] = segmented fold of tile aggregates 0..k-1.  Rounds of
-// 2048 tiles are staged through LDS with coalesced loads; each thread folds
-// 8 contiguous tiles, one block scan per round, and the fold of all earlier
-// rounds is carried in.
+// Carry-in of every scan tile: an exclusive segmented scan of the tile
+// aggregates, in one workgroup.  Each thread takes kPer consecutive tiles
+// per round and loads them straight into registers (every load in flight at
+// once), so up to kBlock * kPer tiles take one round.
 __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
                                                       const uint32_t *tdelta, uint32_t n_tiles,
                                                       uint32_t *cset, uint32_t *cdelta,
                                                       uint32_t *zero_word) {
-    constexpr uint32_t kPer = 8, kRound = kBlock * kPer;
-    __shared__ uint32_t sf[kRound], ss[kRound], sd[kRound];
+    constexpr uint32_t kPer = 32, kRound = kBlock * kPer;
     if (zero_word && threadIdx.x == 0) *zero_word = 0u;  // the next stage's counter
     Seg carry{0u, kNoSet, 0u};
     for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
-        const uint32_t n = min(kRound, n_tiles - c0);
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
-            sf[i] = tflag[c0 + i];
-            ss[i] = tset[c0 + i];
-            sd[i] = tdelta[c0 + i];
+        const uint32_t lo = c0 + threadIdx.x * kPer;
+        uint32_t f[kPer], st[kPer], d[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {  // past the end: the identity {0, no write, 0}
+            const bool in = lo + k < n_tiles;
+            f[k] = in ? tflag[lo + k] : 0u;
+            st[k] = in ? tset[lo + k] : kNoSet;
+            d[k] = in ? tdelta[lo + k] : 0u;
         }
-        __syncthreads();
-        const uint32_t lo = min(n, threadIdx.x * kPer), hi = min(n, lo + kPer);
         Seg agg{0u, kNoSet, 0u};
-        for (uint32_t k = lo; k < hi; ++k) agg = seg_op(agg, Seg{sf[k], ss[k], sd[k]});
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) agg = seg_op(agg, Seg{f[k], st[k], d[k]});
         Seg tot;
         Seg run = seg_op(carry, block_seg_exclusive(agg, &tot));
-        // each thread overwrites only its own tiles' entries with their carries
-        for (uint32_t k = lo; k < hi; ++k) {
-            const Seg x{sf[k], ss[k], sd[k]};
-            ss[k] = run.set;
-            sd[k] = run.d;
-            run = seg_op(run, x);
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
-            cset[c0 + i] = ss[i];
-            cdelta[c0 + i] = sd[i];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (lo + k < n_tiles) {
+                cset[lo + k] = run.set;
+                cdelta[lo + k] = run.d;
+            }
+            run = seg_op(run, Seg{f[k], st[k], d[k]});
         }
         carry = seg_op(carry, tot);
-        __syncthreads();  // the next round overwrites sf / ss / sd
     }
 }
 

@@ -132,13 +132,16 @@ def test_segscan_random(n, p):
     assert (out == (expect & 0xFFFFFFFF).astype(np.uint32)).all()
 
 
-def test_segscan_many_tiles():
-    # more than 2048 tiles of 2048 elements: the carry kernel takes several rounds
-    n = 9_000_000
+@pytest.mark.parametrize("n", [9_000_000, 36_000_000])
+def test_segscan_many_tiles(n):
+    # thousands of 2048-element tiles; at 36M more than 8192 of them, so the
+    # carry kernel takes three rounds
     rng = np.random.default_rng(9)
     vals = rng.integers(0, 1 << 20, n).astype(np.uint32)
     heads = (rng.random(n) < 1e-4).astype(np.uint8)
-    heads[4_194_304] = 0  # a segment across the first round boundary of tiles
+    for b in (4_194_304, 8192 * 2048, 2 * 8192 * 2048):
+        if b < n:
+            heads[b] = 0  # a segment across a tile boundary / a carry round boundary
     with Engine() as e:
         out = e.selftest_segscan(vals, heads)
     excl = np.cumsum(vals, dtype=np.uint64) - vals
