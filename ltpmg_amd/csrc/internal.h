@@ -32,11 +32,19 @@ constexpr uint32_t kTileOpsMax = kTileOps + LTPMG_MAX_TXN_OPS;
 constexpr uint32_t kSortItems = LTPMG_SORT_ITEMS;
 constexpr uint32_t kSortTile = kBlock * kSortItems;
 // Cell moves in flight per lane group (independent loads issued before the
-// stores of a round); a build-time knob for tuning sweeps.
+// stores of a round); build-time knobs for tuning sweeps.  Lowest-TID-wins
+// moves (kMoveDepth) and ordered-mode moves (kMoveDepthO) are separate: the
+// ordered apply kernel measured fastest at 6 (0.815 ms per C2 batch, against
+// 0.88 at 4, 0.84 at 5 and 0.87 at 7, where its registers drop it to three
+// waves per SIMD).
 #ifndef LTPMG_MOVE_DEPTH
 #define LTPMG_MOVE_DEPTH 4
 #endif
+#ifndef LTPMG_MOVE_DEPTH_O
+#define LTPMG_MOVE_DEPTH_O 6
+#endif
 constexpr int kMoveDepth = LTPMG_MOVE_DEPTH;
+constexpr int kMoveDepthO = LTPMG_MOVE_DEPTH_O;
 // Tile order across the XCDs.  Workgroups are dealt to the 8 XCDs in turn
 // (workgroup b to XCD b mod 8), and each XCD has its own L2.  xcd_tile maps
 // workgroup b to a tile so each XCD takes one contiguous run of tiles:

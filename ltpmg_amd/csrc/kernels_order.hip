@@ -295,7 +295,7 @@ template <int G, bool VEC16, bool ONE, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
                                   const uint64_t *args, uint32_t res_span, bool to_result,
                                   uint32_t g, uint32_t groups) {
-    constexpr int R = kMoveDepth;
+    constexpr int R = kMoveDepthO;
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
     // stream64(s, j) = mix64(s + (j + 1) * golden): the lane's two words
     const uint64_t gold0 = (uint64_t)((b0 >> 3) + 1) * kGolden, gold1 = gold0 + kGolden;
