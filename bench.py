@@ -412,10 +412,19 @@ def main() -> int:
     e.create_table(0, args.rows, [width] * n_cols)
     e.fill_table(0, TABLE_SEED)
     log("table filled")
-    t0 = time.perf_counter()
-    e.submit(batch)
-    e.wait()
-    upload_wall = time.perf_counter() - t0
+    # the batch is uploaded before the timed region; the first upload pays
+    # one-time costs, so it is repeated and the last one is the warm figure
+    uploads = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        e.submit(batch)
+        e.wait()
+        wall = time.perf_counter() - t0
+        e.execute(MODE_O)  # the slot's stats need an executed batch
+        e.wait()
+        uploads.append((wall, e.stats()["upload_ms"]))
+    upload_wall, upload_ms = uploads[-1]
+    log(f"upload: {upload_ms:.2f} ms DMA warm ({uploads[0][1]:.2f} ms first), {upload_wall * 1e3:.1f} ms submit wall")
 
     res = {}
     for name, mode in (("ordered", MODE_O), ("lowest_tid_wins", MODE_A)):
@@ -499,7 +508,8 @@ def main() -> int:
         "p50_commit_latency_ms": modes["ordered"]["p50_commit_latency_ms"],
         "p99_commit_latency_ms": modes["ordered"]["p99_commit_latency_ms"],
         "modes": modes,
-        "upload_ms": head["upload_ms"], "upload_wall_s": upload_wall,
+        "upload_ms": upload_ms, "upload_ms_first": uploads[0][1], "upload_wall_s": upload_wall,
+        "upload_bytes": int(batch.n_ops * 14 + (batch.n_txns + 1) * 4),
         "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
         "batching": batching, "serving_large_schema": serving_large, "batch_counts": counts,
     }

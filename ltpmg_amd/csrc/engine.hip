@@ -85,7 +85,8 @@ struct ltpmg_engine {
     uint32_t *d_sync = nullptr;
     int tail_slot = -1;  // slot of the most recently executed batch
     Table tables[LTPMG_MAX_TABLES];
-    ColDesc *d_cols = nullptr;
+    ColDesc *d_cols = nullptr;  // the sparse descriptor table, then the column list (kColList)
+    uint32_t n_clist = 0;
     TabDesc *d_tabs = nullptr;
     unsigned long long *d_digest = nullptr;
     std::vector<Slot> slots;
@@ -153,20 +154,29 @@ uint32_t n_tabs_span(const ltpmg_engine *e) {
     return n;
 }
 
+DevTables dev_tables(const ltpmg_engine *e) {
+    return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist};
+}
+
 // Writes the device descriptors and assigns ordered-mode global row bases.
 int sync_descriptors(ltpmg_engine *e) {
-    ColDesc cols[LTPMG_MAX_TABLES * LTPMG_MAX_COLS] = {};
+    ColDesc cols[LTPMG_MAX_TABLES * LTPMG_MAX_COLS + kColList] = {};
+    ColDesc *clist = cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS;
     TabDesc tabs[LTPMG_MAX_TABLES] = {};
     uint64_t cell_base = 0;  // past 2^32 - 1 cells only the lowest-TID-wins mode runs
+    uint32_t n_clist = 0;
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
         const Table &tb = e->tables[t];
         if (!tb.live) continue;
         tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u};
         cell_base += (uint64_t)tb.n_rows * tb.n_cols;
-        for (uint32_t c = 0; c < tb.n_cols; ++c)
+        for (uint32_t c = 0; c < tb.n_cols; ++c) {
             cols[t * LTPMG_MAX_COLS + c] = ColDesc{tb.col[c], tb.slot[c], tb.width[c]};
+            if (n_clist < kColList) clist[n_clist++] = cols[t * LTPMG_MAX_COLS + c];
+        }
     }
     HIP_TRY(e, hipMemcpy(e->d_cols, cols, sizeof cols, hipMemcpyHostToDevice));
+    e->n_clist = n_clist;
     HIP_TRY(e, hipMemcpy(e->d_tabs, tabs, sizeof tabs, hipMemcpyHostToDevice));
     return LTPMG_OK;
 }
@@ -357,7 +367,7 @@ int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *t
     const uint32_t n_tiles = std::max(1u, (ov.n_ops + kTileOps - 1) / kTileOps);
     int rc = wait_tables();
     if (rc != LTPMG_OK) return rc;
-    DevTables dt{e->d_cols, e->d_tabs};
+    DevTables dt = dev_tables(e);
     rec.begin("tile_bounds");
     launch_zero_u32(st, counters, 4);
     launch_tile_bounds(st, ov.txn_off, ov.n_txns, n_tiles, tile_t0, e->d_sync);
@@ -397,7 +407,7 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     uint32_t key_bits = 0;
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
-    DevTables dt{e->d_cols, e->d_tabs};
+    DevTables dt = dev_tables(e);
     const uint32_t n_tabs = n_tabs_span(e);
     int sorted = 0;
     if (ov.n_ops > 1 && key_bits > 0) {
@@ -474,7 +484,7 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
         return LTPMG_EDEVICE;
     }
     e->budget = cfg->hbm_budget ? cfg->hbm_budget : (uint64_t)(free_b * 0.9);
-    int rc = dev_alloc_n(e, &e->d_cols, LTPMG_MAX_TABLES * LTPMG_MAX_COLS);
+    int rc = dev_alloc_n(e, &e->d_cols, LTPMG_MAX_TABLES * LTPMG_MAX_COLS + kColList);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_tabs, LTPMG_MAX_TABLES);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_digest, 1);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_sync, 4);
@@ -1096,7 +1106,7 @@ int issue_group(ltpmg_engine *e, ltpmg_stream &s, const Group &g, const ltpmg_st
     if (!g.fused)
         return launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit + t0, results, rs, rec, now);
     GridBar bar{e->d_sync + 1, e->d_sync + 2, e->d_sync + 3, 1u << 24};
-    DevTables dt{e->d_cols, e->d_tabs};
+    DevTables dt = dev_tables(e);
     HIP_TRY(e, launch_cc_fused(s.stream, dt, ov, p.fused_tile_ops, g.grid, e->d_sync, s.d_commit + t0, results,
                                rs, s.d_counters, bar));
     return LTPMG_OK;

@@ -170,9 +170,15 @@ __device__ inline u32x4 add16(u32x4 v, uint32_t d, uint32_t b, uint32_t w) {
 }
 
 // ---- launchers (defined in the kernel translation units) ----
+// The live tables' columns in (table, column) order, at most kColList of
+// them: a kernel copies this list into LDS so a cell's column descriptor
+// costs no global load (KeyDecode gives the position in it).
+constexpr uint32_t kColList = 256;
 struct DevTables {
-    const ColDesc *cols;  // [LTPMG_MAX_TABLES * LTPMG_MAX_COLS]
-    const TabDesc *tabs;  // [LTPMG_MAX_TABLES]
+    const ColDesc *cols;   // [LTPMG_MAX_TABLES * LTPMG_MAX_COLS]
+    const TabDesc *tabs;   // [LTPMG_MAX_TABLES]
+    const ColDesc *clist;  // [kColList]
+    uint32_t n_clist;      // entries of clist in use
 };
 
 void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,

@@ -266,8 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, cons
 struct KeyDecode {
     const TabDesc *tabs;
     uint32_t n_tabs;
-    // cell key -> (table, row, col); returns false when out of range
-    __device__ bool operator()(uint32_t key, uint32_t &tab, uint32_t &row, uint32_t &col) const {
+    // cell key -> (table, row, col) and the column's position in the column
+    // list (DevTables::clist order); returns false when out of range
+    __device__ bool operator()(uint32_t key, uint32_t &tab, uint32_t &row, uint32_t &col,
+                               uint32_t &cpos) const {
+        uint32_t cbase = 0;
         for (uint32_t t = 0; t < n_tabs; ++t) {
             const TabDesc &td = tabs[t];
             const uint32_t off = key - td.cell_base;
@@ -275,8 +278,10 @@ struct KeyDecode {
                 tab = t;
                 row = off / td.n_cols;
                 col = off - row * td.n_cols;
+                cpos = cbase + col;
                 return true;
             }
+            cbase += td.live ? td.n_cols : 0u;
         }
         return false;
     }
@@ -291,10 +296,11 @@ struct KeyDecode {
 // ONE: every slot fits G lanes x 16 bytes (the result stride is exactly
 // G * 16), so each lane moves one 16-byte chunk and the payload words it
 // needs are fixed per lane.
+// lcols: the column list in LDS (its first dt.n_clist entries), or null.
 template <int G, bool VEC16, bool ONE, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
                                   const uint64_t *args, uint32_t res_span, bool to_result,
-                                  uint32_t g, uint32_t groups) {
+                                  uint32_t g, uint32_t groups, const ColDesc *lcols) {
     constexpr int R = kMoveDepthO;
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
     // stream64(s, j) = mix64(s + (j + 1) * golden): the lane's two words
@@ -316,9 +322,9 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
             dst[r] = nullptr;
             if (k < n) {
                 fetch(k, key[r], st[r], dst[r]);
-                uint32_t tab, row, col;
-                if (dec(key[r], tab, row, col)) {
-                    cd[r] = dt.cols[tab * LTPMG_MAX_COLS + col];
+                uint32_t tab, row, col, cpos;
+                if (dec(key[r], tab, row, col, cpos)) {
+                    cd[r] = lcols && cpos < dt.n_clist ? lcols[cpos] : dt.cols[tab * LTPMG_MAX_COLS + col];
                     cell[r] = cd[r].base + (uint64_t)row * cd[r].slot;
                     if (!to_result) dst[r] = cd[r].base + (uint64_t)row * cd[r].slot;
                 }
@@ -403,6 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // 32 KiB, so five workgroups fit a CU); the two block scans borrow its
     // first bytes before any entry is written.
     __shared__ uint4 list[kSegTile];
+    __shared__ ColDesc lcols[kColList];  // the column list: no global load per cell for its descriptor
+    for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kBlock) lcols[i] = dt.clist[i];
     // tiles in launch order: the hottest cells' tiles are neighbours in the
     // sorted list, and dealing them round-robin spreads them over the XCDs
     // (a contiguous run per XCD measured 1.22 ms against 0.88 ms here)
@@ -456,14 +464,14 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         key_out = e.w;
         st = St{e.y, e.z};
         dst = results + (uint64_t)e.x * res_stride;
-    }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G);
+    }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G, lcols);
     __syncthreads();  // every read of this tile's cells is done
     move_cells<G, VEC16, ONE>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
         uint4 e = list[kSegTile - 1 - k];
         key_out = e.x;
         st = St{e.y, e.z};
         dst = nullptr;  // the cell itself
-    }, dt, dec, src.args, 0u, false, threadIdx.x / G, kBlock / G);
+    }, dt, dec, src.args, 0u, false, threadIdx.x / G, kBlock / G, lcols);
 }
 
 // The final writes of segments that span tiles, after every tile's reads:
@@ -479,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, 
         key = e.x;
         st = St{e.y, e.z};
         dst = nullptr;
-    }, dt, dec, args, 0u, false, g, groups);
+    }, dt, dec, args, 0u, false, g, groups, nullptr);
 }
 
 // ---------------- radix sort ----------------

@@ -283,6 +283,31 @@ def test_ordered_mode_tile_of_reads_ends_a_written_segment(first_kind):
         _compare(e, im, b, MODE_O)
 
 
+def test_ordered_mode_past_the_column_list():
+    # 5 tables x 64 columns = 320 columns: the first 256 (in table, column
+    # order) come from the kernel's LDS copy of the column list, the rest
+    # from the descriptor table in global memory
+    widths = [[4, 16, 100, 8][c % 4] for c in range(64)]
+    schema = {t: (40 + t, widths[t:] + widths[:t]) for t in range(5)}
+    rng = np.random.default_rng(17)
+    txns = []
+    for _ in range(6000):
+        ops, seen = [], set()
+        for _ in range(int(rng.integers(1, 9))):
+            t = int(rng.integers(0, 5))
+            r, c = int(rng.integers(0, 40 + t)), int(rng.integers(0, 64))
+            if (t, r, c) in seen:
+                continue
+            seen.add((t, r, c))
+            kind = [OP_READ, OP_UPDATE, OP_RMW_ADD][int(rng.integers(0, 3))]
+            ops.append((kind, t, c, r, int(rng.integers(0, 2**62))))
+        txns.append(ops)
+    b = Batch.from_txns(txns)
+    with Engine() as e:
+        im = _setup(e, schema, 0x4C54504D4700000B)
+        _compare(e, im, b, MODE_O)
+
+
 def _sparse_image(schema: dict, seed: int, batches) -> orc.Image:
     """An oracle image that fills only the cells the batches touch."""
     im = orc.Image(schema)
