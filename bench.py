@@ -91,6 +91,7 @@ def algorithmic_bytes(kernel: str, c: dict, commit=None, batch=None) -> float | 
         "rs_hist": 4 * n,                      # keys in
         "rs_scatter": 16 * n,                  # key+val in and out
         "mo_seg_scan": 8 * n,                  # sorted key+val in
+        "mo_warm_seeds": 8 * n,                # payload seeds streamed through the cache
         # sorted key+val; per READ its 100-B result, plus the snapshot cell or
         # the 8-B payload seed of the in-batch UPDATE it sees; per cell whose
         # segment lies in one tile, payload seed 8 + the cell write; per cell

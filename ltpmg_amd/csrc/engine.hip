@@ -191,7 +191,7 @@ void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
     }
     dev_free(e, ob.sort.hist, sort_hist_words(ops) * 4);
     dev_free(e, ob.sort.partials, sort_partials_needed() * 4);
-    dev_free(e, ob.tile_flag, (sg * 5 + 1) * 4);
+    dev_free(e, ob.tile_flag, (sg * 5 + 2) * 4);
     dev_free(e, ob.wlist, sg * 16);
     ob = OrderBufs{};
 }
@@ -206,7 +206,7 @@ int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
     }
     ALLOC(ob.sort.hist, sort_hist_words(ops));
     ALLOC(ob.sort.partials, sort_partials_needed());
-    ALLOC(ob.tile_flag, sg * 5 + 1);
+    ALLOC(ob.tile_flag, sg * 5 + 2);
     ALLOC(ob.wlist, sg);
 #undef ALLOC
     if (rc != LTPMG_OK) return rc;
@@ -445,6 +445,9 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     rec.end();
     rc = wait_tables();
     if (rc != LTPMG_OK) return rc;
+    rec.begin("mo_warm_seeds");
+    launch_mo_warm_seeds(st, ov, ob);
+    rec.end();
     rec.begin("mo_apply_reads");
     launch_mo_apply_reads(st, dt, ov, ob, sorted, n_tabs, results, rs);
     rec.end();

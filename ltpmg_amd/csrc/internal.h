@@ -249,12 +249,13 @@ struct OrderBufs {
     uint32_t *carry_set;    // per tile carry-in
     uint32_t *carry_delta;
     uint4 *wlist;           // final writes {key, set, d, 0} of segments that span tiles, one per tile at most
-    uint32_t *wtotal;       // entries in wlist (cleared by the seg scan)
+    uint32_t *wtotal;       // entries in wlist (cleared by the seg scan), then one spare word
 };
 void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys,
                     uint32_t *vals);
 // ordered-mode stages, run in this order after launch_mo_keys + launch_radix_sort
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted);
+void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob);
 void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
                            uint32_t n_tabs, uint8_t *results,
                            uint32_t res_stride);

@@ -964,6 +964,32 @@ void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
                        ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta, ob.wtotal);
 }
 
+// Streams the batch's payload seeds (8 bytes per op, in op order) through
+// the memory-side cache just before the apply kernel, whose seed reads come
+// in sorted-cell order, one random 8-byte read each.  The loads' values are
+// folded into a word that is stored only if it equals a constant, so the
+// loads stay.
+__global__ __launch_bounds__(kBlock) void k_mo_warm_seeds(const uint4 *p, uint32_t n, uint32_t *sink) {
+    uint32_t acc = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint4 v = p[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9E3779B9u) *sink = acc;
+}
+
+void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob) {
+    // the MI355X memory-side cache holds 256 MB; a larger seed array would
+    // only push itself out
+    constexpr uint64_t kWarmMax = 192ull << 20;
+    if ((uint64_t)ov.n_ops * 8 > kWarmMax || (reinterpret_cast<uintptr_t>(ov.args) & 15)) return;
+    const uint32_t n4 = ov.n_ops / 2;  // whole 16-byte pairs of seeds
+    if (n4 == 0) return;
+    const uint32_t blocks = std::min<uint32_t>(4096u, (n4 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_mo_warm_seeds, dim3(blocks), dim3(kBlock), 0, s,
+                       reinterpret_cast<const uint4 *>(ov.args), n4, ob.wtotal + 1);
+}
+
 void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
                            uint32_t n_tabs, uint8_t *results, uint32_t res_stride) {
     const uint32_t n = ov.n_ops;
