@@ -504,6 +504,12 @@ struct SortedSrc {
         key = k[i];
         val = v[i];
     }
+    // keys i .. i + 7 (i a multiple of 8, the array 16-byte aligned)
+    __device__ void keys8(uint32_t i, uint32_t *key, const Cache &) const {
+        const uint4 a = reinterpret_cast<const uint4 *>(k + i)[0], b = reinterpret_cast<const uint4 *>(k + i)[1];
+        key[0] = a.x; key[1] = a.y; key[2] = a.z; key[3] = a.w;
+        key[4] = b.x; key[5] = b.y; key[6] = b.z; key[7] = b.w;
+    }
 };
 struct OpSrc {
     const TabDesc *tabs;
@@ -529,11 +535,29 @@ struct OpSrc {
         key = ok ? c.base[t] + row * c.n_cols[t] + op_col(m) : 0u;
         val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
+    __device__ uint32_t key_of(uint16_t m, uint32_t row, const Cache &c) const {
+        const uint32_t t = op_tab(m);
+        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
+                        op_col(m) < c.n_cols[t];
+        return ok ? c.base[t] + row * c.n_cols[t] + op_col(m) : 0u;
+    }
+    // keys of ops o .. o + 7 (o a multiple of 8, rows and meta 16-byte aligned)
+    __device__ void keys8(uint32_t o, uint32_t *key, const Cache &c) const {
+        const uint4 r0 = reinterpret_cast<const uint4 *>(rows + o)[0], r1 = reinterpret_cast<const uint4 *>(rows + o)[1];
+        const uint4 m = reinterpret_cast<const uint4 *>(meta + o)[0];
+        const uint32_t rw[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+        const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) key[j] = key_of((uint16_t)(mw[j >> 1] >> ((j & 1u) * 16)), rw[j], c);
+    }
 };
 
 // Digit histogram of one sort tile: per-wave LDS counters, then the
 // tile's row of D counts, written contiguously.
-template <class Src, uint32_t BITS>
+// VEC: each thread takes kSortItems consecutive keys, read 8 at a time
+// with 16-byte loads (the source arrays are 16-byte aligned); the counts do
+// not depend on which thread takes which key.
+template <class Src, uint32_t BITS, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_t shift,
                                                     uint32_t n_tiles, uint32_t *hist) {
     constexpr uint32_t D = 1u << BITS;
@@ -547,12 +571,24 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
     // every key of the thread first (independent loads in flight), then the
     // LDS counts
     uint32_t digit[kSortItems];
+    const uint32_t vbase = base + threadIdx.x * kSortItems;
+    if (VEC && vbase + kSortItems <= n) {
+        static_assert(kSortItems % 8 == 0, "whole groups of 8 keys");
 #pragma unroll
-    for (uint32_t it = 0; it < kSortItems; ++it) {
-        const uint32_t idx = base + it * kBlock + threadIdx.x;
-        uint32_t key = 0u, val = 0u;
-        if (idx < n) src.kv(idx, key, val, cache);
-        digit[it] = idx < n ? (key >> shift) & (D - 1u) : D;
+        for (uint32_t g = 0; g < kSortItems; g += 8) {
+            uint32_t key[8];
+            src.keys8(vbase + g, key, cache);
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) digit[g + j] = (key[j] >> shift) & (D - 1u);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t it = 0; it < kSortItems; ++it) {
+            const uint32_t idx = VEC ? vbase + it : base + it * kBlock + threadIdx.x;
+            uint32_t key = 0u, val = 0u;
+            if (idx < n) src.kv(idx, key, val, cache);
+            digit[it] = idx < n ? (key >> shift) & (D - 1u) : D;
+        }
     }
     // equal neighbouring digits of the thread are merged first: skewed
     // digits (Zipf keys put most of them on one high digit) would otherwise
@@ -852,10 +888,23 @@ size_t sort_hist_words(size_t n) { return (n / kSortTile + 1) << kMaxDigitBits; 
             hipLaunchKernelGGL((KERNEL<SRC, 8>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
 
+#define LTPMG_RS_HIST(BITS_VAR, VEC, SRC, ...)                                                       \
+    do {                                                                                            \
+        if (BITS_VAR == 9)                                                                          \
+            hipLaunchKernelGGL((k_rs_hist<SRC, 9, VEC>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else if (BITS_VAR == 7)                                                                     \
+            hipLaunchKernelGGL((k_rs_hist<SRC, 7, VEC>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+        else                                                                                        \
+            hipLaunchKernelGGL((k_rs_hist<SRC, 8, VEC>), dim3(n_tiles), dim3(kBlock), 0, s, __VA_ARGS__); \
+    } while (0)
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits) {
     const uint32_t n_tiles = (n + kSortTile - 1) / kSortTile;
-    LTPMG_RS_BITS(bits, k_rs_hist, SortedSrc, SortedSrc{b.keys[cur], b.vals[cur]}, n, shift, n_tiles,
-                  b.hist);
+    const SortedSrc src{b.keys[cur], b.vals[cur]};
+    if (aligned16(src.k)) LTPMG_RS_HIST(bits, true, SortedSrc, src, n, shift, n_tiles, b.hist);
+    else LTPMG_RS_HIST(bits, false, SortedSrc, src, n, shift, n_tiles, b.hist);
 }
 
 void launch_rs_scan(hipStream_t s, SortBufs &b, uint32_t n, uint32_t bits) {
@@ -884,7 +933,9 @@ void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t
 
 void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
     const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
-    LTPMG_RS_BITS(bits, k_rs_hist, OpSrc, OpSrc{dt.tabs, ov.rows, ov.meta}, n, 0u, n_tiles, b.hist);
+    const OpSrc src{dt.tabs, ov.rows, ov.meta};
+    if (aligned16(ov.rows) && aligned16(ov.meta)) LTPMG_RS_HIST(bits, true, OpSrc, src, n, 0u, n_tiles, b.hist);
+    else LTPMG_RS_HIST(bits, false, OpSrc, src, n, 0u, n_tiles, b.hist);
 }
 
 void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
