@@ -184,7 +184,7 @@ int sync_descriptors(ltpmg_engine *e) {
 // Ordered-mode scratch for up to ops ops: sort buffers, per-tile scan state
 // and the final-write list.
 void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
-    const size_t sg = seg_tiles((uint32_t)ops) + 1;
+    const size_t sg = (seg_tiles((uint32_t)ops) + 4) & ~(size_t)3;  // a multiple of 4: 16-byte aligned arrays
     for (int i = 0; i < 2; ++i) {
         dev_free(e, ob.sort.keys[i], ops * 4);
         dev_free(e, ob.sort.vals[i], ops * 4);
@@ -197,7 +197,7 @@ void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
 }
 
 int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
-    const size_t sg = seg_tiles((uint32_t)ops) + 1;
+    const size_t sg = (seg_tiles((uint32_t)ops) + 4) & ~(size_t)3;  // a multiple of 4: 16-byte aligned arrays
     int rc = LTPMG_OK;
 #define ALLOC(ptr, n) if (rc == LTPMG_OK) rc = dev_alloc_n(e, &(ptr), (n))
     for (int i = 0; i < 2; ++i) {

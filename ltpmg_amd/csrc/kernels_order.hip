@@ -210,16 +210,30 @@ __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, con
                                                       uint32_t *zero_word) {
     constexpr uint32_t kPer = 32, kRound = kBlock * kPer;
     if (zero_word && threadIdx.x == 0) *zero_word = 0u;  // the next stage's counter
+    const bool vec = ((reinterpret_cast<uintptr_t>(tflag) | reinterpret_cast<uintptr_t>(tset) |
+                       reinterpret_cast<uintptr_t>(tdelta)) & 15u) == 0;
     Seg carry{0u, kNoSet, 0u};
     for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
         const uint32_t lo = c0 + threadIdx.x * kPer;
         uint32_t f[kPer], st[kPer], d[kPer];
+        if (vec && lo + kPer <= n_tiles) {  // 16-byte loads
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {  // past the end: the identity {0, no write, 0}
-            const bool in = lo + k < n_tiles;
-            f[k] = in ? tflag[lo + k] : 0u;
-            st[k] = in ? tset[lo + k] : kNoSet;
-            d[k] = in ? tdelta[lo + k] : 0u;
+            for (uint32_t k = 0; k < kPer; k += 4) {
+                const uint4 a = *reinterpret_cast<const uint4 *>(tflag + lo + k);
+                const uint4 b = *reinterpret_cast<const uint4 *>(tset + lo + k);
+                const uint4 c = *reinterpret_cast<const uint4 *>(tdelta + lo + k);
+                f[k] = a.x; f[k + 1] = a.y; f[k + 2] = a.z; f[k + 3] = a.w;
+                st[k] = b.x; st[k + 1] = b.y; st[k + 2] = b.z; st[k + 3] = b.w;
+                d[k] = c.x; d[k + 1] = c.y; d[k + 2] = c.z; d[k + 3] = c.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {  // past the end: the identity {0, no write, 0}
+                const bool in = lo + k < n_tiles;
+                f[k] = in ? tflag[lo + k] : 0u;
+                st[k] = in ? tset[lo + k] : kNoSet;
+                d[k] = in ? tdelta[lo + k] : 0u;
+            }
         }
         Seg agg{0u, kNoSet, 0u};
 #pragma unroll
