@@ -135,21 +135,47 @@ __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &
 }
 
 
-// Pass 1 on one tile: every valid write op offers its TID for its row.
+// Pass 1 on one tile: every valid write op offers its TID for its row.  The
+// tile's ops on one row first meet in an LDS hash (key = table << 32 | row,
+// value = the lowest TID), so under Zipf a hot row costs the tile one
+// global check and at most one atomicMax instead of one per op.  An op
+// whose slot holds another row goes to memory by itself.  keys / tids are
+// n_slots (a power of two) words of LDS scratch; the check reads the lock
+// word at device scope, so it skips the atomic when a lower TID's tag (or
+// ours) is already there.
 __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const Tile &t,
-                                    const uint16_t *owner, uint32_t epoch) {
+                                    const uint16_t *owner, uint32_t epoch, unsigned long long *keys,
+                                    uint32_t *tids, uint32_t n_slots) {
+    constexpr unsigned long long kEmpty = ~0ull;
     const unsigned long long ep = (unsigned long long)epoch << 32;
+    auto offer = [&](uint32_t tab, uint32_t row, uint32_t tid) {
+        const TabDesc &td = dt.tabs[tab];
+        const unsigned long long tag = ep | (uint32_t)~tid;
+        unsigned long long *p = td.lock + row;
+        const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur < tag) atomicMax(p, tag);
+    };
+    for (uint32_t i = threadIdx.x; i < n_slots; i += blockDim.x) {
+        keys[i] = kEmpty;
+        tids[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < t.nops; i += blockDim.x) {
         uint32_t o = t.op0 + i;
         uint16_t m = ov.meta[o];
         if (op_kind(m) == LTPMG_OP_READ) continue;
         uint32_t row = ov.rows[o];
         if (!op_valid(dt.tabs, m, row)) continue;  // never touch memory outside the tables
-        const TabDesc &td = dt.tabs[op_tab(m)];
-        unsigned long long tag = ep | (uint32_t)~(t.t0 + owner[i]);
-        unsigned long long *p = td.lock + row;
-        unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur < tag) atomicMax(p, tag);
+        const unsigned long long key = ((unsigned long long)op_tab(m) << 32) | row;
+        const uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n_slots - 1);
+        const unsigned long long prev = atomicCAS(&keys[slot], kEmpty, key);
+        if (prev == kEmpty || prev == key) atomicMin(&tids[slot], t.t0 + owner[i]);
+        else offer(op_tab(m), row, t.t0 + owner[i]);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n_slots; i += blockDim.x) {
+        const unsigned long long key = keys[i];
+        if (key != kEmpty) offer((uint32_t)(key >> 32), (uint32_t)key, tids[i]);
     }
 }
 
@@ -310,13 +336,17 @@ __global__ void k_tile_bounds(const uint32_t *txn_off, uint32_t n_txns, uint32_t
     tile_t0[k] = lo;
 }
 
+constexpr uint32_t kReserveSlots = 2048;  // LDS hash of a tile's written rows
+
 __global__ __launch_bounds__(kBlock) void k_cc_reserve(DevTables dt, OpView ov,
                                                        const uint32_t *tile_t0,
                                                        const uint32_t *epoch_ctr) {
     __shared__ uint16_t owner[kTileOpsMax];
+    __shared__ unsigned long long keys[kReserveSlots];
+    __shared__ uint32_t tids[kReserveSlots];
     const uint32_t epoch = *epoch_ctr;
     Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
-    tile_reserve(dt, ov, t, owner, epoch);
+    tile_reserve(dt, ov, t, owner, epoch, keys, tids, kReserveSlots);
 }
 
 // G lanes cooperate on one op; each moves 16 bytes (VEC16) or a 4-byte word.
@@ -359,7 +389,11 @@ __global__ __launch_bounds__(kBlock) void k_cc_fused(DevTables dt, OpView ov, ui
                                                      GridBar bar) {
     __shared__ uint16_t owner[TILE + LTPMG_MAX_TXN_OPS];
     __shared__ uint8_t ok[TILE];
-    __shared__ uint32_t list[TILE + LTPMG_MAX_TXN_OPS];
+    // list: the validate and write passes' op list; the reserve pass, which
+    // runs before them, uses it as its LDS hash (12 bytes per slot)
+    __shared__ __align__(16) uint32_t list[TILE + LTPMG_MAX_TXN_OPS];
+    constexpr uint32_t kSlots = TILE / 4;  // 12 * TILE / 4 bytes <= sizeof list
+    static_assert(12 * kSlots <= 4 * (TILE + LTPMG_MAX_TXN_OPS), "hash fits the list");
     __shared__ uint32_t s_t0[kFusedMaxTiles + 1];
     const uint32_t epoch = *epoch_ctr + 1;  // every workgroup reads it before the first barrier
     const uint32_t k0 = min(n_tiles, blockIdx.x * tpb), k1 = min(n_tiles, k0 + tpb);
@@ -388,7 +422,8 @@ __global__ __launch_bounds__(kBlock) void k_cc_fused(DevTables dt, OpView ov, ui
     Tile t{0u, 0u, 0u, 0u};
     for (uint32_t j = 0; j < nk; ++j) {
         t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
-        tile_reserve(dt, ov, t, owner, epoch);
+        tile_reserve(dt, ov, t, owner, epoch, reinterpret_cast<unsigned long long *>(list),
+                     list + 2 * kSlots, kSlots);
     }
     if (!grid_sync(bar)) return;
     uint32_t committed = 0;
