@@ -8,7 +8,8 @@
 // it is three passes over the SoA batch:
 //
 //   cc_reserve        write op of txn t on row k: atomicMax(lock[k], epoch<<32 | ~t)
-//                     (skipped when a plain load already shows a lower TID)
+//                     (a tile's ops on one row meet in LDS first; skipped when
+//                     the lock already shows a lower TID)
 //   cc_validate_read  txn t commits iff no op's lock holds a lower TID of this
 //                     epoch; committed READs copy the cell to the result slot
 //   cc_write          committed writes (UPDATE payload / RMW_ADD)

@@ -7,16 +7,20 @@
 // applies the list serially (2525-2700).  Under zipf keys the hottest row's
 // serial list is the tail.  Here the order is made data-parallel instead:
 //
-//   mo_keys       key = cell number (table by table, row-major), val = op | kind<<30
-//   radix sort    stable LSD sort on key, 8-bit digits (ops are already in
-//                 TID order, so a stable sort yields per-cell TID order)
+//   radix sort    stable LSD sort on key = cell number (table by table,
+//                 row-major), val = op | kind<<30; the first pass builds both
+//                 from the ops (mo_keys only for one-op batches); 7- or 8-bit
+//                 digits (ops are already in TID order, so a stable sort
+//                 yields per-cell TID order)
 //   seg reduce /  segmented scan of the per-cell state monoid
 //   carry / apply   state = (last UPDATE op or none, sum of RMW deltas since),
 //                 combine(a, b) = b has an UPDATE ? b : (a.set, a.d + b.d);
 //                 a READ's value = (set ? payload(args[set]) : snapshot) + d,
 //                 so no cell is ever walked serially
-//   final writes  one write per touched cell (the segment's inclusive state),
-//                 in a later kernel so every snapshot read has finished.
+//   final writes  one write per written cell (the segment's inclusive
+//                 state): in the apply kernel after its tile's reads when the
+//                 segment lies in one tile, else in a later kernel, after
+//                 every tile's reads.
 #include "internal.h"
 
 namespace ltpmg {
