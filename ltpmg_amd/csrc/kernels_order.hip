@@ -308,26 +308,26 @@ struct KeyDecode {
 // Pipelined cell moves.  A group of G lanes takes R entries per round and
 // issues every independent source load (the payload seed of the UPDATE an
 // entry sees, or the snapshot cell) before any store, so each lane keeps R
-// HBM requests in flight instead of one.  fetch(k, key, st, dst) gives entry
-// k's cell key, state and destination: a result slot (to_result; span bytes,
-// zero past the slot) or the cell itself (final writes).
-// ONE: every slot fits G lanes x 16 bytes (the result stride is exactly
-// G * 16), so each lane moves one 16-byte chunk and the payload words it
-// needs are fixed per lane.
-// lcols: the column list in LDS (its first dt.n_clist entries), or null.
+// HBM requests in flight instead of one.  fetch(k, key, st, op) gives entry
+// k's cell key, state and destination: op's result slot (res_span bytes at
+// results + op * res_span, zero past the cell) or, for op = kCellOp, the
+// cell itself.  lcols: the column list in LDS (its first dt.n_clist
+// entries), or null.  Per entry a round keeps only what its stores need
+// (the cell address, slot and width packed in one word, the op), so six
+// entries fit the registers of four waves per SIMD.
+constexpr uint32_t kCellOp = 0xFFFFFFFFu;
 template <int G, bool VEC16, bool ONE, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
-                                  const uint64_t *args, uint32_t res_span, bool to_result,
+                                  const uint64_t *args, uint8_t *results, uint32_t res_span,
                                   uint32_t g, uint32_t groups, const ColDesc *lcols) {
     constexpr int R = kMoveDepthO;
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
     // stream64(s, j) = mix64(s + (j + 1) * golden): the lane's two words
     const uint64_t gold0 = (uint64_t)((b0 >> 3) + 1) * kGolden, gold1 = gold0 + kGolden;
     for (uint32_t base = g; base < n; base += groups * R) {
-        uint32_t key[R];
         St st[R];
-        uint8_t *dst[R];
-        ColDesc cd[R];
+        uint32_t op[R];
+        uint32_t sw[R];  // slot | width << 16 (both <= LTPMG_MAX_COL_BYTES)
         const uint8_t *cell[R];
         uint64_t arg[R];
         u32x4 v[R];
@@ -335,16 +335,18 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
         for (int r = 0; r < R; ++r) {
             const uint32_t k = base + r * groups;
             cell[r] = nullptr;
-            cd[r] = ColDesc{nullptr, 0u, 0u};
+            sw[r] = 0u;
             st[r] = St{kNoSet, 0u};
-            dst[r] = nullptr;
+            op[r] = kCellOp;
             if (k < n) {
-                fetch(k, key[r], st[r], dst[r]);
+                uint32_t key;
+                fetch(k, key, st[r], op[r]);
                 uint32_t tab, row, col, cpos;
-                if (dec(key[r], tab, row, col, cpos)) {
-                    cd[r] = lcols && cpos < dt.n_clist ? lcols[cpos] : dt.cols[tab * LTPMG_MAX_COLS + col];
-                    cell[r] = cd[r].base + (uint64_t)row * cd[r].slot;
-                    if (!to_result) dst[r] = cd[r].base + (uint64_t)row * cd[r].slot;
+                if (dec(key, tab, row, col, cpos)) {
+                    const ColDesc cd =
+                        lcols && cpos < dt.n_clist ? lcols[cpos] : dt.cols[tab * LTPMG_MAX_COLS + col];
+                    cell[r] = cd.base + (uint64_t)row * cd.slot;
+                    sw[r] = cd.slot | (cd.width << 16);
                 }
             }
         }
@@ -357,48 +359,51 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const bool has_set = st[r].set < kRmwOnly;
+            const uint32_t slot = sw[r] & 0xFFFFu;
             v[r] = u32x4{0u, 0u, 0u, 0u};
-            if (VEC16 && cell[r] && !has_set && cd[r].slot >= 16 && b0 < cd[r].slot)
-                v[r] = ld16(cell[r] + b0);
+            if (VEC16 && cell[r] && !has_set && slot >= 16 && b0 < slot) v[r] = ld16(cell[r] + b0);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!cell[r]) continue;
-            const uint32_t span = to_result ? res_span : cd[r].slot;
+            const uint32_t slot = sw[r] & 0xFFFFu, width = sw[r] >> 16;
+            const bool to_result = op[r] != kCellOp;
+            uint8_t *dst = to_result ? results + (uint64_t)op[r] * res_span : const_cast<uint8_t *>(cell[r]);
+            const uint32_t span = to_result ? res_span : slot;
             const bool has_set = st[r].set < kRmwOnly;
-            if (ONE && cd[r].slot >= 16) {
+            if (ONE && slot >= 16) {
                 u32x4 x{0u, 0u, 0u, 0u};
-                if (b0 < cd[r].slot) {
+                if (b0 < slot) {
                     if (has_set) {
                         const uint64_t lo = mix64(arg[r] + gold0), hi = mix64(arg[r] + gold1);
                         x = u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-                        x = clip16(x, b0, cd[r].width);
+                        x = clip16(x, b0, width);
                     } else {
                         x = v[r];
                     }
-                    x = add16(x, st[r].d, b0, cd[r].width);
+                    x = add16(x, st[r].d, b0, width);
                 }
-                if (to_result) st16_nt(dst[r] + b0, x);
-                else if (b0 < span) st16(dst[r] + b0, x);
-            } else if (VEC16 && cd[r].slot >= 16) {
+                if (to_result) st16_nt(dst + b0, x);
+                else if (b0 < span) st16(dst + b0, x);
+            } else if (VEC16 && slot >= 16) {
                 for (uint32_t b = b0; b < span; b += G * 16) {
                     u32x4 x{0u, 0u, 0u, 0u};
-                    if (b < cd[r].slot) {
-                        x = has_set ? payload16(arg[r], b, cd[r].width) : (b == b0 ? v[r] : ld16(cell[r] + b));
-                        x = add16(x, st[r].d, b, cd[r].width);
+                    if (b < slot) {
+                        x = has_set ? payload16(arg[r], b, width) : (b == b0 ? v[r] : ld16(cell[r] + b));
+                        x = add16(x, st[r].d, b, width);
                     }
-                    if (to_result) st16_nt(dst[r] + b, x);
-                    else st16(dst[r] + b, x);
+                    if (to_result) st16_nt(dst + b, x);
+                    else st16(dst + b, x);
                 }
             } else {  // 4- and 8-byte slots: one word per lane step
                 for (uint32_t b = lane * 4; b < span; b += G * 4) {
                     uint32_t x = 0;
-                    if (b < cd[r].width) {
+                    if (b < width) {
                         x = has_set ? stream_word(arg[r], b >> 2)
                                     : *reinterpret_cast<const uint32_t *>(cell[r] + b);
                         x += st[r].d;
                     }
-                    *reinterpret_cast<uint32_t *>(dst[r] + b) = x;
+                    *reinterpret_cast<uint32_t *>(dst + b) = x;
                 }
             }
         }
@@ -423,9 +428,10 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // READs from the front {op, set, d, key}; in-tile final writes from the
     // back {key, set, d, 0}.  Such a write entry needs a write op of this
     // tile in its segment, so reads + write entries <= tile elements and the
-    // two ends never meet.  The list is the kernel's only LDS (exactly
-    // 32 KiB, so five workgroups fit a CU); the two block scans borrow its
-    // first bytes before any entry is written.
+    // two ends never meet.  With the column list beside it the kernel takes
+    // 36 KiB of LDS, four workgroups per CU (the register count allows the
+    // same four waves per SIMD); the two block scans borrow the list's first
+    // bytes before any entry is written.
     __shared__ uint4 list[kSegTile];
     __shared__ ColDesc lcols[kColList];  // the column list: no global load per cell for its descriptor
     for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kBlock) lcols[i] = dt.clist[i];
@@ -477,19 +483,19 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         if (wr && key[j] == cont_key) wlist[atomicAdd(wtotal, 1u)] = make_uint4(key[j], run.set, run.d, 0u);
     });
     __syncthreads();
-    move_cells<G, VEC16, ONE>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
         uint4 e = list[k];
         key_out = e.w;
         st = St{e.y, e.z};
-        dst = results + (uint64_t)e.x * res_stride;
-    }, dt, dec, src.args, res_stride, true, threadIdx.x / G, kBlock / G, lcols);
+        op = e.x;
+    }, dt, dec, src.args, results, res_stride, threadIdx.x / G, kBlock / G, lcols);
     __syncthreads();  // every read of this tile's cells is done
-    move_cells<G, VEC16, ONE>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
         uint4 e = list[kSegTile - 1 - k];
         key_out = e.x;
         st = St{e.y, e.z};
-        dst = nullptr;  // the cell itself
-    }, dt, dec, src.args, 0u, false, threadIdx.x / G, kBlock / G, lcols);
+        op = kCellOp;  // the cell itself
+    }, dt, dec, src.args, results, res_stride, threadIdx.x / G, kBlock / G, lcols);
 }
 
 // The final writes of segments that span tiles, after every tile's reads:
@@ -500,12 +506,12 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, 
                                                             const uint64_t *args) {
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    move_cells<G, VEC16, ONE>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint8_t *&dst) {
+    move_cells<G, VEC16, ONE>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint32_t &op) {
         uint4 e = wlist[k];
         key = e.x;
         st = St{e.y, e.z};
-        dst = nullptr;
-    }, dt, dec, args, 0u, false, g, groups, nullptr);
+        op = kCellOp;
+    }, dt, dec, args, nullptr, 0u, g, groups, nullptr);
 }
 
 // ---------------- radix sort ----------------
