@@ -222,13 +222,15 @@ struct SortBufs {
     uint32_t *hist;      // sort_hist_words(n): per tile, one row of digit counts
     uint32_t *partials;  // sort_partials_needed() words
 };
-constexpr uint32_t kMaxDigitBits = 9;  // radix digits of 8 or 9 bits
+constexpr uint32_t kMaxDigitBits = 9;  // radix digits of 7, 8 or 9 bits
 // words of SortBufs::hist for a sort of n keys
 size_t sort_hist_words(size_t n);
 // Sorts (keys[0], vals[0]) by the low key_bits bits, stably; returns which
 // buffer index holds the result.
 int launch_radix_sort(hipStream_t s, SortBufs &b, uint32_t n, uint32_t key_bits);
-// digit width for key_bits: 9 when that saves a pass, else 8
+// digit width for key_bits: the narrowest of 7 or 8 bits that keeps the
+// 8-bit pass count (9 bits, when that saves a pass, only if built with
+// LTPMG_RADIX_MAX_BITS=9)
 uint32_t radix_digit_bits(uint32_t key_bits);
 // one LSD pass, split so each kernel can be timed: hist, scan, scatter
 void launch_rs_hist(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t shift, uint32_t bits);

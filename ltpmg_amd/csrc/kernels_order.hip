@@ -973,8 +973,9 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
 // twice as long, so fewer of its lines are written in part (measured 0.05 ms
 // per C2 batch faster than 8 bits).  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are taken when
 // they save a pass; on MI355X that measured slower for the 27-bit keys of a
-// 10M x 10 table (three 9-bit passes 1.624 ms per batch, four 8-bit passes
-// 1.610 ms): the 9-bit scatter's LDS drops it to 3 workgroups per CU.
+// 10M x 10 table (three 9-bit passes 1.28 ms per batch, four 7-bit passes
+// 1.22 ms): a 9-bit scatter pass takes 0.10 ms against 0.07 ms, and its
+// digit-offset scan reads four times the counts.
 #ifndef LTPMG_RADIX_MAX_BITS
 #define LTPMG_RADIX_MAX_BITS 8
 #endif
