@@ -363,7 +363,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample-txns", type=int, default=1 << 20)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8],
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8, 2e8],
                     help="arrival rates (txn/s) for the served-vs-plain comparison; none to skip")
     ap.add_argument("--serve-arrivals", type=int, default=1 << 22)
     ap.add_argument("--plain-batch", type=int, default=1 << 18)

@@ -238,13 +238,21 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
                     uint32_t pt = pool_txn(entries[first + j]);
                     st.txn_off[j + 1] = st.txn_off[j] + (pool->txn_off[pt + 1] - pool->txn_off[pt]);
                 }
+                // arrivals in order map to consecutive pool txns, whose ops
+                // are contiguous in the pool: each such run is one copy per
+                // array (a retried txn or the pool's wrap ends a run)
                 auto copy = [&](uint32_t a, uint32_t b) {
-                    for (uint32_t j = a; j < b; ++j) {
-                        uint32_t pt = pool_txn(entries[first + j]);
-                        uint32_t src = pool->txn_off[pt], len = pool->txn_off[pt + 1] - src, dst = st.txn_off[j];
+                    for (uint32_t j = a; j < b;) {
+                        const uint32_t pt = pool_txn(entries[first + j]);
+                        uint32_t end = j + 1, pt_end = pt + 1;
+                        while (end < b && pt_end < pool->n_txns && pool_txn(entries[first + end]) == pt_end)
+                            ++end, ++pt_end;
+                        const uint32_t src = pool->txn_off[pt], len = pool->txn_off[pt_end] - src;
+                        const uint32_t dst = st.txn_off[j];
                         std::memcpy(st.rows + dst, pool->rows + src, (size_t)len * 4);
                         std::memcpy(st.meta + dst, pool->meta + src, (size_t)len * 2);
                         std::memcpy(st.args + dst, pool->args + src, (size_t)len * 8);
+                        j = end;
                     }
                 };
                 // the workers pay off once a batch carries a few MB of ops
