@@ -197,6 +197,21 @@ def test_multi_table(mode):
         _compare(e, im, Batch.from_txns(txns), mode)
 
 
+@pytest.mark.parametrize("mode", BOTH)
+@pytest.mark.parametrize("widths", [[8, 4], [12, 4], [40, 8], [8, 96], [148, 4, 20]],
+                         ids=["stride8", "stride16", "stride64", "stride96", "stride160"])
+def test_result_strides(mode, widths):
+    # the result stride is the widest slot: 8 B (word moves), 16 and 64 B
+    # (one 16-byte chunk per lane), 96 and 160 B (lanes loop over chunks)
+    rng = np.random.default_rng(len(widths) * 1000 + widths[0])
+    schema = {0: (1500, widths)}
+    b = random_batch(rng, n_txns=4000, n_rows=1500, n_cols=len(widths), max_len=10, zipf=0.9)
+    with Engine() as e:
+        im = _setup(e, schema, 0x5EED + widths[0])
+        assert e.result_stride == {8: 8, 12: 16, 40: 64, 96: 96, 148: 160}[max(widths)]
+        _compare(e, im, b, mode)
+
+
 @pytest.mark.parametrize("n_ops", [1, 2047, 2048, 2049, 4096 + 255])
 @pytest.mark.parametrize("mode", BOTH)
 def test_tile_boundaries(n_ops, mode):
