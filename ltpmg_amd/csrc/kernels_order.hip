@@ -215,7 +215,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, con
     constexpr uint32_t kPer = 32, kRound = kBlock * kPer;
     if (zero_word && threadIdx.x == 0) *zero_word = 0u;  // the next stage's counter
     const bool vec = ((reinterpret_cast<uintptr_t>(tflag) | reinterpret_cast<uintptr_t>(tset) |
-                       reinterpret_cast<uintptr_t>(tdelta)) & 15u) == 0;
+                       reinterpret_cast<uintptr_t>(tdelta) | reinterpret_cast<uintptr_t>(cset) |
+                       reinterpret_cast<uintptr_t>(cdelta)) & 15u) == 0;
     Seg carry{0u, kNoSet, 0u};
     for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
         const uint32_t lo = c0 + threadIdx.x * kPer;
@@ -244,13 +245,28 @@ __global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, con
         for (uint32_t k = 0; k < kPer; ++k) agg = seg_op(agg, Seg{f[k], st[k], d[k]});
         Seg tot;
         Seg run = seg_op(carry, block_seg_exclusive(agg, &tot));
+        if (vec && lo + kPer <= n_tiles) {  // 16-byte stores
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            if (lo + k < n_tiles) {
-                cset[lo + k] = run.set;
-                cdelta[lo + k] = run.d;
+            for (uint32_t k = 0; k < kPer; k += 4) {
+                uint32_t s4[4], d4[4];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    s4[j] = run.set;
+                    d4[j] = run.d;
+                    run = seg_op(run, Seg{f[k + j], st[k + j], d[k + j]});
+                }
+                *reinterpret_cast<uint4 *>(cset + lo + k) = make_uint4(s4[0], s4[1], s4[2], s4[3]);
+                *reinterpret_cast<uint4 *>(cdelta + lo + k) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
             }
-            run = seg_op(run, Seg{f[k], st[k], d[k]});
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                if (lo + k < n_tiles) {
+                    cset[lo + k] = run.set;
+                    cdelta[lo + k] = run.d;
+                }
+                run = seg_op(run, Seg{f[k], st[k], d[k]});
+            }
         }
         carry = seg_op(carry, tot);
     }
