@@ -191,7 +191,7 @@ void free_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
     }
     dev_free(e, ob.sort.hist, sort_hist_words(ops) * 4);
     dev_free(e, ob.sort.partials, sort_partials_needed() * 4);
-    dev_free(e, ob.tile_flag, (sg * 5 + 2) * 4);
+    dev_free(e, ob.tile_flag, (sg * 3 + 2) * 4);
     dev_free(e, ob.wlist, sg * 16);
     ob = OrderBufs{};
 }
@@ -206,15 +206,13 @@ int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
     }
     ALLOC(ob.sort.hist, sort_hist_words(ops));
     ALLOC(ob.sort.partials, sort_partials_needed());
-    ALLOC(ob.tile_flag, sg * 5 + 2);
+    ALLOC(ob.tile_flag, sg * 3 + 2);
     ALLOC(ob.wlist, sg);
 #undef ALLOC
     if (rc != LTPMG_OK) return rc;
     ob.tile_set = ob.tile_flag + sg;
     ob.tile_delta = ob.tile_flag + 2 * sg;
-    ob.carry_set = ob.tile_flag + 3 * sg;
-    ob.carry_delta = ob.tile_flag + 4 * sg;
-    ob.wtotal = ob.tile_flag + 5 * sg;
+    ob.wtotal = ob.tile_flag + 3 * sg;
     return LTPMG_OK;
 }
 
@@ -915,7 +913,7 @@ int ltpmg_selftest_segscan(ltpmg_engine *e, const uint32_t *vals, const uint8_t 
     hipStream_t st = e->slots[0].stream;
     uint32_t *d_vals = nullptr, *d_out = nullptr, *d_scr = nullptr;
     uint8_t *d_head = nullptr;
-    const size_t nn = std::max<size_t>(n, 1), scr = 5ull * (seg_tiles(n) + 1);
+    const size_t nn = std::max<size_t>(n, 1), scr = 3ull * (seg_tiles(n) + 1);
     int rc = dev_alloc_n(e, &d_vals, nn);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &d_out, nn);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &d_head, nn);

@@ -248,8 +248,6 @@ struct OrderBufs {
     uint32_t *tile_flag;    // per seg-scan tile: has a segment head
     uint32_t *tile_set;     // per tile aggregate state
     uint32_t *tile_delta;
-    uint32_t *carry_set;    // per tile carry-in
-    uint32_t *carry_delta;
     uint4 *wlist;           // final writes {key, set, d, 0} of segments that span tiles, one per tile at most
     uint32_t *wtotal;       // entries in wlist (cleared by the seg scan), then one spare word
 };

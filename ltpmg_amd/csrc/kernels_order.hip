@@ -13,7 +13,7 @@
 //                 digits (ops are already in TID order, so a stable sort
 //                 yields per-cell TID order)
 //   seg reduce /  segmented scan of the per-cell state monoid
-//   carry / apply   state = (last UPDATE op or none, sum of RMW deltas since),
+//   apply           state = (last UPDATE op or none, sum of RMW deltas since),
 //                 combine(a, b) = b has an UPDATE ? b : (a.set, a.d + b.d);
 //                 a READ's value = (set ? payload(args[set]) : snapshot) + d,
 //                 so no cell is ever walked serially
@@ -85,6 +85,38 @@ __device__ inline Seg block_seg_exclusive(Seg x, Seg *total, Seg *ws) {
 __device__ inline Seg block_seg_exclusive(Seg x, Seg *total) {
     __shared__ Seg ws[kBlock / 64];
     return block_seg_exclusive(x, total, ws);
+}
+
+// Carry-in of scan tile `tile` from the tile aggregates, read newest first.
+// A tile whose aggregate holds a segment head resets the state, so the fold
+// only needs the tiles from the newest such one on; folding older tiles in
+// as well changes nothing, because seg_op(a, b) = b when b has a head.  The
+// calling wave reads 64 aggregates per step: most tiles stop at their
+// neighbour, and a tile inside a hot cell's run reads back to where the run
+// began.  Every lane of one wave calls it; every lane gets the result.
+__device__ inline Seg tile_carry(const uint32_t *tflag, const uint32_t *tset, const uint32_t *tdelta,
+                                 uint32_t tile) {
+    const uint32_t lane = __lane_id();
+    Seg acc{0u, kNoSet, 0u};  // fold of the newer tiles read so far
+    for (uint32_t top = tile; top > 0;) {  // lane l reads tile top - 1 - l
+        const bool in = lane < top;
+        const uint32_t j = in ? top - 1 - lane : 0u;
+        Seg x = in ? Seg{tflag[j], tset[j], tdelta[j]} : Seg{0u, kNoSet, 0u};
+        const bool found = __ballot(in && x.f != 0u) != 0ull;
+        // fold lanes 63 .. 0 (oldest first): lane l ends with its own tile
+        // and the older ones above it
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            Seg y{(uint32_t)__shfl_down((int)x.f, off), (uint32_t)__shfl_down((int)x.set, off),
+                  (uint32_t)__shfl_down((int)x.d, off)};
+            if (lane + off < 64) x = seg_op(y, x);
+        }
+        const Seg step{(uint32_t)__shfl((int)x.f, 0), (uint32_t)__shfl((int)x.set, 0),
+                       (uint32_t)__shfl((int)x.d, 0)};
+        acc = seg_op(step, acc);
+        if (found) break;
+        top = top > 64 ? top - 64 : 0u;
+    }
+    return acc;
 }
 
 // Exclusive sum over the block's threads; ws as for block_seg_exclusive.
@@ -178,7 +210,9 @@ __device__ inline Seg agg8(const OrderSrc &src, uint32_t first, const uint32_t *
 
 // Tile aggregates of the sorted (cell, op) list.
 __global__ __launch_bounds__(kBlock) void k_seg_reduce(OrderSrc src, uint32_t *tflag,
-                                                       uint32_t *tset, uint32_t *tdelta) {
+                                                       uint32_t *tset, uint32_t *tdelta,
+                                                       uint32_t *zero_word) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0u;  // the apply kernel's counter
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
     uint32_t key[kSegItems], val[kSegItems], prev;
     load8(src, first, key, val, prev);
@@ -204,88 +238,28 @@ __global__ __launch_bounds__(kBlock) void k_selftest_reduce(SelftestSrc src, uin
     }
 }
 
-// Carry-in of every scan tile: an exclusive segmented scan of the tile
-// aggregates, in one workgroup.  Each thread takes kPer consecutive tiles
-// per round and loads them straight into registers (every load in flight at
-// once), so up to kBlock * kPer tiles take one round.
-__global__ __launch_bounds__(kBlock) void k_seg_carry(const uint32_t *tflag, const uint32_t *tset,
-                                                      const uint32_t *tdelta, uint32_t n_tiles,
-                                                      uint32_t *cset, uint32_t *cdelta,
-                                                      uint32_t *zero_word) {
-    constexpr uint32_t kPer = 32, kRound = kBlock * kPer;
-    if (zero_word && threadIdx.x == 0) *zero_word = 0u;  // the next stage's counter
-    const bool vec = ((reinterpret_cast<uintptr_t>(tflag) | reinterpret_cast<uintptr_t>(tset) |
-                       reinterpret_cast<uintptr_t>(tdelta) | reinterpret_cast<uintptr_t>(cset) |
-                       reinterpret_cast<uintptr_t>(cdelta)) & 15u) == 0;
-    Seg carry{0u, kNoSet, 0u};
-    for (uint32_t c0 = 0; c0 < n_tiles; c0 += kRound) {
-        const uint32_t lo = c0 + threadIdx.x * kPer;
-        uint32_t f[kPer], st[kPer], d[kPer];
-        if (vec && lo + kPer <= n_tiles) {  // 16-byte loads
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; k += 4) {
-                const uint4 a = *reinterpret_cast<const uint4 *>(tflag + lo + k);
-                const uint4 b = *reinterpret_cast<const uint4 *>(tset + lo + k);
-                const uint4 c = *reinterpret_cast<const uint4 *>(tdelta + lo + k);
-                f[k] = a.x; f[k + 1] = a.y; f[k + 2] = a.z; f[k + 3] = a.w;
-                st[k] = b.x; st[k + 1] = b.y; st[k + 2] = b.z; st[k + 3] = b.w;
-                d[k] = c.x; d[k + 1] = c.y; d[k + 2] = c.z; d[k + 3] = c.w;
-            }
-        } else {
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; ++k) {  // past the end: the identity {0, no write, 0}
-                const bool in = lo + k < n_tiles;
-                f[k] = in ? tflag[lo + k] : 0u;
-                st[k] = in ? tset[lo + k] : kNoSet;
-                d[k] = in ? tdelta[lo + k] : 0u;
-            }
-        }
-        Seg agg{0u, kNoSet, 0u};
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) agg = seg_op(agg, Seg{f[k], st[k], d[k]});
-        Seg tot;
-        Seg run = seg_op(carry, block_seg_exclusive(agg, &tot));
-        if (vec && lo + kPer <= n_tiles) {  // 16-byte stores
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; k += 4) {
-                uint32_t s4[4], d4[4];
-#pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) {
-                    s4[j] = run.set;
-                    d4[j] = run.d;
-                    run = seg_op(run, Seg{f[k + j], st[k + j], d[k + j]});
-                }
-                *reinterpret_cast<uint4 *>(cset + lo + k) = make_uint4(s4[0], s4[1], s4[2], s4[3]);
-                *reinterpret_cast<uint4 *>(cdelta + lo + k) = make_uint4(d4[0], d4[1], d4[2], d4[3]);
-            }
-        } else {
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; ++k) {
-                if (lo + k < n_tiles) {
-                    cset[lo + k] = run.set;
-                    cdelta[lo + k] = run.d;
-                }
-                run = seg_op(run, Seg{f[k], st[k], d[k]});
-            }
-        }
-        carry = seg_op(carry, tot);
-    }
-}
-
 // Per-thread walk with the exclusive prefix of the thread's first element.
+// The tile's carry comes from tile_carry in wave 0, through LDS (the scan's
+// barriers order its store before the read).
 template <class Src>
-__device__ inline St thread_prefix(const Src &src, const uint32_t *cset, const uint32_t *cdelta,
-                                   uint32_t first) {
+__device__ inline St thread_prefix(const Src &src, const uint32_t *tflag, const uint32_t *tset,
+                                   const uint32_t *tdelta, uint32_t first) {
+    __shared__ Seg carry;
+    if (threadIdx.x < 64) {
+        const Seg c = tile_carry(tflag, tset, tdelta, blockIdx.x);
+        if (threadIdx.x == 0) carry = c;
+    }
     Seg tot;
     Seg ex = block_seg_exclusive(thread_aggregate(src, first), &tot);
-    Seg p = seg_op(Seg{0u, cset[blockIdx.x], cdelta[blockIdx.x]}, ex);
+    Seg p = seg_op(Seg{0u, carry.set, carry.d}, ex);
     return {p.set, p.d};
 }
 
-__global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, const uint32_t *cset,
-                                                           const uint32_t *cdelta, uint32_t *out) {
+__global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, const uint32_t *tflag,
+                                                           const uint32_t *tset, const uint32_t *tdelta,
+                                                           uint32_t *out) {
     uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
-    St run = thread_prefix(src, cset, cdelta, first);
+    St run = thread_prefix(src, tflag, tset, tdelta, first);
     for (uint32_t j = 0; j < kSegItems; ++j) {
         uint32_t i = first + j;
         if (i >= src.n) break;
@@ -436,8 +410,9 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
 // final-write list (at most one per tile, one global atomic each) for
 // k_mo_final_writes, which runs after every tile's reads.
 template <int G, bool VEC16, bool ONE>
-__global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *cset,
-                                                           const uint32_t *cdelta, DevTables dt,
+__global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *tflag,
+                                                           const uint32_t *tset,
+                                                           const uint32_t *tdelta, DevTables dt,
                                                            KeyDecode dec, uint8_t *results,
                                                            uint32_t res_stride, uint4 *wlist,
                                                            uint32_t *wtotal) {
@@ -457,6 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     const uint32_t tile = blockIdx.x;
     Seg *seg_ws = reinterpret_cast<Seg *>(list);                   // bytes 0..47
     uint32_t *sum_ws = reinterpret_cast<uint32_t *>(list) + 16;    // bytes 64..79
+    Seg *carry_ws = reinterpret_cast<Seg *>(reinterpret_cast<uint32_t *>(list) + 24);  // bytes 96..107
     const uint32_t first = tile * kSegTile + threadIdx.x * kSegItems;
     uint32_t key[kSegItems], val[kSegItems], prev;
     load8(src, first, key, val, prev);
@@ -464,9 +440,16 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // the key of a segment continuing from the previous tile (all-ones: none)
     const uint32_t t0 = tile * kSegTile;
     const uint32_t cont_key = t0 > 0 && src.keys[t0 - 1] == src.keys[t0] ? src.keys[t0] : 0xFFFFFFFFu;
+    // the tile's carry-in, after the key loads are issued so the two
+    // latencies overlap; the scan's barriers order the LDS store before
+    // the read below
+    if (threadIdx.x < 64) {
+        const Seg c = tile_carry(tflag, tset, tdelta, tile);
+        if (threadIdx.x == 0) *carry_ws = c;
+    }
     Seg tot;
-    const Seg pre = seg_op(Seg{0u, cset[tile], cdelta[tile]},
-                           block_seg_exclusive(agg8(src, first, key, val, prev), &tot, seg_ws));
+    const Seg ex0 = block_seg_exclusive(agg8(src, first, key, val, prev), &tot, seg_ws);
+    const Seg pre = seg_op(Seg{0u, carry_ws->set, carry_ws->d}, ex0);
     // item j of the thread: its state before (ex) and after (run) it, and
     // whether it is a READ (rd) or the end of a written segment (wr)
     auto walk = [&](auto &&visit) {
@@ -1051,9 +1034,7 @@ void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
     hipLaunchKernelGGL(k_seg_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, ob.tile_flag,
-                       ob.tile_set, ob.tile_delta);
-    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, ob.tile_flag, ob.tile_set,
-                       ob.tile_delta, n_tiles, ob.carry_set, ob.carry_delta, ob.wtotal);
+                       ob.tile_set, ob.tile_delta, ob.wtotal);
 }
 
 // Streams the batch's payload seeds (8 bytes per op, in op order) through
@@ -1089,7 +1070,7 @@ void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
     KeyDecode dec{dt.tabs, n_tabs};
-    LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.carry_set, ob.carry_delta, dt, dec,
+    LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta, dt, dec,
                       results, res_stride, ob.wlist, ob.wtotal);
 }
 
@@ -1108,11 +1089,9 @@ void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t 
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     uint32_t *tf = scratch, *ts = scratch + n_tiles, *td = scratch + 2 * n_tiles;
-    uint32_t *cs = scratch + 3 * n_tiles, *cd = scratch + 4 * n_tiles;
     SelftestSrc src{vals, head, n};
     hipLaunchKernelGGL(k_selftest_reduce, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts, td);
-    hipLaunchKernelGGL(k_seg_carry, dim3(1), dim3(kBlock), 0, s, tf, ts, td, n_tiles, cs, cd, nullptr);
-    hipLaunchKernelGGL(k_selftest_apply, dim3(n_tiles), dim3(kBlock), 0, s, src, cs, cd, out);
+    hipLaunchKernelGGL(k_selftest_apply, dim3(n_tiles), dim3(kBlock), 0, s, src, tf, ts, td, out);
 }
 
 uint32_t seg_tiles(uint32_t n) { return (n + kSegTile - 1) / kSegTile; }

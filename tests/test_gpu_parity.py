@@ -132,13 +132,16 @@ def test_segscan_random(n, p):
     assert (out == (expect & 0xFFFFFFFF).astype(np.uint32)).all()
 
 
-@pytest.mark.parametrize("n", [9_000_000, 36_000_000])
-def test_segscan_many_tiles(n):
-    # thousands of 2048-element tiles; at 36M more than 8192 of them, so the
-    # carry kernel takes three rounds
+@pytest.mark.parametrize("n,p", [(9_000_000, 1e-4), (36_000_000, 1e-4), (9_000_000, 2e-7),
+                                 (3_000_000, 0.0)])
+def test_segscan_many_tiles(n, p):
+    # thousands of 2048-element tiles.  A tile's carry folds the tile
+    # aggregates back to the newest one holding a segment head, 64 per step:
+    # at p = 2e-7 a segment spans thousands of tiles, and with no head but
+    # the first the last tile reads back over all 1464 tiles before it
     rng = np.random.default_rng(9)
     vals = rng.integers(0, 1 << 20, n).astype(np.uint32)
-    heads = (rng.random(n) < 1e-4).astype(np.uint8)
+    heads = (rng.random(n) < p).astype(np.uint8)
     for b in (4_194_304, 8192 * 2048, 2 * 8192 * 2048):
         if b < n:
             heads[b] = 0  # a segment across a tile boundary / a carry round boundary
