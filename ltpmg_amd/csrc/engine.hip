@@ -450,8 +450,7 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     launch_mo_apply_reads(st, dt, ov, ob, sorted, n_tabs, results, rs);
     rec.end();
     rec.begin("mo_final_writes");
-    launch_mo_final_writes(st, dt, ov, ob, n_tabs, rs);
-    launch_fill_u8(st, commit, ov.n_txns, 1);
+    launch_mo_final_writes(st, dt, ov, ob, n_tabs, rs, commit);
     rec.end();
     return LTPMG_OK;
 }

@@ -260,7 +260,7 @@ void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob
                            uint32_t n_tabs, uint8_t *results,
                            uint32_t res_stride);
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t n_tabs, uint32_t res_stride);
+                            uint32_t n_tabs, uint32_t res_stride, uint8_t *commit);  // also sets every commit flag
 uint32_t seg_tiles(uint32_t n);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,

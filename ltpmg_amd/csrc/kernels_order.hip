@@ -498,11 +498,24 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
 }
 
 // The final writes of segments that span tiles, after every tile's reads:
-// entries {key, set, d} from k_mo_apply_reads, *wtotal of them.
+// entries {key, set, d} from k_mo_apply_reads, *wtotal of them.  The same
+// grid also sets the batch's commit flags (ordered mode commits every txn),
+// which saves a launch.
 template <int G, bool VEC16, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, const uint32_t *wtotal,
                                                             DevTables dt, KeyDecode dec,
-                                                            const uint64_t *args) {
+                                                            const uint64_t *args, uint8_t *commit,
+                                                            uint32_t n_txns) {
+    {  // every txn of the batch commits: its flag is 1
+        const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+        uint32_t head = 0;
+        if (!(reinterpret_cast<uintptr_t>(commit) & 15)) {
+            for (uint32_t i = tid; i < n_txns / 16; i += nt)
+                reinterpret_cast<uint4 *>(commit)[i] = make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
+            head = n_txns / 16 * 16;
+        }
+        for (uint32_t i = head + tid; i < n_txns; i += nt) commit[i] = 1;
+    }
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
     move_cells<G, VEC16, ONE>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint32_t &op) {
@@ -1075,13 +1088,17 @@ void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob
 }
 
 void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t n_tabs, uint32_t res_stride) {
-    if (ov.n_ops == 0) return;
+                            uint32_t n_tabs, uint32_t res_stride, uint8_t *commit) {
+    if (ov.n_ops == 0) {
+        launch_fill_u8(s, commit, ov.n_txns, 1);
+        return;
+    }
     KeyDecode dec{dt.tabs, n_tabs};
     // at most one entry per scan tile; one lane group per entry
     const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
     const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
-    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.wlist, ob.wtotal, dt, dec, ov.args);
+    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.wlist, ob.wtotal, dt, dec, ov.args, commit,
+                      ov.n_txns);
 }
 
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
