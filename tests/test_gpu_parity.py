@@ -144,7 +144,7 @@ def test_segscan_many_tiles(n, p):
     heads = (rng.random(n) < p).astype(np.uint8)
     for b in (4_194_304, 8192 * 2048, 2 * 8192 * 2048):
         if b < n:
-            heads[b] = 0  # a segment across a tile boundary / a carry round boundary
+            heads[b] = 0  # a segment across a tile boundary / a 64-tile look-back step
     with Engine() as e:
         out = e.selftest_segscan(vals, heads)
     excl = np.cumsum(vals, dtype=np.uint64) - vals
