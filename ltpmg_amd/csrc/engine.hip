@@ -158,6 +158,17 @@ DevTables dev_tables(const ltpmg_engine *e) {
     return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist};
 }
 
+// Column stride of the ordered-mode cell keys (TabDesc::cstride): each
+// table's column count rounded up to a power of two when the padded keys of
+// all live tables fit below 2^32 - 1, else the column count itself.
+bool pad_key_columns(const ltpmg_engine *e) {
+    uint64_t padded = 0;
+    for (const Table &t : e->tables)
+        if (t.live) padded += (uint64_t)t.n_rows << ceil_log2(t.n_cols);
+    return padded <= 0xFFFFFFFFull;
+}
+uint32_t key_col_stride(const Table &t, bool pad) { return pad ? 1u << ceil_log2(t.n_cols) : t.n_cols; }
+
 // Writes the device descriptors and assigns ordered-mode global row bases.
 int sync_descriptors(ltpmg_engine *e) {
     ColDesc cols[LTPMG_MAX_TABLES * LTPMG_MAX_COLS + kColList] = {};
@@ -165,11 +176,14 @@ int sync_descriptors(ltpmg_engine *e) {
     TabDesc tabs[LTPMG_MAX_TABLES] = {};
     uint64_t cell_base = 0;  // past 2^32 - 1 cells only the lowest-TID-wins mode runs
     uint32_t n_clist = 0;
+    const bool pad = pad_key_columns(e);
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
         const Table &tb = e->tables[t];
         if (!tb.live) continue;
-        tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u};
-        cell_base += (uint64_t)tb.n_rows * tb.n_cols;
+        const uint32_t stride = key_col_stride(tb, pad);
+        tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u,
+                          stride};
+        cell_base += (uint64_t)tb.n_rows * stride;
         for (uint32_t c = 0; c < tb.n_cols; ++c) {
             cols[t * LTPMG_MAX_COLS + c] = ColDesc{tb.col[c], tb.slot[c], tb.width[c]};
             if (n_clist < kColList) clist[n_clist++] = cols[t * LTPMG_MAX_COLS + c];
@@ -383,12 +397,14 @@ int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *t
 }
 
 // Key geometry of ordered mode: cells are numbered table by table, row
-// major, so a key needs ceil(log2(cells)) bits; all-ones stays free as the
-// kernels' end-of-list sentinel.
+// major, each row taking TabDesc::cstride numbers, so a key needs
+// ceil(log2(numbers)) bits; all-ones stays free as the kernels' end-of-list
+// sentinel.
 int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
     uint64_t cells = 0;
+    const bool pad = pad_key_columns(e);
     for (const Table &t : e->tables)
-        if (t.live) cells += (uint64_t)t.n_rows * t.n_cols;
+        if (t.live) cells += (uint64_t)t.n_rows * key_col_stride(t, pad);
     if (cells > 0xFFFFFFFFull)
         return fail(e, LTPMG_EINVAL, "ordered mode numbers at most 2^32 - 1 cells; the tables hold %llu",
                     (unsigned long long)cells);

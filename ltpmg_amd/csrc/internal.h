@@ -97,8 +97,11 @@ struct TabDesc {
     uint32_t n_rows;
     uint32_t n_cols;
     uint32_t cell_base;  // ordered-mode key of (row 0, col 0): cells are numbered
-                         // table by table, row-major, key = cell_base + row * n_cols + col
+                         // table by table, row-major, key = cell_base + row * cstride + col
     uint32_t live;
+    uint32_t cstride;    // the keys' column stride: n_cols rounded up to a power of two
+                         // (a key then decodes with a shift and a mask) when every live
+                         // table's padded keys fit below 2^32 - 1, else n_cols
 };
 
 struct OpView {

@@ -8,7 +8,9 @@
 // serial list is the tail.  Here the order is made data-parallel instead:
 //
 //   radix sort    stable LSD sort on key = cell number (table by table,
-//                 row-major), val = op | kind<<30; the first pass builds both
+//                 row-major, a row's columns padded to a power of two when
+//                 the keys still fit 32 bits, so a key decodes with a shift),
+//                 val = op | kind<<30; the first pass builds both
 //                 from the ops (mo_keys only for one-op batches); 7- or 8-bit
 //                 digits (ops are already in TID order, so a stable sort
 //                 yields per-cell TID order)
@@ -282,10 +284,15 @@ struct KeyDecode {
         for (uint32_t t = 0; t < n_tabs; ++t) {
             const TabDesc &td = tabs[t];
             const uint32_t off = key - td.cell_base;
-            if (td.live && key >= td.cell_base && (uint64_t)off < (uint64_t)td.n_rows * td.n_cols) {
+            if (td.live && key >= td.cell_base && (uint64_t)off < (uint64_t)td.n_rows * td.cstride) {
                 tab = t;
-                row = off / td.n_cols;
-                col = off - row * td.n_cols;
+                if ((td.cstride & (td.cstride - 1u)) == 0u) {  // a shift and a mask
+                    row = off >> __builtin_ctz(td.cstride);
+                    col = off & (td.cstride - 1u);
+                } else {
+                    row = off / td.cstride;
+                    col = off - row * td.cstride;
+                }
                 cpos = cbase + col;
                 return true;
             }
@@ -530,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, 
 
 // Sort inputs.  SortedSrc: the key and value arrays of the previous pass.
 // OpSrc: the first pass of the ordered mode reads the ops themselves and
-// builds key = cell_base + row * n_cols + col and val = op | invalid << 29 |
+// builds key = cell_base + row * cstride + col and val = op | invalid << 29 |
 // kind << 30 on the fly, so the keys are never written out unsorted.
 struct SortedSrc {
     const uint32_t *k, *v;
@@ -553,7 +560,8 @@ struct OpSrc {
     const uint16_t *meta;
     // the table descriptors in LDS, so a key costs no dependent global load
     struct Cache {
-        uint32_t base[LTPMG_MAX_TABLES], n_rows[LTPMG_MAX_TABLES], n_cols[LTPMG_MAX_TABLES];
+        uint32_t base[LTPMG_MAX_TABLES], n_rows[LTPMG_MAX_TABLES], n_cols[LTPMG_MAX_TABLES],
+            cstride[LTPMG_MAX_TABLES];
     };
     __device__ void prepare(Cache &c) const {
         if (threadIdx.x < LTPMG_MAX_TABLES) {
@@ -561,6 +569,7 @@ struct OpSrc {
             c.base[threadIdx.x] = td.cell_base;
             c.n_rows[threadIdx.x] = td.live ? td.n_rows : 0u;
             c.n_cols[threadIdx.x] = td.n_cols;
+            c.cstride[threadIdx.x] = td.cstride;
         }
     }
     __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
@@ -568,14 +577,14 @@ struct OpSrc {
         const uint32_t t = op_tab(m), row = rows[o];
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
                         op_col(m) < c.n_cols[t];
-        key = ok ? c.base[t] + row * c.n_cols[t] + op_col(m) : 0u;
+        key = ok ? c.base[t] + row * c.cstride[t] + op_col(m) : 0u;
         val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
     __device__ uint32_t key_of(uint16_t m, uint32_t row, const Cache &c) const {
         const uint32_t t = op_tab(m);
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
                         op_col(m) < c.n_cols[t];
-        return ok ? c.base[t] + row * c.n_cols[t] + op_col(m) : 0u;
+        return ok ? c.base[t] + row * c.cstride[t] + op_col(m) : 0u;
     }
     // keys of ops o .. o + 7 (o a multiple of 8, rows and meta 16-byte aligned)
     __device__ void keys8(uint32_t o, uint32_t *key, const Cache &c) const {
@@ -903,7 +912,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
         uint32_t row = ov.rows[o];
         bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
                   op_col(m) < td.n_cols;
-        keys[o] = ok ? td.cell_base + row * td.n_cols + op_col(m) : 0u;
+        keys[o] = ok ? td.cell_base + row * td.cstride + op_col(m) : 0u;
         vals[o] = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 }
@@ -984,8 +993,8 @@ void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, 
 // that keeps it: half as many digits make each digit's run in the scatter
 // twice as long, so fewer of its lines are written in part (measured 0.05 ms
 // per C2 batch faster than 8 bits).  With LTPMG_RADIX_MAX_BITS=9, 9-bit digits are taken when
-// they save a pass; on MI355X that measured slower for the 27-bit keys of a
-// 10M x 10 table (three 9-bit passes 1.28 ms per batch, four 7-bit passes
+// they save a pass; on MI355X that measured slower for the 27-bit unpadded
+// keys of a 10M x 10 table (three 9-bit passes 1.28 ms per batch, four 7-bit passes
 // 1.22 ms): a 9-bit scatter pass takes 0.10 ms against 0.07 ms, and its
 // digit-offset scan reads four times the counts.
 #ifndef LTPMG_RADIX_MAX_BITS

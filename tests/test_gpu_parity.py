@@ -338,9 +338,11 @@ def _sparse_image(schema: dict, seed: int, batches) -> orc.Image:
     return im
 
 
-def test_ordered_mode_past_2e31_cells():
-    # 34M rows x 64 columns = 2.18e9 cells: keys above 2^31 reach the sort and decode
-    rows, cols = 34_000_000, 64
+@pytest.mark.parametrize("rows,cols", [(34_000_000, 64), (70_000_000, 33)])
+def test_ordered_mode_past_2e31_cells(rows, cols):
+    # 34M rows x 64 columns = 2.18e9 cells: keys above 2^31 reach the sort and
+    # decode.  70M x 33 = 2.31e9 cells, but 33 columns padded to 64 would need
+    # 4.48e9 keys, so the keys keep a stride of 33 and decode with a division
     rng = np.random.default_rng(12)
     n = 60_000
     hi = rng.integers(rows - 2_000_000, rows, n // 2)
