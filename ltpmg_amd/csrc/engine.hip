@@ -87,6 +87,7 @@ struct ltpmg_engine {
     Table tables[LTPMG_MAX_TABLES];
     ColDesc *d_cols = nullptr;  // the sparse descriptor table, then the column list (kColList)
     uint32_t n_clist = 0;
+    KeyDecode kdec{};  // ordered-mode key decode of the live tables
     TabDesc *d_tabs = nullptr;
     unsigned long long *d_digest = nullptr;
     std::vector<Slot> slots;
@@ -147,15 +148,8 @@ uint32_t max_slot_all(const ltpmg_engine *e) {
     return m;
 }
 
-uint32_t n_tabs_span(const ltpmg_engine *e) {
-    uint32_t n = 0;
-    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
-        if (e->tables[t].live) n = t + 1;
-    return n;
-}
-
 DevTables dev_tables(const ltpmg_engine *e) {
-    return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist};
+    return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist, e->kdec};
 }
 
 // Column stride of the ordered-mode cell keys (TabDesc::cstride): each
@@ -177,6 +171,7 @@ int sync_descriptors(ltpmg_engine *e) {
     uint64_t cell_base = 0;  // past 2^32 - 1 cells only the lowest-TID-wins mode runs
     uint32_t n_clist = 0;
     const bool pad = pad_key_columns(e);
+    KeyDecode kd{};
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
         const Table &tb = e->tables[t];
         if (!tb.live) continue;
@@ -184,6 +179,14 @@ int sync_descriptors(ltpmg_engine *e) {
         tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u,
                           stride};
         cell_base += (uint64_t)tb.n_rows * stride;
+        kd.tab[kd.n] = t;
+        kd.end[kd.n] = (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu);
+        kd.stride[kd.n] = stride;
+        kd.shift[kd.n] = (stride & (stride - 1u)) == 0u ? (uint32_t)__builtin_ctz(stride) : 32u;
+        kd.cpos[kd.n] = kd.n_cols_all;
+        kd.n_cols_all += tb.n_cols;
+        kd.end_all = kd.end[kd.n];
+        ++kd.n;
         for (uint32_t c = 0; c < tb.n_cols; ++c) {
             cols[t * LTPMG_MAX_COLS + c] = ColDesc{tb.col[c], tb.slot[c], tb.width[c]};
             if (n_clist < kColList) clist[n_clist++] = cols[t * LTPMG_MAX_COLS + c];
@@ -191,6 +194,7 @@ int sync_descriptors(ltpmg_engine *e) {
     }
     HIP_TRY(e, hipMemcpy(e->d_cols, cols, sizeof cols, hipMemcpyHostToDevice));
     e->n_clist = n_clist;
+    e->kdec = kd;
     HIP_TRY(e, hipMemcpy(e->d_tabs, tabs, sizeof tabs, hipMemcpyHostToDevice));
     return LTPMG_OK;
 }
@@ -422,7 +426,6 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
     DevTables dt = dev_tables(e);
-    const uint32_t n_tabs = n_tabs_span(e);
     int sorted = 0;
     if (ov.n_ops > 1 && key_bits > 0) {
         // digit 0 straight from the ops: the unsorted keys are never stored
@@ -463,10 +466,10 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     launch_mo_warm_seeds(st, ov, ob);
     rec.end();
     rec.begin("mo_apply_reads");
-    launch_mo_apply_reads(st, dt, ov, ob, sorted, n_tabs, results, rs);
+    launch_mo_apply_reads(st, dt, ov, ob, sorted, results, rs);
     rec.end();
     rec.begin("mo_final_writes");
-    launch_mo_final_writes(st, dt, ov, ob, n_tabs, rs, commit);
+    launch_mo_final_writes(st, dt, ov, ob, rs, commit);
     rec.end();
     return LTPMG_OK;
 }

@@ -132,22 +132,30 @@ __device__ inline uint32_t lanemask_lt_popc(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Vector helpers: 16-byte cell chunks.
+// Vector helpers: 16-byte cell chunks.  Cells, result slots and lists are
+// global memory, and these helpers say so (address space 1): a pointer read
+// out of a descriptor is otherwise a generic one, its loads and stores
+// compile to flat instructions, and a flat access counts as LDS traffic as
+// well, so every wait for an LDS read would also wait for every cell load
+// and store in flight.
+#define LTPMG_GLOBAL __attribute__((address_space(1)))
 struct u32x4 {
     uint32_t x, y, z, w;
 };
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 __device__ inline u32x4 ld16(const uint8_t *p) {
-    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    const v4u32 v = *(const LTPMG_GLOBAL v4u32 *)p;
     return {v.x, v.y, v.z, v.w};
 }
 __device__ inline void st16(uint8_t *p, u32x4 v) {
-    *reinterpret_cast<uint4 *>(p) = make_uint4(v.x, v.y, v.z, v.w);
+    *(LTPMG_GLOBAL v4u32 *)p = v4u32{v.x, v.y, v.z, v.w};
 }
-typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+__device__ inline uint32_t ld4(const uint8_t *p) { return *(const LTPMG_GLOBAL uint32_t *)p; }
+__device__ inline void st4(uint8_t *p, uint32_t x) { *(LTPMG_GLOBAL uint32_t *)p = x; }
 // streaming store for write-once outputs (read results)
 __device__ inline void st16_nt(uint8_t *p, u32x4 v) {
     v4u32 x = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(x, reinterpret_cast<v4u32 *>(p));
+    __builtin_nontemporal_store(x, (LTPMG_GLOBAL v4u32 *)p);
 }
 // zero the words of a 16-byte chunk at byte b that lie past width w
 __device__ inline u32x4 clip16(u32x4 v, uint32_t b, uint32_t w) {
@@ -177,11 +185,55 @@ __device__ inline u32x4 add16(u32x4 v, uint32_t d, uint32_t b, uint32_t w) {
 // them: a kernel copies this list into LDS so a cell's column descriptor
 // costs no global load (KeyDecode gives the position in it).
 constexpr uint32_t kColList = 256;
+// Ordered-mode cell key -> (table, row, column), passed by value so every
+// field reaches a kernel in its arguments (scalar loads, no per-cell
+// global load).  The live tables own contiguous key ranges from 0, in table
+// order (TabDesc::cell_base, TabDesc::cstride).
+struct KeyDecode {
+    uint32_t n;                         // live tables
+    uint32_t end_all;                   // one past the last key
+    uint32_t n_cols_all;                // columns of all live tables
+    uint32_t end[LTPMG_MAX_TABLES];     // one past each live table's last key
+    uint32_t tab[LTPMG_MAX_TABLES];     // its table id
+    uint32_t shift[LTPMG_MAX_TABLES];   // log2 of its column stride, 32 if no power of two
+    uint32_t stride[LTPMG_MAX_TABLES];  // its column stride
+    uint32_t cpos[LTPMG_MAX_TABLES];    // position of its first column in the column list
+    // cell key -> (table, row, col) and the column's position in the column
+    // list (DevTables::clist order); false past the last key
+    __device__ bool operator()(uint32_t key, uint32_t &t, uint32_t &row, uint32_t &col,
+                               uint32_t &cp) const {
+        if (key >= end_all) return false;
+        uint32_t b = 0, i_t = tab[0], sh = shift[0], st = stride[0], c0 = cpos[0];
+        // the table: the last one whose range starts at or below the key
+        // (one live table, the usual case, skips the walk); every index is
+        // uniform, so the fields are scalar loads
+        for (uint32_t i = 1; i < n; ++i) {
+            const bool in = key >= end[i - 1];
+            b = in ? end[i - 1] : b;
+            i_t = in ? tab[i] : i_t;
+            sh = in ? shift[i] : sh;
+            st = in ? stride[i] : st;
+            c0 = in ? cpos[i] : c0;
+        }
+        const uint32_t off = key - b;
+        if (sh < 32u) {
+            row = off >> sh;
+            col = off & (st - 1u);
+        } else {
+            row = off / st;
+            col = off - row * st;
+        }
+        t = i_t;
+        cp = c0 + col;
+        return true;
+    }
+};
 struct DevTables {
     const ColDesc *cols;   // [LTPMG_MAX_TABLES * LTPMG_MAX_COLS]
     const TabDesc *tabs;   // [LTPMG_MAX_TABLES]
     const ColDesc *clist;  // [kColList]
     uint32_t n_clist;      // entries of clist in use
+    KeyDecode dec;
 };
 
 void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
@@ -259,11 +311,10 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys,
 // ordered-mode stages, run in this order after launch_mo_keys + launch_radix_sort
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted);
 void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob);
-void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint32_t n_tabs, uint8_t *results,
-                           uint32_t res_stride);
-void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t n_tabs, uint32_t res_stride, uint8_t *commit);  // also sets every commit flag
+void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob, int sorted,
+                           uint8_t *results, uint32_t res_stride);
+void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob,
+                            uint32_t res_stride, uint8_t *commit);  // also sets every commit flag
 uint32_t seg_tiles(uint32_t n);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,

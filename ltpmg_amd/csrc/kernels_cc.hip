@@ -115,8 +115,8 @@ __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &
                     }
                 } else {  // 4- or 8-byte slots (also a narrow column beside wider ones)
                     for (uint32_t b = lane * 4; b < res_stride; b += G * 4) {
-                        uint32_t x = b < cd[r].slot ? *reinterpret_cast<const uint32_t *>(cell[r] + b) : 0u;
-                        *reinterpret_cast<uint32_t *>(dst + b) = x;
+                        uint32_t x = b < cd[r].slot ? ld4(cell[r] + b) : 0u;
+                        st4(dst + b, x);
                     }
                 }
             } else if (VEC16 && cd[r].slot >= 16) {
@@ -127,8 +127,8 @@ __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &
                 }
             } else {
                 for (uint32_t b = lane * 4; b < cd[r].slot; b += G * 4) {
-                    uint32_t *c = reinterpret_cast<uint32_t *>(cell[r] + b);
-                    if (b < cd[r].width) *c = upd[r] ? stream_word(arg[r], b >> 2) : *c + (uint32_t)arg[r];
+                    uint8_t *c = cell[r] + b;
+                    if (b < cd[r].width) st4(c, upd[r] ? stream_word(arg[r], b >> 2) : ld4(c) + (uint32_t)arg[r]);
                 }
             }
         }

@@ -273,48 +273,19 @@ __global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, cons
     }
 }
 
-struct KeyDecode {
-    const TabDesc *tabs;
-    uint32_t n_tabs;
-    // cell key -> (table, row, col) and the column's position in the column
-    // list (DevTables::clist order); returns false when out of range
-    __device__ bool operator()(uint32_t key, uint32_t &tab, uint32_t &row, uint32_t &col,
-                               uint32_t &cpos) const {
-        uint32_t cbase = 0;
-        for (uint32_t t = 0; t < n_tabs; ++t) {
-            const TabDesc &td = tabs[t];
-            const uint32_t off = key - td.cell_base;
-            if (td.live && key >= td.cell_base && (uint64_t)off < (uint64_t)td.n_rows * td.cstride) {
-                tab = t;
-                if ((td.cstride & (td.cstride - 1u)) == 0u) {  // a shift and a mask
-                    row = off >> __builtin_ctz(td.cstride);
-                    col = off & (td.cstride - 1u);
-                } else {
-                    row = off / td.cstride;
-                    col = off - row * td.cstride;
-                }
-                cpos = cbase + col;
-                return true;
-            }
-            cbase += td.live ? td.n_cols : 0u;
-        }
-        return false;
-    }
-};
-
 // Pipelined cell moves.  A group of G lanes takes R entries per round and
 // issues every independent source load (the payload seed of the UPDATE an
 // entry sees, or the snapshot cell) before any store, so each lane keeps R
 // HBM requests in flight instead of one.  fetch(k, key, st, op) gives entry
 // k's cell key, state and destination: op's result slot (res_span bytes at
 // results + op * res_span, zero past the cell) or, for op = kCellOp, the
-// cell itself.  lcols: the column list in LDS (its first dt.n_clist
-// entries), or null.  Per entry a round keeps only what its stores need
+// cell itself.  LCOLS: every live column's descriptor is in lcols, the
+// column list in LDS (else they come from dt.cols).  Per entry a round keeps only what its stores need
 // (the cell address, slot and width packed in one word, the op), so six
 // entries fit the registers of four waves per SIMD.
 constexpr uint32_t kCellOp = 0xFFFFFFFFu;
-template <int G, bool VEC16, bool ONE, class Fetch>
-__device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const KeyDecode &dec,
+template <int G, bool VEC16, bool ONE, bool LCOLS, class Fetch>
+__device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, const KeyDecode &dec,
                                   const uint64_t *args, uint8_t *results, uint32_t res_span,
                                   uint32_t g, uint32_t groups, const ColDesc *lcols) {
     constexpr int R = kMoveDepthO;
@@ -340,8 +311,7 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
                 fetch(k, key, st[r], op[r]);
                 uint32_t tab, row, col, cpos;
                 if (dec(key, tab, row, col, cpos)) {
-                    const ColDesc cd =
-                        lcols && cpos < dt.n_clist ? lcols[cpos] : dt.cols[tab * LTPMG_MAX_COLS + col];
+                    const ColDesc cd = LCOLS ? lcols[cpos] : dt.cols[tab * LTPMG_MAX_COLS + col];
                     cell[r] = cd.base + (uint64_t)row * cd.slot;
                     sw[r] = cd.slot | (cd.width << 16);
                 }
@@ -396,11 +366,10 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
                 for (uint32_t b = lane * 4; b < span; b += G * 4) {
                     uint32_t x = 0;
                     if (b < width) {
-                        x = has_set ? stream_word(arg[r], b >> 2)
-                                    : *reinterpret_cast<const uint32_t *>(cell[r] + b);
+                        x = has_set ? stream_word(arg[r], b >> 2) : ld4(cell[r] + b);
                         x += st[r].d;
                     }
-                    *reinterpret_cast<uint32_t *>(dst + b) = x;
+                    st4(dst + b, x);
                 }
             }
         }
@@ -416,13 +385,12 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, DevTables dt, const K
 // ends here with a write, its entry {key, set, d} goes to the batch's
 // final-write list (at most one per tile, one global atomic each) for
 // k_mo_final_writes, which runs after every tile's reads.
-template <int G, bool VEC16, bool ONE>
+template <int G, bool VEC16, bool ONE, bool LCOLS>
 __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *tflag,
                                                            const uint32_t *tset,
                                                            const uint32_t *tdelta, DevTables dt,
-                                                           KeyDecode dec, uint8_t *results,
-                                                           uint32_t res_stride, uint4 *wlist,
-                                                           uint32_t *wtotal) {
+                                                           uint8_t *results, uint32_t res_stride,
+                                                           uint4 *wlist, uint32_t *wtotal) {
     // READs from the front {op, set, d, key}; in-tile final writes from the
     // back {key, set, d, 0}.  Such a write entry needs a write op of this
     // tile in its segment, so reads + write entries <= tile elements and the
@@ -432,7 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // bytes before any entry is written.
     __shared__ uint4 list[kSegTile];
     __shared__ ColDesc lcols[kColList];  // the column list: no global load per cell for its descriptor
-    for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kBlock) lcols[i] = dt.clist[i];
+    if (LCOLS)
+        for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kBlock) lcols[i] = dt.clist[i];
+    const KeyDecode &dec = dt.dec;
     // tiles in launch order: the hottest cells' tiles are neighbours in the
     // sorted list, and dealing them round-robin spreads them over the XCDs
     // (a contiguous run per XCD measured 1.22 ms against 0.88 ms here)
@@ -489,14 +459,14 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         if (wr && key[j] == cont_key) wlist[atomicAdd(wtotal, 1u)] = make_uint4(key[j], run.set, run.d, 0u);
     });
     __syncthreads();
-    move_cells<G, VEC16, ONE>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
+    move_cells<G, VEC16, ONE, LCOLS>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
         uint4 e = list[k];
         key_out = e.w;
         st = St{e.y, e.z};
         op = e.x;
     }, dt, dec, src.args, results, res_stride, threadIdx.x / G, kBlock / G, lcols);
     __syncthreads();  // every read of this tile's cells is done
-    move_cells<G, VEC16, ONE>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
+    move_cells<G, VEC16, ONE, LCOLS>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
         uint4 e = list[kSegTile - 1 - k];
         key_out = e.x;
         st = St{e.y, e.z};
@@ -508,11 +478,10 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
 // entries {key, set, d} from k_mo_apply_reads, *wtotal of them.  The same
 // grid also sets the batch's commit flags (ordered mode commits every txn),
 // which saves a launch.
-template <int G, bool VEC16, bool ONE>
+template <int G, bool VEC16, bool ONE, bool LCOLS>  // LCOLS: unused, descriptors come from dt.cols
 __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, const uint32_t *wtotal,
-                                                            DevTables dt, KeyDecode dec,
-                                                            const uint64_t *args, uint8_t *commit,
-                                                            uint32_t n_txns) {
+                                                            DevTables dt, const uint64_t *args,
+                                                            uint8_t *commit, uint32_t n_txns) {
     {  // every txn of the batch commits: its flag is 1
         const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
         uint32_t head = 0;
@@ -525,12 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, 
     }
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint32_t groups = gridDim.x * blockDim.x / G;
-    move_cells<G, VEC16, ONE>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint32_t &op) {
+    move_cells<G, VEC16, ONE, false>(*wtotal, [&](uint32_t k, uint32_t &key, St &st, uint32_t &op) {
         uint4 e = wlist[k];
         key = e.x;
         st = St{e.y, e.z};
         op = kCellOp;
-    }, dt, dec, args, nullptr, 0u, g, groups, nullptr);
+    }, dt, dt.dec, args, nullptr, 0u, g, groups, nullptr);
 }
 
 // ---------------- radix sort ----------------
@@ -1029,25 +998,25 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint
     hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, keys, vals);
 }
 
-#define LTPMG_MO_LAUNCH(KERNEL, G_, GRID, ...)                                                 \
-    do {                                                                                       \
-        if (res_stride == G_ * 16)                                                             \
-            hipLaunchKernelGGL((KERNEL<G_, true, true>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
-        else                                                                                   \
-            hipLaunchKernelGGL((KERNEL<G_, true, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+#define LTPMG_MO_LAUNCH(KERNEL, G_, LC, GRID, ...)                                                 \
+    do {                                                                                           \
+        if (res_stride == G_ * 16)                                                                 \
+            hipLaunchKernelGGL((KERNEL<G_, true, true, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+        else                                                                                       \
+            hipLaunchKernelGGL((KERNEL<G_, true, false, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
-#define LTPMG_MO_DISPATCH(KERNEL, GRID, ...)                                                \
-    do {                                                                                    \
-        if (res_stride >= 128)                                                              \
-            LTPMG_MO_LAUNCH(KERNEL, 8, GRID, __VA_ARGS__);                                  \
-        else if (res_stride >= 64)                                                          \
-            LTPMG_MO_LAUNCH(KERNEL, 4, GRID, __VA_ARGS__);                                  \
-        else if (res_stride >= 32)                                                          \
-            LTPMG_MO_LAUNCH(KERNEL, 2, GRID, __VA_ARGS__);                                  \
-        else if (res_stride >= 16)                                                          \
-            LTPMG_MO_LAUNCH(KERNEL, 1, GRID, __VA_ARGS__);                                  \
-        else                                                                                \
-            hipLaunchKernelGGL((KERNEL<1, false, false>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+#define LTPMG_MO_DISPATCH(KERNEL, LC, GRID, ...)                                                \
+    do {                                                                                        \
+        if (res_stride >= 128)                                                                  \
+            LTPMG_MO_LAUNCH(KERNEL, 8, LC, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 64)                                                              \
+            LTPMG_MO_LAUNCH(KERNEL, 4, LC, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 32)                                                              \
+            LTPMG_MO_LAUNCH(KERNEL, 2, LC, GRID, __VA_ARGS__);                                  \
+        else if (res_stride >= 16)                                                              \
+            LTPMG_MO_LAUNCH(KERNEL, 1, LC, GRID, __VA_ARGS__);                                  \
+        else                                                                                    \
+            hipLaunchKernelGGL((KERNEL<1, false, false, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
     } while (0)
 
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
@@ -1085,28 +1054,31 @@ void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob) {
                        reinterpret_cast<const uint4 *>(ov.args), n4, ob.wtotal + 1);
 }
 
-void launch_mo_apply_reads(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint32_t n_tabs, uint8_t *results, uint32_t res_stride) {
+void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob, int sorted,
+                           uint8_t *results, uint32_t res_stride) {
     const uint32_t n = ov.n_ops;
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
-    KeyDecode dec{dt.tabs, n_tabs};
-    LTPMG_MO_DISPATCH(k_mo_apply_reads, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta, dt, dec,
-                      results, res_stride, ob.wlist, ob.wtotal);
+    // the column list in LDS when it holds every live column
+    if (dt.n_clist >= dt.dec.n_cols_all)
+        LTPMG_MO_DISPATCH(k_mo_apply_reads, true, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta,
+                          dt, results, res_stride, ob.wlist, ob.wtotal);
+    else
+        LTPMG_MO_DISPATCH(k_mo_apply_reads, false, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta,
+                          dt, results, res_stride, ob.wlist, ob.wtotal);
 }
 
-void launch_mo_final_writes(hipStream_t s, DevTables dt, OpView ov, OrderBufs &ob,
-                            uint32_t n_tabs, uint32_t res_stride, uint8_t *commit) {
+void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob,
+                            uint32_t res_stride, uint8_t *commit) {
     if (ov.n_ops == 0) {
         launch_fill_u8(s, commit, ov.n_txns, 1);
         return;
     }
-    KeyDecode dec{dt.tabs, n_tabs};
     // at most one entry per scan tile; one lane group per entry
     const uint32_t n_tiles = (ov.n_ops + kSegTile - 1) / kSegTile;
     const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
-    LTPMG_MO_DISPATCH(k_mo_final_writes, dim3(blocks), ob.wlist, ob.wtotal, dt, dec, ov.args, commit,
+    LTPMG_MO_DISPATCH(k_mo_final_writes, false, dim3(blocks), ob.wlist, ob.wtotal, dt, ov.args, commit,
                       ov.n_txns);
 }
 
