@@ -34,14 +34,15 @@ constexpr uint32_t kSortTile = kBlock * kSortItems;
 // Cell moves in flight per lane group (independent loads issued before the
 // stores of a round); build-time knobs for tuning sweeps.  Lowest-TID-wins
 // moves (kMoveDepth) and ordered-mode moves (kMoveDepthO) are separate: the
-// ordered apply kernel measured fastest at 6 (0.815 ms per C2 batch, against
-// 0.88 at 4, 0.84 at 5 and 0.87 at 7, where its registers drop it to three
-// waves per SIMD).
+// ordered apply kernel measured fastest at 7 (0.561 ms per C2 batch, against
+// 0.578 at 5, 0.570 at 6 and 0.609 at 8, where its registers drop it to
+// three waves per SIMD); the lowest-TID-wins kernels at 4 (0.117 ms for
+// cc_validate_read against 0.136 at 6).
 #ifndef LTPMG_MOVE_DEPTH
 #define LTPMG_MOVE_DEPTH 4
 #endif
 #ifndef LTPMG_MOVE_DEPTH_O
-#define LTPMG_MOVE_DEPTH_O 6
+#define LTPMG_MOVE_DEPTH_O 7
 #endif
 constexpr int kMoveDepth = LTPMG_MOVE_DEPTH;
 constexpr int kMoveDepthO = LTPMG_MOVE_DEPTH_O;

@@ -152,9 +152,10 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
     auto offer = [&](uint32_t tab, uint32_t row, uint32_t tid) {
         const TabDesc &td = dt.tabs[tab];
         const unsigned long long tag = ep | (uint32_t)~tid;
-        unsigned long long *p = td.lock + row;
+        // global-address-space pointer: global (not flat) load and atomic
+        LTPMG_GLOBAL unsigned long long *p = (LTPMG_GLOBAL unsigned long long *)(td.lock + row);
         const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur < tag) atomicMax(p, tag);
+        if (cur < tag) __hip_atomic_fetch_max(p, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     for (uint32_t i = threadIdx.x; i < n_slots; i += blockDim.x) {
         keys[i] = kEmpty;
@@ -201,7 +202,7 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
             continue;
         }
         const TabDesc &td = dt.tabs[op_tab(m)];
-        unsigned long long lk = td.lock[row];
+        const unsigned long long lk = *(const LTPMG_GLOBAL unsigned long long *)(td.lock + row);
         uint32_t tid = t.t0 + owner[i];
         if ((uint32_t)(lk >> 32) == epoch && (uint32_t)~(uint32_t)lk < tid) ok[owner[i]] = 0;
     }
