@@ -516,6 +516,14 @@ struct SortedSrc {
         key = k[i];
         val = v[i];
     }
+    // kv in two steps: raw() only loads, cook() only computes, so a caller
+    // can have every load of a tile in flight before the first result is
+    // needed
+    __device__ void raw(uint32_t i, uint32_t &a, uint32_t &b) const {
+        a = k[i];
+        b = v[i];
+    }
+    __device__ void cook(uint32_t, uint32_t &key, uint32_t &val, const Cache &) const {}
     // keys i .. i + 7 (i a multiple of 8, the array 16-byte aligned)
     __device__ void keys8(uint32_t i, uint32_t *key, const Cache &) const {
         const uint4 a = reinterpret_cast<const uint4 *>(k + i)[0], b = reinterpret_cast<const uint4 *>(k + i)[1];
@@ -542,8 +550,18 @@ struct OpSrc {
         }
     }
     __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
-        const uint16_t m = meta[o];
-        const uint32_t t = op_tab(m), row = rows[o];
+        raw(o, key, val);
+        cook(o, key, val, c);
+    }
+    // raw: key = the op's meta word, val = its row; cook turns them into the
+    // cell key and the op's sort value
+    __device__ void raw(uint32_t o, uint32_t &a, uint32_t &b) const {
+        a = meta[o];
+        b = rows[o];
+    }
+    __device__ void cook(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
+        const uint16_t m = (uint16_t)key;
+        const uint32_t t = op_tab(m), row = val;
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
                         op_col(m) < c.n_cols[t];
         key = ok ? c.base[t] + row * c.cstride[t] + op_col(m) : 0u;
@@ -648,12 +666,28 @@ __device__ inline void rank_tile(const Src &src, const typename Src::Cache &cach
     constexpr uint32_t D = 1u << BITS;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     const uint32_t wbase = tile * kSortTile + w * (kSortTile / (kBlock / 64));
+    // every chunk's loads first: the ranking below has a wave barrier per
+    // chunk, which the compiler does not move loads across, so loading
+    // inside it would cost one memory round trip per chunk
+    // (lanes past n load element n - 1, which exists, so no load is
+    // conditional, and then read as zero)
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        const uint32_t idx = wbase + c * 64 + lane;
+        src.raw(idx < n ? idx : n - 1u, k[c], v[c]);
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        const uint32_t idx = wbase + c * 64 + lane;
+        src.cook(idx, k[c], v[c], cache);
+        k[c] = idx < n ? k[c] : 0u;
+        v[c] = idx < n ? v[c] : 0u;
+    }
 #pragma unroll
     for (uint32_t c = 0; c < kSortItems; ++c) {
         uint32_t idx = wbase + c * 64 + lane;
         bool valid = idx < n;
-        uint32_t key = 0u, val = 0u;
-        if (valid) src.kv(idx, key, val, cache);
+        const uint32_t key = k[c];
         uint32_t d = (key >> shift) & (D - 1u);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -669,8 +703,6 @@ __device__ inline void rank_tile(const Src &src, const typename Src::Cache &cach
         }
         prior = __shfl(prior, (int)leader);
         r[c] = prior + lanemask_lt_popc(peers);
-        k[c] = key;
-        v[c] = val;
         __builtin_amdgcn_wave_barrier();
     }
 }
