@@ -536,17 +536,15 @@ struct OpSrc {
     const uint32_t *rows;
     const uint16_t *meta;
     // the table descriptors in LDS, so a key costs no dependent global load
+    // one 16-byte word per table {cell_base, n_rows (0 if not live), n_cols,
+    // cstride}: a key costs one LDS read
     struct Cache {
-        uint32_t base[LTPMG_MAX_TABLES], n_rows[LTPMG_MAX_TABLES], n_cols[LTPMG_MAX_TABLES],
-            cstride[LTPMG_MAX_TABLES];
+        uint4 t[LTPMG_MAX_TABLES];
     };
     __device__ void prepare(Cache &c) const {
         if (threadIdx.x < LTPMG_MAX_TABLES) {
             const TabDesc &td = tabs[threadIdx.x];
-            c.base[threadIdx.x] = td.cell_base;
-            c.n_rows[threadIdx.x] = td.live ? td.n_rows : 0u;
-            c.n_cols[threadIdx.x] = td.n_cols;
-            c.cstride[threadIdx.x] = td.cstride;
+            c.t[threadIdx.x] = make_uint4(td.cell_base, td.live ? td.n_rows : 0u, td.n_cols, td.cstride);
         }
     }
     __device__ void kv(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
@@ -561,17 +559,16 @@ struct OpSrc {
     }
     __device__ void cook(uint32_t o, uint32_t &key, uint32_t &val, const Cache &c) const {
         const uint16_t m = (uint16_t)key;
-        const uint32_t t = op_tab(m), row = val;
-        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
-                        op_col(m) < c.n_cols[t];
-        key = ok ? c.base[t] + row * c.cstride[t] + op_col(m) : 0u;
+        const uint32_t row = val;
+        const uint4 td = c.t[op_tab(m)];
+        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.y && op_col(m) < td.z;
+        key = ok ? td.x + row * td.w + op_col(m) : 0u;
         val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
     __device__ uint32_t key_of(uint16_t m, uint32_t row, const Cache &c) const {
-        const uint32_t t = op_tab(m);
-        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < c.n_rows[t] &&
-                        op_col(m) < c.n_cols[t];
-        return ok ? c.base[t] + row * c.cstride[t] + op_col(m) : 0u;
+        const uint4 td = c.t[op_tab(m)];
+        const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.y && op_col(m) < td.z;
+        return ok ? td.x + row * td.w + op_col(m) : 0u;
     }
     // keys of ops o .. o + 7 (o a multiple of 8, rows and meta 16-byte aligned)
     __device__ void keys8(uint32_t o, uint32_t *key, const Cache &c) const {
