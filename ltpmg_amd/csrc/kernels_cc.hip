@@ -136,6 +136,10 @@ __device__ inline void move_ops(uint32_t n, const uint32_t *list, const OpView &
 }
 
 
+// The lock word an invalid op reads in the validation pass (never written,
+// so it holds no tag of any epoch).
+__device__ unsigned long long g_no_lock = 0ull;
+
 // Pass 1 on one tile: every valid write op offers its TID for its row.  The
 // tile's ops on one row first meet in an LDS hash (key = table << 32 | row,
 // value = the lowest TID), so under Zipf a hot row costs the tile one
@@ -189,22 +193,53 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
                                               uint32_t *list, uint8_t *commit, uint8_t *results,
                                               uint32_t res_stride) {
     __shared__ uint32_t n_list, n_commit;
+    // {rows (0 for a dead table), columns, lock words} per table
+    __shared__ uint4 tc[LTPMG_MAX_TABLES];
     __syncthreads();  // the previous pass's readers of ok[] and list[] are done
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) ok[j] = 1;
     if (threadIdx.x == 0) n_list = n_commit = 0;
+    if (threadIdx.x < LTPMG_MAX_TABLES) {
+        const TabDesc &td = dt.tabs[threadIdx.x];
+        const uint64_t lp = reinterpret_cast<uint64_t>(td.lock);
+        tc[threadIdx.x] = make_uint4(td.live ? td.n_rows : 0u, td.n_cols, (uint32_t)lp, (uint32_t)(lp >> 32));
+    }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < t.nops; i += blockDim.x) {
-        uint32_t o = t.op0 + i;
-        uint16_t m = ov.meta[o];
-        uint32_t row = ov.rows[o];
-        if (!op_valid(dt.tabs, m, row)) {  // an invalid op aborts its txn
-            ok[owner[i]] = 0;
-            continue;
+    // kBatch ops per thread at a time, every load of a batch issued before
+    // any result is used: first the ops, then their lock words (an invalid
+    // op reads a dummy word instead, so no load is conditional)
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t base = 0; base < t.nops; base += kBatch * blockDim.x) {
+        uint32_t mm[kBatch], rr[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * blockDim.x + threadIdx.x;
+            const uint32_t o = t.op0 + (i < t.nops ? i : t.nops - 1u);
+            mm[j] = ov.meta[o];
+            rr[j] = ov.rows[o];
         }
-        const TabDesc &td = dt.tabs[op_tab(m)];
-        const unsigned long long lk = *(const LTPMG_GLOBAL unsigned long long *)(td.lock + row);
-        uint32_t tid = t.t0 + owner[i];
-        if ((uint32_t)(lk >> 32) == epoch && (uint32_t)~(uint32_t)lk < tid) ok[owner[i]] = 0;
+        unsigned long long lk[kBatch];
+        bool valid[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint16_t m = (uint16_t)mm[j];
+            const uint4 c = tc[op_tab(m)];
+            valid[j] = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && rr[j] < c.x && op_col(m) < c.y;
+            const unsigned long long *p =
+                valid[j] ? reinterpret_cast<const unsigned long long *>(((uint64_t)c.w << 32) | c.z) + rr[j]
+                         : &g_no_lock;
+            lk[j] = *(const LTPMG_GLOBAL unsigned long long *)p;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * blockDim.x + threadIdx.x;
+            if (i >= t.nops) continue;
+            if (!valid[j]) {  // an invalid op aborts its txn
+                ok[owner[i]] = 0;
+                continue;
+            }
+            const uint32_t tid = t.t0 + owner[i];
+            if ((uint32_t)(lk[j] >> 32) == epoch && (uint32_t)~(uint32_t)lk[j] < tid) ok[owner[i]] = 0;
+        }
     }
     __syncthreads();
     uint32_t mine = 0;
