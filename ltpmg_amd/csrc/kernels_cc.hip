@@ -153,11 +153,19 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
                                     uint32_t *tids, uint32_t n_slots) {
     constexpr unsigned long long kEmpty = ~0ull;
     const unsigned long long ep = (unsigned long long)epoch << 32;
+    // {rows (0 for a dead table), columns, lock words} per table, in LDS
+    __shared__ uint4 tc[LTPMG_MAX_TABLES];
+    if (threadIdx.x < LTPMG_MAX_TABLES) {
+        const TabDesc &td = dt.tabs[threadIdx.x];
+        const uint64_t lp = reinterpret_cast<uint64_t>(td.lock);
+        tc[threadIdx.x] = make_uint4(td.live ? td.n_rows : 0u, td.n_cols, (uint32_t)lp, (uint32_t)(lp >> 32));
+    }
     auto offer = [&](uint32_t tab, uint32_t row, uint32_t tid) {
-        const TabDesc &td = dt.tabs[tab];
+        const uint4 c = tc[tab];
         const unsigned long long tag = ep | (uint32_t)~tid;
         // global-address-space pointer: global (not flat) load and atomic
-        LTPMG_GLOBAL unsigned long long *p = (LTPMG_GLOBAL unsigned long long *)(td.lock + row);
+        LTPMG_GLOBAL unsigned long long *p =
+            (LTPMG_GLOBAL unsigned long long *)(((uint64_t)c.w << 32) | c.z) + row;
         const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur < tag) __hip_atomic_fetch_max(p, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
@@ -166,17 +174,34 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
         tids[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < t.nops; i += blockDim.x) {
-        uint32_t o = t.op0 + i;
-        uint16_t m = ov.meta[o];
-        if (op_kind(m) == LTPMG_OP_READ) continue;
-        uint32_t row = ov.rows[o];
-        if (!op_valid(dt.tabs, m, row)) continue;  // never touch memory outside the tables
-        const unsigned long long key = ((unsigned long long)op_tab(m) << 32) | row;
-        const uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n_slots - 1);
-        const unsigned long long prev = atomicCAS(&keys[slot], kEmpty, key);
-        if (prev == kEmpty || prev == key) atomicMin(&tids[slot], t.t0 + owner[i]);
-        else offer(op_tab(m), row, t.t0 + owner[i]);
+    // the ops' own loads go out eight per thread at a time; the lock words
+    // are still read one row at a time below (reading them together measured
+    // slower: a later read sees more lower-TID tags and skips its atomic)
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t base = 0; base < t.nops; base += kBatch * blockDim.x) {
+        uint32_t mm[kBatch], rr[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * blockDim.x + threadIdx.x;
+            const uint32_t o = t.op0 + (i < t.nops ? i : t.nops - 1u);
+            mm[j] = ov.meta[o];
+            rr[j] = ov.rows[o];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * blockDim.x + threadIdx.x;
+            const uint16_t m = (uint16_t)mm[j];
+            const uint32_t row = rr[j];
+            if (i >= t.nops || op_kind(m) == LTPMG_OP_READ) continue;
+            const uint4 c = tc[op_tab(m)];
+            // never touch memory outside the tables
+            if ((m >> 12) != 0 || op_kind(m) > LTPMG_OP_RMW_ADD || row >= c.x || op_col(m) >= c.y) continue;
+            const unsigned long long key = ((unsigned long long)op_tab(m) << 32) | row;
+            const uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n_slots - 1);
+            const unsigned long long prev = atomicCAS(&keys[slot], kEmpty, key);
+            if (prev == kEmpty || prev == key) atomicMin(&tids[slot], t.t0 + owner[i]);
+            else offer(op_tab(m), row, t.t0 + owner[i]);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n_slots; i += blockDim.x) {
