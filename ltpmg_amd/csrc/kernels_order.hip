@@ -350,8 +350,10 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, 
                     }
                     x = add16(x, st[r].d, b0, width);
                 }
-                if (to_result) st16_nt(dst + b0, x);
-                else if (b0 < span) st16(dst + b0, x);
+                // one streaming store for result slots and cells alike:
+                // two store forms cost two instructions per move (0.566
+                // against 0.530 ms for the apply kernel)
+                if (to_result || b0 < span) st16_nt(dst + b0, x);
             } else if (VEC16 && slot >= 16) {
                 for (uint32_t b = b0; b < span; b += G * 16) {
                     u32x4 x{0u, 0u, 0u, 0u};
@@ -359,8 +361,7 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, 
                         x = has_set ? payload16(arg[r], b, width) : (b == b0 ? v[r] : ld16(cell[r] + b));
                         x = add16(x, st[r].d, b, width);
                     }
-                    if (to_result) st16_nt(dst + b, x);
-                    else st16(dst + b, x);
+                    st16_nt(dst + b, x);
                 }
             } else {  // 4- and 8-byte slots: one word per lane step
                 for (uint32_t b = lane * 4; b < span; b += G * 4) {
