@@ -771,11 +771,26 @@ __device__ inline void stage_and_write(ScatterLds<BITS> &L, uint32_t tile, uint3
     __syncthreads();
     const uint32_t tile0 = tile * kSortTile;
     const uint32_t tile_n = n - tile0 < kSortTile ? n - tile0 : kSortTile;
-    for (uint32_t j = threadIdx.x; j < tile_n; j += kBlock) {
-        uint32_t key = L.sk[j], d = (key >> shift) & (D - 1u);
-        uint32_t pos = L.gbase[d] + (j - L.tile_off[d]);
-        kout[pos] = key;
-        vout[pos] = L.sv[j];
+    // every LDS read first, into registers of its own, then every store:
+    // reading the next key into the registers a store still reads from
+    // would make each store wait for the one before it
+    uint32_t kk[kSortItems], vv[kSortItems], pp[kSortItems];
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        const uint32_t j = threadIdx.x + c * kBlock;
+        if (j < tile_n) {
+            kk[c] = L.sk[j];
+            const uint32_t d = (kk[c] >> shift) & (D - 1u);
+            pp[c] = L.gbase[d] + (j - L.tile_off[d]);
+            vv[c] = L.sv[j];
+        }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < kSortItems; ++c) {
+        if (threadIdx.x + c * kBlock < tile_n) {
+            kout[pp[c]] = kk[c];
+            vout[pp[c]] = vv[c];
+        }
     }
 }
 
