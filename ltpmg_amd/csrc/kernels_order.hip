@@ -280,9 +280,10 @@ __global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, cons
 // k's cell key, state and destination: op's result slot (res_span bytes at
 // results + op * res_span, zero past the cell) or, for op = kCellOp, the
 // cell itself.  LCOLS: every live column's descriptor is in lcols, the
-// column list in LDS (else they come from dt.cols).  Per entry a round keeps only what its stores need
-// (the cell address, slot and width packed in one word, the op), so seven
-// entries fit the registers of four waves per SIMD.
+// column list in LDS (else they come from dt.cols).  Per entry a round
+// keeps only what its stores need (the cell address, slot and width packed
+// in one word, the op), so seven entries fit the registers of four waves
+// per SIMD.
 constexpr uint32_t kCellOp = 0xFFFFFFFFu;
 template <int G, bool VEC16, bool ONE, bool LCOLS, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, const KeyDecode &dec,
