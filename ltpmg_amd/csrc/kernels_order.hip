@@ -873,6 +873,55 @@ __global__ __launch_bounds__(kBlock) void k_rs_offsets(uint32_t *hist, uint32_t 
     const bool own = DS::owns();
     const uint32_t ct = chunk_tiles(n_tiles), t0 = blockIdx.x * ct;
     const uint32_t t1 = t0 + ct < n_tiles ? t0 + ct : n_tiles;
+    if constexpr (2 * D <= kBlock) {
+        if (t1 - t0 <= 2 * kOffsetRows) {
+            // Two threads per digit, each with half of the chunk's rows and
+            // half of the earlier chunks' sums; the row loads go out first,
+            // so both sets of loads are in flight together.
+            __shared__ uint32_t sb[2][D], sa[2][D], sr[2][D], sbase[D];
+            const uint32_t d = threadIdx.x % D, h = threadIdx.x / D;
+            const bool act = h < 2;
+            const uint32_t nrows = t1 - t0, half = (nrows + 1) / 2;
+            const uint32_t r0 = t0 + (h ? half : 0u);
+            const uint32_t nr = act ? (h ? nrows - half : half) : 0u;
+            uint32_t cnt[kOffsetRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) cnt[r] = r < nr ? hist[(uint64_t)(r0 + r) * D + d] : 0u;
+            uint32_t b = 0, a = 0;
+            for (uint32_t c = h * kOffsetRows; act && c < n_chunks; c += 2 * kOffsetRows) {
+                uint32_t x[kOffsetRows];
+#pragma unroll
+                for (uint32_t r = 0; r < kOffsetRows; ++r) x[r] = c + r < n_chunks ? chunks[(c + r) * D + d] : 0u;
+#pragma unroll
+                for (uint32_t r = 0; r < kOffsetRows; ++r) (c + r < blockIdx.x ? b : a) += x[r];
+            }
+            uint32_t rsum = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) rsum += cnt[r];
+            if (act) {
+                sb[h][d] = b;
+                sa[h][d] = a;
+                sr[h][d] = rsum;
+            }
+            __syncthreads();
+            const uint32_t before = act ? sb[0][d] + sb[1][d] : 0u;
+            const uint32_t total = act ? before + sa[0][d] + sa[1][d] : 0u;
+            uint32_t all;
+            // threads 0 .. D - 1 hold the digits in order; the rest add 0
+            const uint32_t base = block_exclusive_sum(h == 0 ? total : 0u, &all);
+            if (h == 0) sbase[d] = base + before;
+            __syncthreads();
+            if (act) {
+                uint32_t run = sbase[d] + (h ? sr[0][d] : 0u);
+#pragma unroll
+                for (uint32_t r = 0; r < kOffsetRows; ++r) {
+                    if (r < nr) hist[(uint64_t)(r0 + r) * D + d] = run;
+                    run += cnt[r];
+                }
+            }
+            return;
+        }
+    }
     // thread owns digits threadIdx.x * PER + p, so thread order is digit order
     uint32_t before[PER], total[PER], mine = 0;
 #pragma unroll
