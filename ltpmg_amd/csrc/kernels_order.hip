@@ -852,6 +852,24 @@ __global__ __launch_bounds__(kBlock) void k_rs_chunk_sum(const uint32_t *hist, u
     constexpr uint32_t D = 1u << BITS;
     const uint32_t ct = chunk_tiles(n_tiles), t0 = blockIdx.x * ct;
     const uint32_t t1 = t0 + ct < n_tiles ? t0 + ct : n_tiles;
+    if constexpr (2 * D <= kBlock) {
+        if (t1 - t0 <= 2 * kOffsetRows) {  // two threads per digit, half of the rows each
+            __shared__ uint32_t part[D];
+            const uint32_t d = threadIdx.x % D, h = threadIdx.x / D;
+            const uint32_t half = (t1 - t0 + 1) / 2, r0 = t0 + (h ? half : 0u);
+            const uint32_t nr = h == 0 ? half : h == 1 ? t1 - t0 - half : 0u;
+            uint32_t x[kOffsetRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) x[r] = r < nr ? hist[(uint64_t)(r0 + r) * D + d] : 0u;
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kOffsetRows; ++r) sum += x[r];
+            if (h == 1) part[d] = sum;
+            __syncthreads();
+            if (h == 0) chunks[blockIdx.x * D + d] = sum + part[d];
+            return;
+        }
+    }
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
         uint32_t sum = 0;
         for (uint32_t t = t0; t < t1; t += kOffsetRows) {  // kOffsetRows loads in flight
