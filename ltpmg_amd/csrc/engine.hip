@@ -293,9 +293,11 @@ int ensure_slot_capacity(ltpmg_engine *e, Slot &s, uint32_t txns, uint32_t ops, 
     return LTPMG_OK;
 }
 
-// Checks a batch as submitted; max_ops is 2^29 for one batch (ordered-mode
-// op indices have 29 bits) and larger for a stream of batches.
-int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ull << 29) {
+// The CSR structure of a batch: txn_off starts at 0, every txn holds
+// 1..LTPMG_MAX_TXN_OPS ops, and the last offset is n_ops.  The kernels derive
+// every op range from txn_off, so this check runs on every submit, trusted
+// or not.
+int validate_structure(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops) {
     if (!b || !b->txn_off || (b->n_ops && (!b->rows || !b->meta || !b->args)))
         return fail(e, LTPMG_EINVAL, "null batch array");
     if (b->n_ops >= max_ops) return fail(e, LTPMG_EINVAL, "n_ops must be < %llu", (unsigned long long)max_ops);
@@ -308,24 +310,17 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ul
     }
     if (b->txn_off[b->n_txns] != b->n_ops)
         return fail(e, LTPMG_EINVAL, "txn_off[n_txns]=%u != n_ops=%u", b->txn_off[b->n_txns], b->n_ops);
-    // per-op checks, split over host threads for large batches; the first
-    // bad op (lowest index) is reported
-    auto bad_op = [&](uint32_t i) -> bool {
-        uint16_t m = b->meta[i];
-        uint32_t kind = m & 3u, col = (m >> 2) & 63u;
-        const Table &t = e->tables[(m >> 8) & 15u];
-        return (m >> 12) || kind > LTPMG_OP_RMW_ADD || !t.live || col >= t.n_cols || b->rows[i] >= t.n_rows;
-    };
-    const uint32_t n = b->n_ops;
+    return LTPMG_OK;
+}
+
+// Runs fn(lo, hi) over [0, n) split across host threads (one thread below
+// 2^20 items); returns the lowest index any range reported, or UINT32_MAX.
+template <class Fn>
+uint32_t first_bad_parallel(uint32_t n, Fn fn) {
     const uint32_t n_threads = n < (1u << 20) ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     std::vector<uint32_t> first(n_threads, UINT32_MAX);
     auto scan = [&](uint32_t w) {
-        uint32_t a = (uint32_t)((uint64_t)n * w / n_threads), z = (uint32_t)((uint64_t)n * (w + 1) / n_threads);
-        for (uint32_t i = a; i < z; ++i)
-            if (bad_op(i)) {
-                first[w] = i;
-                return;
-            }
+        first[w] = fn((uint32_t)((uint64_t)n * w / n_threads), (uint32_t)((uint64_t)n * (w + 1) / n_threads));
     };
     if (n_threads == 1) {
         scan(0);
@@ -334,7 +329,27 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ul
         for (uint32_t w = 0; w < n_threads; ++w) th.emplace_back(scan, w);
         for (auto &t : th) t.join();
     }
-    uint32_t i = *std::min_element(first.begin(), first.end());
+    return *std::min_element(first.begin(), first.end());
+}
+
+// Checks a batch as submitted; max_ops is 2^29 for one batch (ordered-mode
+// op indices have 29 bits) and larger for a stream of batches.
+int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ull << 29) {
+    int rc = validate_structure(e, b, max_ops);
+    if (rc != LTPMG_OK) return rc;
+    // per-op checks, split over host threads for large batches; the first
+    // bad op (lowest index) is reported
+    auto bad_op = [&](uint32_t i) -> bool {
+        uint16_t m = b->meta[i];
+        uint32_t kind = m & 3u, col = (m >> 2) & 63u;
+        const Table &t = e->tables[(m >> 8) & 15u];
+        return (m >> 12) || kind > LTPMG_OP_RMW_ADD || !t.live || col >= t.n_cols || b->rows[i] >= t.n_rows;
+    };
+    uint32_t i = first_bad_parallel(b->n_ops, [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t k = lo; k < hi; ++k)
+            if (bad_op(k)) return k;
+        return UINT32_MAX;
+    });
     if (i != UINT32_MAX) {
         uint16_t m = b->meta[i];
         uint32_t kind = m & 3u, col = (m >> 2) & 63u, tab = (m >> 8) & 15u;
@@ -343,6 +358,31 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ul
         if (!t.live) return fail(e, LTPMG_EINVAL, "op %u: table %u does not exist", i, tab);
         if (col >= t.n_cols) return fail(e, LTPMG_EINVAL, "op %u: column %u >= %u", i, col, t.n_cols);
         return fail(e, LTPMG_EINVAL, "op %u: row %u >= %u rows of table %u", i, b->rows[i], t.n_rows, tab);
+    }
+    // Within a txn, a cell that an op writes is not accessed by any later op
+    // of the txn (a READ before the write is fine).  The kernels rely on it:
+    // lowest-TID-wins reads see the batch-start snapshot and applies a txn's
+    // writes to one cell in parallel.  Only pairs whose first op writes are
+    // compared, so a 16-op txn with 8 writes costs at most 8 x 15 compares.
+    const uint32_t *off = b->txn_off;
+    uint32_t bad_t = first_bad_parallel(b->n_txns, [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t t = lo; t < hi; ++t)
+            for (uint32_t p = off[t]; p + 1 < off[t + 1]; ++p) {
+                if ((b->meta[p] & 3u) == LTPMG_OP_READ) continue;
+                const uint16_t cell = b->meta[p] & (uint16_t)~3u;
+                for (uint32_t q = p + 1; q < off[t + 1]; ++q)
+                    if (b->rows[q] == b->rows[p] && (b->meta[q] & (uint16_t)~3u) == cell) return t;
+            }
+        return UINT32_MAX;
+    });
+    if (bad_t != UINT32_MAX) {
+        for (uint32_t p = off[bad_t]; p + 1 < off[bad_t + 1]; ++p)
+            for (uint32_t q = p + 1; (b->meta[p] & 3u) != LTPMG_OP_READ && q < off[bad_t + 1]; ++q)
+                if (b->rows[q] == b->rows[p] && (b->meta[q] & (uint16_t)~3u) == (b->meta[p] & (uint16_t)~3u))
+                    return fail(e, LTPMG_EINVAL,
+                                "txn %u: op %u accesses the cell (table %u, row %u, column %u) that op %u of "
+                                "the same txn writes",
+                                bad_t, q, (b->meta[p] >> 8) & 15u, b->rows[p], (b->meta[p] >> 2) & 63u, p);
     }
     return LTPMG_OK;
 }
@@ -707,9 +747,11 @@ static int stage_locked(ltpmg_engine *e, Slot &s, uint32_t n_txns, uint32_t n_op
 static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
     const uint32_t n_txns = s.staged_txns, n_ops = s.staged_ops;
     ltpmg_staging st = stage_ptrs(s, n_txns, n_ops);
-    if (!trusted) {
+    {
+        // a trusted submit skips the per-op and per-txn checks, never the
+        // structure the kernels derive their op ranges from
         ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};
-        int rc = validate_batch(e, &b);
+        int rc = trusted ? validate_structure(e, &b, 1ull << 29) : validate_batch(e, &b);
         if (rc != LTPMG_OK) return rc;
     }
     HIP_TRY(e, hipEventRecord(s.up0, s.stream));
@@ -1136,7 +1178,11 @@ int check_barrier(ltpmg_engine *e) {
     uint32_t w[4] = {};
     HIP_TRY(e, hipMemcpy(w, e->d_sync, sizeof w, hipMemcpyDeviceToHost));
     if (w[3] == 0) return LTPMG_OK;
-    w[1] = w[2] = w[3] = 0;  // keep the epoch
+    // The failed launch may have tagged lock words with epoch + 1 (its own)
+    // without storing that epoch back; move past it so the next batch does
+    // not read those tags as live reservations.
+    w[0] += 2;
+    w[1] = w[2] = w[3] = 0;
     HIP_TRY(e, hipMemcpy(e->d_sync, w, sizeof w, hipMemcpyHostToDevice));
     return fail(e, LTPMG_EHIP, "a fused kernel's grid barrier timed out (grid not co-resident); tables are undefined");
 }

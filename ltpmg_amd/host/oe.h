@@ -48,13 +48,19 @@ uint32_t last_name_code(uint64_t table_seed, uint32_t customer_row);
 // reference's index scan, cuda/src/gpuquery.cu:2641-2650).
 uint32_t customer_by_last_name(uint64_t table_seed, uint32_t w, uint32_t dd, uint32_t k);
 
-// Collects ops of one batch in SoA form.
+// Collects ops of one batch in SoA form.  begin() marks the start of a
+// transaction: within it, a second RMW_ADD of a cell is folded into the
+// first one's delta (a query may name one stock row on two order lines, as
+// the reference's generator allows, src/txn_generator.cpp:262-276), because
+// the engine takes a written cell at most once per transaction.
 struct Sink {
     std::vector<uint32_t> &rows;
     std::vector<uint16_t> &meta;
     std::vector<uint64_t> &args;
     uint64_t seed;        // payload seeds of inserted cells
     uint64_t table_seed;  // the tables' fill seed (last-name lookups)
+    size_t txn_first = 0;
+    void begin() { txn_first = rows.size(); }
     void put(uint32_t kind, uint32_t table, uint32_t col, uint32_t row, uint64_t arg) {
         rows.push_back(row);
         meta.push_back(LTPMG_META(kind, table, col));
@@ -62,6 +68,12 @@ struct Sink {
     }
     void read(uint32_t t, uint32_t col, uint32_t row) { put(LTPMG_OP_READ, t, col, row, 0); }
     void add(uint32_t t, uint32_t col, uint32_t row, int32_t delta) {
+        const uint16_t m = LTPMG_META(LTPMG_OP_RMW_ADD, t, col);
+        for (size_t k = txn_first; k < rows.size(); ++k)
+            if (rows[k] == row && meta[k] == m) {
+                args[k] = (uint64_t)(uint32_t)((uint32_t)args[k] + (uint32_t)delta);
+                return;
+            }
         put(LTPMG_OP_RMW_ADD, t, col, row, (uint64_t)(uint32_t)delta);
     }
     // an inserted cell gets a payload seeded by (seed, table, row, col)

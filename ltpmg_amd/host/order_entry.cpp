@@ -25,6 +25,7 @@ uint32_t customer_by_last_name(uint64_t table_seed, uint32_t w, uint32_t dd, uin
 }
 
 void decompose(const Neworder_Query &q, Sink &out) {
+    out.begin();
     const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.W_ID, q.D_ID, q.C_ID);
     out.read(WAREHOUSE, w::TAX, q.W_ID);
     out.read(DISTRICT, d::TAX, dr);
@@ -50,6 +51,7 @@ void decompose(const Neworder_Query &q, Sink &out) {
 }
 
 void decompose(const Payment_Query &q, Sink &out) {
+    out.begin();
     const uint32_t cid = q.isName ? customer_by_last_name(out.table_seed, q.C_W_ID, q.C_D_ID, q.C_LAST) : q.C_ID;
     const uint32_t dr = district_row(q.W_ID, q.D_ID), cr = customer_row(q.C_W_ID, q.C_D_ID, cid);
     out.add(WAREHOUSE, w::YTD, q.W_ID, (int32_t)q.H_AMOUNT);
@@ -66,6 +68,7 @@ void decompose(const Payment_Query &q, Sink &out) {
 }
 
 void decompose(const Orderstatus_Query &q, Sink &out) {
+    out.begin();
     const uint32_t cid = q.isName ? customer_by_last_name(out.table_seed, q.W_ID, q.D_ID, q.C_LAST) : q.C_ID;
     const uint32_t cr = customer_row(q.W_ID, q.D_ID, cid);
     out.read(CUSTOMER, c::BALANCE, cr);
@@ -82,6 +85,7 @@ void decompose(const Orderstatus_Query &q, Sink &out) {
 }
 
 void decompose(const Stocklevel_Query &q, Sink &out) {
+    out.begin();
     out.read(DISTRICT, d::NEXT_O_ID, district_row(q.W_ID, q.D_ID));
     for (uint32_t k = 0; k < 10; ++k) {
         out.read(ORDERLINE, ol::I_ID, q.O_OL_ID[k]);

@@ -164,6 +164,51 @@ def digest_columns(table: int, cols: list[np.ndarray]) -> int:
     return int(total)
 
 
+def digest_rows(table: int, col: int, rows: np.ndarray, cells: np.ndarray) -> int:
+    """The engine's table-digest terms of the given cells only: the sum over
+    rows[i] of the same per-cell hash digest_columns sums (cells[i] holds
+    the cell's bytes).  With it, digest(after) - digest(before) of a whole
+    table equals the sum over the touched cells of (term(after) -
+    term(before)), which checks a table too large to compare cell by cell."""
+    rows = np.asarray(rows, np.uint64)
+    n, w = cells.shape
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    ids = (rows << np.uint64(10)) | np.uint64(table << 6) | np.uint64(col)
+    with np.errstate(over="ignore"):
+        h = _np_mix64(golden * (ids + np.uint64(1)))
+    padded = np.zeros((n, (w + 7) // 8 * 8), np.uint8)
+    padded[:, :w] = cells
+    words = padded.view("<u8")
+    for k in range(words.shape[1]):
+        h = _np_mix64(h ^ words[:, k])
+    with np.errstate(over="ignore"):
+        return int(h.sum(dtype=np.uint64))
+
+
+def touched_cells(batches, schema: dict) -> dict:
+    """{(table, col): sorted unique rows} that any of the batches names."""
+    out = {}
+    for t, (n, widths) in schema.items():
+        for c in range(len(widths)):
+            parts = [b.rows[(((b.meta >> 8) & 15) == t) & (((b.meta >> 2) & 63) == c)] for b in batches]
+            rows = np.unique(np.concatenate(parts)) if parts else np.zeros(0, np.uint32)
+            if rows.size:
+                out[(t, c)] = rows.astype(np.uint32)
+    return out
+
+
+def sparse_image(schema: dict, seed: int, batches) -> tuple[Image, dict]:
+    """An image of the tables the batches name that fills only the cells they
+    touch (untouched pages of the zeroed columns stay unmapped), and those
+    cells as {(table, col): rows}."""
+    cells = touched_cells(batches, schema)
+    used = {t for t, _ in cells}
+    im = Image({t: schema[t] for t in sorted(used)})
+    for (t, c), rows in cells.items():
+        im.fill_cells(seed, t, c, rows)
+    return im, cells
+
+
 def replay(image: Image, batch, mode: int, res_stride: int):
     """Serial replay in TID order; mutates image.  Returns (commit, results)."""
     n_txns = len(batch.txn_off) - 1

@@ -88,8 +88,11 @@ def test_sort_reference_kat_descending_million():
 
 @pytest.mark.parametrize("n,bits", [(1, 8), (100, 7), (4097, 8), (70000, 13), (300001, 24),
                                     (1 << 20, 28), (123457, 32), (100000, 9), (65537, 17),
-                                    (250001, 18), (1 << 21, 27)])
+                                    (250001, 18), (1 << 21, 27), ((1 << 24) + 4097, 28)])
 def test_sort_is_stable_and_matches_numpy(n, bits):
+    # (2^24 + 4097, 28): 7-bit digits over 8195 tiles of 2048 keys, more than
+    # 64 tile rows per offset-scan chunk, so the scans take their general
+    # (one thread per digit) path
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
     if bits < 32:
@@ -483,6 +486,54 @@ def test_invalid_batches_are_rejected():
             e.execute(5)
         # the engine is still usable
         _compare(e, im, Batch.from_txns([[(OP_UPDATE, 0, 0, 3, 5)], [(OP_READ, 0, 0, 3, 0)]]), MODE_O)
+
+
+def test_written_cell_accessed_again_in_a_txn_is_rejected():
+    # a txn may read a cell and then write it, but nothing may touch a cell
+    # after the txn wrote it (the engine's reads see the batch-start snapshot
+    # and a txn's writes of one cell would race)
+    with Engine() as e:
+        im = _setup(e, {0: (100, [4, 8])}, 2)
+        bad = [
+            [(OP_UPDATE, 0, 0, 5, 1), (OP_READ, 0, 0, 5, 0)],
+            [(OP_RMW_ADD, 0, 1, 7, 1), (OP_RMW_ADD, 0, 1, 7, 2)],
+            [(OP_READ, 0, 0, 9, 0), (OP_UPDATE, 0, 0, 9, 3), (OP_RMW_ADD, 0, 0, 9, 1)],
+            [(OP_RMW_ADD, 0, 1, 3, 1), (OP_READ, 0, 0, 4, 0), (OP_UPDATE, 0, 1, 3, 1)],
+        ]
+        for ops in bad:
+            b = Batch.from_txns([[(OP_READ, 0, 0, 1, 0)], ops])
+            with pytest.raises(LtpmgError) as ei:
+                e.submit(b)
+            assert ei.value.code == -1 and "txn 1" in str(ei.value), str(ei.value)
+            with pytest.raises(LtpmgError):
+                e.submit_staged(b, trusted=False)
+        ok = Batch.from_txns([
+            [(OP_READ, 0, 0, 5, 0), (OP_UPDATE, 0, 0, 5, 1)],           # read, then write
+            [(OP_READ, 0, 1, 5, 0), (OP_READ, 0, 1, 5, 0)],             # two reads
+            [(OP_UPDATE, 0, 0, 6, 2), (OP_UPDATE, 0, 1, 6, 3)],         # two cells of one row
+            [(OP_RMW_ADD, 0, 0, 7, 4), (OP_RMW_ADD, 0, 0, 8, 5)],
+        ])
+        for mode in BOTH:
+            _compare(e, im, ok, mode)
+
+
+def test_trusted_submit_still_checks_the_txn_offsets():
+    # a trusted submit skips the per-op checks, but the kernels derive every
+    # op range from txn_off, so its structure is always checked
+    with Engine() as e:
+        _setup(e, {0: (100, [4])}, 3)
+        bad = [
+            Batch(np.array([0, 1, 1], np.uint32), np.array([1], np.uint32),
+                  np.array([0], np.uint16), np.array([0], np.uint64)),   # empty txn
+            Batch(np.array([0, 2, 1], np.uint32), np.array([1, 2], np.uint32),
+                  np.array([0, 0], np.uint16), np.array([0, 0], np.uint64)),  # offsets go back
+            Batch(np.array([1, 2], np.uint32), np.array([1, 2], np.uint32),
+                  np.array([0, 0], np.uint16), np.array([0, 0], np.uint64)),  # txn_off[0] != 0
+        ]
+        for b in bad:
+            with pytest.raises(LtpmgError) as ei:
+                e.submit_staged(b, trusted=True)
+            assert ei.value.code == -1
 
 
 def test_hbm_budget_is_enforced():

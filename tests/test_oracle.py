@@ -161,6 +161,47 @@ def test_mode_o_last_writer(seed):
         assert im.cols[0][col][row].tobytes() == orc.payload(arg, snap.schema[0][1][col]).tobytes()
 
 
+def _add_words(cell: np.ndarray, delta: int) -> np.ndarray:
+    """Every little-endian u32 word of the cell plus delta (mod 2^32)."""
+    w = cell.copy().view("<u4")
+    return (w + np.uint32(delta & 0xFFFFFFFF)).astype("<u4").view(np.uint8)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mode_o_mixed_update_and_rmw(seed):
+    # UPDATE and RMW_ADD interleaved on the same cells: a READ returns the
+    # payload of the last UPDATE before it (or the snapshot when there is
+    # none) plus the sum of the RMW deltas since that UPDATE; the final cell
+    # is the same fold over all of the cell's ops.  Deltas are signed and
+    # widths are not multiples of 8, so a dropped delta, a delta carried
+    # across an UPDATE, a sign slip or a lost payload tail all fail.
+    rng = np.random.default_rng(500 + seed)
+    b = random_batch(rng, n_txns=300, n_rows=12, n_cols=3, max_len=5,
+                     kinds=(OP_READ, OP_UPDATE, OP_RMW_ADD))
+    widths = [4, 12, 100]
+    im = orc.Image({0: (12, widths)}).fill(13 + seed)
+    snap = im.copy()
+    commit, res = orc.replay(im, b, orc.MODE_O, 128)
+    assert commit.all()
+    state = {}  # (row, col) -> (base bytes, delta since the base)
+    n_mixed = 0
+    for i in range(b.n_ops):
+        kind, col, row = int(b.meta[i] & 3), int((b.meta[i] >> 2) & 63), int(b.rows[i])
+        w = widths[col]
+        base, d = state.get((row, col), (snap.cols[0][col][row], 0))
+        if kind == OP_READ:
+            assert res[i, :w].tobytes() == _add_words(base, d).tobytes(), (i, row, col)
+            assert not res[i, w:].any()
+            n_mixed += int(d != 0 and (row, col) in state and base is not snap.cols[0][col][row])
+        elif kind == OP_UPDATE:
+            state[(row, col)] = (orc.payload(int(b.args[i]), w), 0)
+        else:
+            state[(row, col)] = (base, d + int(np.int64(b.args[i].astype(np.int64))))
+    assert n_mixed > 0  # some READs see an UPDATE followed by RMWs
+    for (row, col), (base, d) in state.items():
+        assert im.cols[0][col][row].tobytes() == _add_words(base, d).tobytes(), (row, col)
+
+
 def test_replay_rejects_malformed_batches():
     im = _image()
     bad_row = Batch.from_txns([[(OP_READ, 0, 0, 64, 0)]])
