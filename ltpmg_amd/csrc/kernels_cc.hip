@@ -42,10 +42,27 @@ struct Tile {
 };
 
 // Txns [t0, t1) of a tile; fills owner[o - op0] = txn index in the tile.
-__device__ inline Tile fill_owner(const OpView &ov, uint32_t t0, uint32_t t1, uint16_t *owner) {
-    __syncthreads();  // the previous user of owner[] is done
-    const uint32_t op0 = ov.txn_off[t0];
-    Tile t{t0, t1 - t0, op0, ov.txn_off[t1] - op0};
+// max_nt / max_ops are the LDS capacities of owner[] and the caller's
+// per-txn arrays.  The host checks txn_off before a batch runs, but a batch
+// handed over on the device (or submitted as trusted) is only checked here:
+// a tile whose offsets are not nondecreasing, or that would overrun LDS or
+// the batch, becomes empty, so a malformed batch never makes a kernel
+// access memory outside its buffers (its txns then simply do not commit).
+__device__ inline Tile fill_owner(const OpView &ov, uint32_t t0, uint32_t t1, uint16_t *owner,
+                                  uint32_t max_nt = kTileOps, uint32_t max_ops = kTileOpsMax) {
+    __shared__ uint32_t s_bad;
+    __syncthreads();  // the previous user of owner[] and s_bad is done
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    bool bad = t1 < t0 || t1 > ov.n_txns || t1 - t0 > max_nt;
+    const uint32_t op0 = bad ? 0u : ov.txn_off[t0];
+    const uint32_t end = bad ? 0u : ov.txn_off[t1];
+    bad = bad || end < op0 || end > ov.n_ops || end - op0 > max_ops;
+    Tile t{t0, bad ? 0u : t1 - t0, op0, bad ? 0u : end - op0};
+    for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x)
+        if (ov.txn_off[t.t0 + j] > ov.txn_off[t.t0 + j + 1]) s_bad = 1;
+    __syncthreads();
+    if (s_bad) t.nt = t.nops = 0;  // uniform: every thread read the same flag
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) {
         uint32_t a = ov.txn_off[t.t0 + j] - t.op0, b = ov.txn_off[t.t0 + j + 1] - t.op0;
         for (uint32_t o = a; o < b; ++o) owner[o] = (uint16_t)j;
@@ -483,21 +500,21 @@ __global__ __launch_bounds__(kBlock) void k_cc_fused(DevTables dt, OpView ov, ui
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[0] = 0;
     Tile t{0u, 0u, 0u, 0u};
     for (uint32_t j = 0; j < nk; ++j) {
-        t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner, TILE, TILE + LTPMG_MAX_TXN_OPS);
         tile_reserve(dt, ov, t, owner, epoch, reinterpret_cast<unsigned long long *>(list),
                      list + 2 * kSlots, kSlots);
     }
     if (!grid_sync(bar)) return;
     uint32_t committed = 0;
     for (uint32_t j = 0; j < nk; ++j) {
-        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner, TILE, TILE + LTPMG_MAX_TXN_OPS);
         committed += tile_validate_read<G, VEC16>(dt, ov, t, owner, epoch, ok, list, commit,
                                                   results, res_stride);
     }
     if (threadIdx.x == 0 && committed) atomicAdd(&counters[0], committed);
     if (!grid_sync(bar)) return;
     for (uint32_t j = 0; j < nk; ++j) {
-        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner);
+        if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner, TILE, TILE + LTPMG_MAX_TXN_OPS);
         tile_write<G, VEC16>(dt, ov, t, owner, ok, nk > 1, commit, list);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *epoch_ctr = epoch;

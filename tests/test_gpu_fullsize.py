@@ -212,7 +212,9 @@ def test_c5_served_run_replayed():
         t0 = time.time()
         d0 = _create(e, schema)
         log(f"C5: {e.stats_hbm() / 2**30:.1f} GiB allocated in {time.time() - t0:.1f} s")
-        out = e.serve(pool, arrival_rate=2e7, n_arrivals=pool.n_txns, max_batch=1 << 15, max_wait_us=500.0,
+        # recording copies every batch's result slots back, so the run is
+        # host-bound; batches of at most 2^14 keep it to 64 or more of them
+        out = e.serve(pool, arrival_rate=2e7, n_arrivals=pool.n_txns, max_batch=1 << 14, max_wait_us=500.0,
                       n_inflight=2, mode=MODE_O, seed=11, record=True, record_results=True)
         assert not out["record_overflow"] and out["committed"] == pool.n_txns
         off, ids, flags = out["batch_off"], out["batch_txns"], out["batch_commit"]
