@@ -195,6 +195,30 @@ int ltpmg_selftest_segscan(ltpmg_engine *e, const uint32_t *vals, const uint8_t 
  * in the reference. */
 int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t iters, double *gbps);
 
+/* Allocation-free execute.  No counterpart in the reference: this is for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It runs
+ * the CC + execute kernels of ltpmg_execute on a batch that is already in
+ * device memory (dev->txn_off / rows / meta / args are device pointers,
+ * n_txns / n_ops host values) and writes dev_commit (n_txns bytes) and
+ * dev_results (n_ops * ltpmg_result_stride() bytes), both device buffers
+ * the caller owns, on hip_stream (a hipStream_t; NULL = the null stream).
+ * It allocates nothing: its sort, scan and tile scratch is one static
+ * device array per GPU of about 5 MiB, held until the process ends, which
+ * bounds a batch to LTPMG_STATIC_MAX_OPS ops (larger ones fail with
+ * LTPMG_EINVAL).  It issues only kernel launches (no memset or copy nodes),
+ * so it can be captured into a graph.  Every call on one GPU shares that
+ * array: calls that overlap on a GPU race, and nothing orders a call
+ * against batches run through slots -- the caller serialises them (e.g.
+ * ltpmg_wait on every slot first).  The host cannot inspect device arrays,
+ * so op fields are checked by the kernels (invalid ops are skipped), and a
+ * malformed txn_off only leaves the affected commit flags undefined (every
+ * access stays inside the batch and the tables); the batch must otherwise
+ * follow the rules of ltpmg_submit_batch. */
+#define LTPMG_STATIC_MAX_OPS (1u << 18)
+int ltpmg_execute_static(ltpmg_engine *e, uint32_t mode, const ltpmg_batch *dev, uint8_t *dev_commit,
+                         void *dev_results, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,10 +90,80 @@ struct ltpmg_engine {
     KeyDecode kdec{};  // ordered-mode key decode of the live tables
     TabDesc *d_tabs = nullptr;
     unsigned long long *d_digest = nullptr;
+    uint8_t *static_scratch = nullptr;  // this device's copy of `scratch` (ltpmg_execute_static)
     std::vector<Slot> slots;
     std::string err;
     std::mutex mu;
 };
+
+namespace {
+
+// Scratch of ltpmg_execute_static: the sort, scan and tile buffers of one
+// batch of up to LTPMG_STATIC_MAX_OPS ops (about 4.7 MB of the 5 MiB below
+// at the default sort tile).  A file-scope device array, so each GPU holds
+// one copy from module load to process end and the entry point never
+// allocates.  Host code must not pass `scratch` itself to a kernel (in host
+// code the name is a host-side stand-in, not a device address): the device
+// address of each GPU's copy is looked up once per device in ltpmg_open and
+// kept in g_static_ptr / ltpmg_engine::static_scratch.
+constexpr size_t kStaticScratchBytes = 5u << 20;
+__device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+constexpr int kStaticDevices = 64;
+std::mutex g_static_mu;
+uint8_t *g_static_ptr[kStaticDevices] = {};
+
+// Carves the static scratch into the buffers of one batch of up to
+// LTPMG_STATIC_MAX_OPS ops; false if they do not fit (a build with larger
+// sort tiles).
+struct StaticBufs {
+    OrderBufs ob{};
+    uint32_t *tile_t0 = nullptr, *counters = nullptr;
+};
+bool carve_static(uint8_t *base, StaticBufs &out) {
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *p = base + at;
+        at += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t n = LTPMG_STATIC_MAX_OPS;
+    const size_t sg = (seg_tiles((uint32_t)n) + 4) & ~(size_t)3;
+    StaticBufs b;
+    for (int i = 0; i < 2; ++i) {
+        b.ob.sort.keys[i] = reinterpret_cast<uint32_t *>(take(n * 4));
+        b.ob.sort.vals[i] = reinterpret_cast<uint32_t *>(take(n * 4));
+    }
+    b.ob.sort.hist = reinterpret_cast<uint32_t *>(take(sort_hist_words(n) * 4));
+    b.ob.sort.partials = reinterpret_cast<uint32_t *>(take(sort_partials_needed() * 4));
+    b.ob.tile_flag = reinterpret_cast<uint32_t *>(take((sg * 3 + 2) * 4));
+    b.ob.tile_set = b.ob.tile_flag + sg;
+    b.ob.tile_delta = b.ob.tile_flag + 2 * sg;
+    b.ob.wtotal = b.ob.tile_flag + 3 * sg;
+    b.ob.wlist = reinterpret_cast<uint4 *>(take(sg * 16));
+    b.tile_t0 = reinterpret_cast<uint32_t *>(take((n / kTileOps + 2) * 4));
+    b.counters = reinterpret_cast<uint32_t *>(take(16));
+    if (at > kStaticScratchBytes) return false;
+    out = b;
+    return true;
+}
+
+// The device address of this device's `scratch`, looked up once per device
+// (the device must be current).
+hipError_t static_scratch_of(int device, uint8_t **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kStaticDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_static_mu);
+    if (!g_static_ptr[device]) {
+        void *p = nullptr;
+        hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess) return st;
+        g_static_ptr[device] = static_cast<uint8_t *>(p);
+    }
+    *out = g_static_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
 
 namespace {
 
@@ -552,6 +622,11 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
         hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess)
         rc = LTPMG_EHIP;
     if (rc == LTPMG_OK) rc = sync_descriptors(e);
+    if (rc == LTPMG_OK && cfg->device < kStaticDevices) {
+        // the allocation-free entry point's scratch on this device (current)
+        hipError_t st = static_scratch_of(cfg->device, &e->static_scratch);
+        if (st != hipSuccess) rc = fail(e, LTPMG_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(st));
+    }
     e->slots.resize(cfg->n_slots);
     for (Slot &s : e->slots) {
         if (rc != LTPMG_OK) break;
@@ -1026,6 +1101,42 @@ extern "C" int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t i
     dev_free(e, a, bytes);
     dev_free(e, b, bytes);
     return rc;
+}
+
+// The allocation-free execute (include/ltpmg.h): the kernels of
+// ltpmg_execute on a device-resident batch, their scratch carved from this
+// device's static array.  Only kernel launches are issued (counters and the
+// final-write list are cleared by kernels), no events are recorded and no
+// memory is allocated or freed.
+extern "C" int ltpmg_execute_static(ltpmg_engine *e, uint32_t mode, const ltpmg_batch *dev, uint8_t *dev_commit,
+                                    void *dev_results, void *hip_stream) {
+    if (!e || !dev) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (mode != LTPMG_MODE_A && mode != LTPMG_MODE_O) return fail(e, LTPMG_EINVAL, "mode %u", mode);
+    if (dev->n_ops > LTPMG_STATIC_MAX_OPS)
+        return fail(e, LTPMG_EINVAL, "batch of %u ops exceeds the %u ops the static scratch holds", dev->n_ops,
+                    LTPMG_STATIC_MAX_OPS);
+    if (dev->n_txns > dev->n_ops)
+        return fail(e, LTPMG_EINVAL, "%u txns but %u ops (every txn has at least one op)", dev->n_txns, dev->n_ops);
+    if (dev->n_ops == 0) return LTPMG_OK;
+    if (!dev->txn_off || !dev->rows || !dev->meta || !dev->args || !dev_commit || !dev_results)
+        return fail(e, LTPMG_EINVAL, "null device array");
+    if (!e->static_scratch) return fail(e, LTPMG_ESTATE, "no static scratch on device %d", e->device);
+    StaticBufs sb;
+    if (!carve_static(e->static_scratch, sb))
+        return fail(e, LTPMG_ESTATE, "the static scratch is too small for this build's sort tiles");
+    const uint32_t rs = max_slot_all(e);
+    OpView ov{dev->txn_off, dev->rows, dev->meta, dev->args, dev->n_txns, dev->n_ops};
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    NoRec rec;
+    auto now = []() { return LTPMG_OK; };
+    uint8_t *res = static_cast<uint8_t *>(dev_results);
+    int rc = mode == LTPMG_MODE_A ? launch_mode_a(e, st, ov, sb.tile_t0, sb.counters, dev_commit, res, rs, rec, now)
+                                  : launch_mode_o(e, st, ov, sb.ob, dev_commit, res, rs, rec, now);
+    if (rc != LTPMG_OK) return rc;
+    HIP_TRY(e, hipGetLastError());
+    return LTPMG_OK;
 }
 
 // ---------------- batching form (include/ltpmg_stream.h) ----------------

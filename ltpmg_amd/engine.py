@@ -104,7 +104,10 @@ ENGINE_API = {
                                          C.c_uint32]),
     "ltpmg_selftest_copy": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32,
                                       C.POINTER(C.c_double)]),
+    "ltpmg_execute_static": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Batch), C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
 }
+STATIC_MAX_OPS = 1 << 18  # LTPMG_STATIC_MAX_OPS
 
 class _StreamPlan(C.Structure):
     _fields_ = [("merge_max_ops", C.c_uint32), ("fused_max_ops", C.c_uint32),
