@@ -117,6 +117,14 @@ typedef struct {
  * (cuda/include/gpuscheduler.cuh:54) with an explicit handle. */
 int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out);
 int ltpmg_close(ltpmg_engine *e);
+/* The process-wide engine of a device: opened with one slot (default
+ * budget) on the first call, the same handle on every later call, until
+ * ltpmg_close of that handle.  The reference's eight C++ entry points take no
+ * handle (its GPUScheduler is a global singleton,
+ * cuda/include/gpuscheduler.cuh:54); the library that implements them runs
+ * on this engine, and host code beside it reads what they ran (commit
+ * counts, results, digests) through the same handle. */
+int ltpmg_shared_engine(int32_t device, ltpmg_engine **out);
 const char *ltpmg_last_error(const ltpmg_engine *e);
 
 /* Tables in HBM: columnar, each column a 256-byte-aligned array of slots of

@@ -14,12 +14,15 @@
 #include "define.h"
 #include "txn_structure.h"
 
+// Result and Param keep the reference's members, types and order exactly
+// (include/param.h:7-118), so the object a reference host allocates and
+// passes through the extern entry points has the layout this build reads.
+// This build's added settings are in run_options.h; commit counts come from
+// the engine (ltpmg_stats), not from Result.
 struct Result {
     double cost;          // device seconds of the last batch's CC + execute
     uint32_t epoch;
     uint32_t batch_size;
-    uint32_t committed = 0;  // added: txns committed by the last batch
-    uint32_t aborted = 0;
 };
 
 class Param {
@@ -33,7 +36,7 @@ public:
     uint32_t neworder_percent;
     std::string filepath;
     uint32_t table_cnt;
-    uint32_t device_cnt = 0;
+    uint32_t device_cnt = 0;        // default member initialisers leave the layout unchanged
     uint32_t *device_IDs = nullptr;
     std::string data_distribution;
     double zipf_config;
@@ -105,17 +108,6 @@ public:
     uint32_t scan_batch_size = 0;
     uint32_t delete_batch_size = 0;
 
-    // added by this build
-    uint32_t cc_mode = 1;      // LTPMG_MODE_O (1) or LTPMG_MODE_A (0)
-    uint64_t seed = 0;
-    uint32_t kv_size = 0;      // rows of the KV / KV_C1 table
-    uint32_t kv_fields = 10;
-    uint32_t kv_field_bytes = 100;
-    uint32_t kv_ops = 16;
-    uint32_t read_percent = 50;
-    uint32_t kv_sub_txn_size = 0;  // upper bound on device ops per batch
-    uint32_t oe_append_cap = 1u << 20;  // order-entry append tables, in transactions
-
     Result result;
 
     Param();
@@ -138,9 +130,6 @@ public:
     uint32_t get_subtxn_cnt(const std::type_info &query_kind);
 
     uint32_t get_txn_batch_size(const std::type_info &query_kind);
-
-    bool is_ycsb() const { return benchmark.rfind("YCSB_", 0) == 0; }
-    bool is_kv() const { return benchmark == "KV" || benchmark == "KV_C1"; }
-    // order-entry schema (include/ltpmg_oe.h): TPCC_PART, TPCC_ALL and the OE mix
-    bool is_oe() const { return benchmark == "TPCC_PART" || benchmark == "TPCC_ALL" || benchmark == "OE"; }
 };
+
+#include "run_options.h"

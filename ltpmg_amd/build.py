@@ -1,6 +1,8 @@
 """Builds the native libraries in-tree.
 
   ltpmg_amd/lib/libltpmg_hip.so   HIP engine for gfx950 (C ABI, include/ltpmg.h)
+  ltpmg_amd/lib/libltpmg_gpu.so   the reference's eight extern entry points over the C ABI
+                                  (the counterpart of its libgpu; exports only those)
   ltpmg_amd/lib/libltpmg_host.so  C++ host API (drop-in Scheduler/Database/Query/...)
   ltpmg_amd/lib/ltpmg             command-line driver (the reference's ltpmg binary)
   oracle/_build/liboracle.so      the CPU checker (test infrastructure)
@@ -72,30 +74,42 @@ def build_oracle() -> Path:
     return ROOT / "oracle" / "_build" / "liboracle.so"
 
 
+# libltpmg_gpu.so: the eight entry points and what they call, linked only to
+# libltpmg_hip.so; everything but the eight is kept local by the version script
+GPU_SOURCES = ["gpu_shim.cpp", "pack.cpp", "oe_decompose.cpp"]
+
+
+def _link(out: Path, objs: list[Path], *libs: str) -> None:
+    tmp = out.with_name(out.name + ".tmp")
+    _run([CXX, "-shared", "-pthread", "-o", str(tmp), *map(str, objs), *libs])
+    os.replace(tmp, out)
+
+
 def build_host(jobs: int = 8) -> Path | None:
     srcs = sorted(HOST.glob("*.cpp"))
     if not srcs:
         return None
     OBJ.mkdir(parents=True, exist_ok=True)
     LIB.mkdir(parents=True, exist_ok=True)
-    lib_srcs = [s for s in srcs if s.name != "main.cpp"]
-    cmds, objs = [], []
-    for src in lib_srcs:
-        obj = OBJ / ("host_" + src.stem + ".o")
-        objs.append(obj)
-        cmds.append([CXX, *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
+    cmds, obj = [], {}
+    for src in srcs:
+        if src.name == "main.cpp":
+            continue
+        obj[src.name] = OBJ / ("host_" + src.stem + ".o")
+        cmds.append([CXX, *CXX_FLAGS, "-c", str(src), "-o", str(obj[src.name])])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
+    rpath = "-Wl,-rpath,$ORIGIN"
+    _link(LIB / "libltpmg_gpu.so", [obj[n] for n in GPU_SOURCES], f"-Wl,--version-script={HOST / 'libltpmg_gpu.map'}",
+          f"-L{LIB}", "-lltpmg_hip", rpath)
+    host_objs = [o for n, o in obj.items() if n != "gpu_shim.cpp"]
     out = LIB / "libltpmg_host.so"
-    tmp = LIB / "libltpmg_host.so.tmp"
-    _run([CXX, "-shared", "-pthread", "-o", str(tmp), *map(str, objs), f"-L{LIB}", "-lltpmg_hip",
-          "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, out)
+    _link(out, host_objs, f"-L{LIB}", "-lltpmg_gpu", "-lltpmg_hip", rpath)
     main = HOST / "main.cpp"
     if main.exists():
         exe = LIB / "ltpmg"
         _run([CXX, *CXX_FLAGS, str(main), "-o", str(exe) + ".tmp", f"-L{LIB}", "-lltpmg_host",
-              "-lltpmg_hip", "-Wl,-rpath,$ORIGIN"])
+              "-lltpmg_gpu", "-lltpmg_hip", rpath])
         os.replace(str(exe) + ".tmp", exe)
     return out
 

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -109,6 +110,9 @@ namespace {
 constexpr size_t kStaticScratchBytes = 5u << 20;
 __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
 constexpr int kStaticDevices = 64;
+// ltpmg_shared_engine's engines, by device
+std::mutex g_shared_mu, g_shared_open_mu;
+std::map<int32_t, ltpmg_engine *> g_shared;
 std::mutex g_static_mu;
 uint8_t *g_static_ptr[kStaticDevices] = {};
 
@@ -646,6 +650,14 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
 
 int ltpmg_close(ltpmg_engine *e) {
     if (!e) return LTPMG_EINVAL;
+    {
+        std::lock_guard<std::mutex> g(g_shared_mu);
+        for (auto it = g_shared.begin(); it != g_shared.end(); ++it)
+            if (it->second == e) {
+                g_shared.erase(it);
+                break;
+            }
+    }
     (void)hipSetDevice(e->device);
     for (Slot &s : e->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
@@ -668,6 +680,31 @@ int ltpmg_close(ltpmg_engine *e) {
     if (e->d_sync) (void)hipFree(e->d_sync);
     delete e;
     return LTPMG_OK;
+}
+
+int ltpmg_shared_engine(int32_t device, ltpmg_engine **out) {
+    if (!out) return LTPMG_EINVAL;
+    *out = nullptr;
+    // one opener at a time; the map's own lock is not held across
+    // ltpmg_open, whose failure path calls ltpmg_close, which takes it
+    std::lock_guard<std::mutex> opening(g_shared_open_mu);
+    {
+        std::lock_guard<std::mutex> g(g_shared_mu);
+        auto it = g_shared.find(device);
+        if (it != g_shared.end()) {
+            *out = it->second;
+            return LTPMG_OK;
+        }
+    }
+    ltpmg_cfg cfg{};
+    cfg.device = device;
+    cfg.n_slots = 1;
+    int rc = ltpmg_open(&cfg, out);
+    if (rc == LTPMG_OK) {
+        std::lock_guard<std::mutex> g(g_shared_mu);
+        g_shared[device] = *out;
+    }
+    return rc;
 }
 
 const char *ltpmg_last_error(const ltpmg_engine *e) { return e ? e->err.c_str() : "no engine"; }

@@ -78,6 +78,7 @@ class _KvSpec(C.Structure):
 ENGINE_API = {
     "ltpmg_open": (C.c_int, [C.POINTER(_Cfg), C.POINTER(C.c_void_p)]),
     "ltpmg_close": (C.c_int, [C.c_void_p]),
+    "ltpmg_shared_engine": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "ltpmg_last_error": (C.c_char_p, [C.c_void_p]),
     "ltpmg_create_table": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_TableDesc)]),
     "ltpmg_fill_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),

@@ -2,6 +2,7 @@
 // (the reference's src/common.cpp; flag names and defaults kept, except
 // deviceIDs, which defaults to the one GPU of this build).
 #include "common.h"
+#include "run_options.h"
 
 #include <ctime>
 #include <map>
@@ -142,6 +143,9 @@ void initialize_dependency(int argc, char **argv) {
         std::exit(2);
     }
     ltpmg_log::min_level = (int)FLAGS_minloglevel;
+    // the GPU library (libltpmg_gpu.so) reads this build's settings from the
+    // environment: it sees only the reference's Param
+    ltpmg_host::export_gpu_options();
 }
 
 void free_dependency() {}

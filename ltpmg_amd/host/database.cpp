@@ -82,7 +82,7 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
         tables_for_GPU[i] = Global_Table{nullptr, nullptr, nullptr};
         index_for_GPU[i].index = nullptr;
     }
-    if (param->is_oe()) {
+    if (ltpmg_host::is_oe(*param)) {
         // nine order-entry tables with the widths of include/ltpmg_oe.h; the
         // per-type counts of Global_Table_Info cannot express them, so only
         // the row counts are recorded and the engine fills the tables.  The
@@ -97,9 +97,9 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
                 LOG(FATAL) << filepath << ": table " << i << " is " << rows[i].get_table_name() << ", expected "
                            << kNames[i];
         ltpmg_table_desc d[LTPMG_OE_TABLES];
-        if (n_tables != LTPMG_OE_TABLES || ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, d))
+        if (n_tables != LTPMG_OE_TABLES || ltpmg_oe_schema(param->warehouse_size, ltpmg_host::options().oe_append_cap, d))
             LOG(FATAL) << "order-entry schema: " << param->warehouse_size << " partitions, append capacity "
-                       << param->oe_append_cap;
+                       << ltpmg_host::options().oe_append_cap;
         for (uint32_t i = 0; i < n_tables; ++i) {
             tables_for_GPU_info[i].table_size = d[i].n_rows;
             tables_for_GPU_info[i].int_size = d[i].n_cols;
@@ -107,10 +107,10 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
         LOG(INFO) << "order-entry tables for " << param->warehouse_size << " partitions are filled on the device";
         return;
     }
-    if (!param->is_ycsb() && !param->is_kv())
+    if (!ltpmg_host::is_ycsb(*param) && !ltpmg_host::is_kv(*param))
         LOG(FATAL) << "benchmark " << param->benchmark << ": schema not built in this engine";
     Global_Table_Info &t = tables_for_GPU_info[0];
-    if (param->is_ycsb()) {
+    if (ltpmg_host::is_ycsb(*param)) {
         // column counts by type from the schema (src/database.cpp:86-95)
         if (rows.size() != 1) LOG(FATAL) << "YCSB needs a one-table schema file (YCSB.json)";
         for (const Attribute &a : rows[0].getAttributes()) {
@@ -123,14 +123,14 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
         t.string_length = 8;
         t.table_size = param->ycsb_size;
     } else if (param->benchmark == "KV_C1") {
-        t.int_size = param->kv_fields;
-        t.table_size = param->kv_size;
+        t.int_size = ltpmg_host::options().kv_fields;
+        t.table_size = ltpmg_host::options().kv_size;
     } else {  // KV: device-filled, no host copy
-        t.string_size = param->kv_fields;
-        t.string_length = param->kv_field_bytes / 4;
-        t.table_size = param->kv_size;
+        t.string_size = ltpmg_host::options().kv_fields;
+        t.string_length = ltpmg_host::options().kv_field_bytes / 4;
+        t.table_size = ltpmg_host::options().kv_size;
         LOG(INFO) << "KV table " << t.table_size << " x " << t.string_size << " x "
-                  << param->kv_field_bytes << " B is filled on the device";
+                  << ltpmg_host::options().kv_field_bytes << " B is filled on the device";
         return;
     }
     const size_t n = t.table_size;
@@ -141,12 +141,12 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
               << ", string " << t.string_size;
 }
 
-void Database::copy_to_global_row(std::shared_ptr<Param> param) {
+void Database::copy_to_global_row(std::shared_ptr<Param>) {
     for (uint32_t tb = 0; tb < n_tables; ++tb) {
         const Global_Table_Info &t = tables_for_GPU_info[tb];
         Global_Table &g = tables_for_GPU[tb];
         if (!g.int_data && !g.double_data && !g.string_data) continue;
-        const uint64_t seed = param->seed;
+        const uint64_t seed = ltpmg_host::options().seed;
         const uint32_t sw = t.string_length * 4;
         auto fill = [&](size_t r0, size_t r1) {
             for (size_t r = r0; r < r1; ++r) {

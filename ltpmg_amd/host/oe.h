@@ -40,6 +40,10 @@ inline uint32_t district_row(uint32_t w, uint32_t dd) { return w * kDistricts + 
 inline uint32_t customer_row(uint32_t w, uint32_t dd, uint32_t cc) { return district_row(w, dd) * kCustomers + cc; }
 inline uint32_t stock_row(uint32_t w, uint32_t item) { return w * kStock + item; }
 
+// The nine table descriptors for W partitions and append tables of cap
+// transactions (the body of ltpmg_oe_schema).
+int schema(uint32_t W, uint32_t cap, ltpmg_table_desc out[LTPMG_OE_TABLES]);
+
 // Last-name code of a customer row: the first little-endian u64 of its
 // c_last cell, as ltpmg_fill_table(table_seed) writes it, mod kLastNames.
 uint32_t last_name_code(uint64_t table_seed, uint32_t customer_row);

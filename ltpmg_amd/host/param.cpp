@@ -18,9 +18,7 @@ Param::Param()
     result.cost = 0.0;
     result.epoch = epoch_tp;
     result.batch_size = batch_size;
-    cc_mode = (FLAGS_cc_mode == "A" || FLAGS_cc_mode == "a") ? LTPMG_MODE_A : LTPMG_MODE_O;
-    seed = FLAGS_seed;
-    read_percent = FLAGS_read_percent;
+    ltpmg_host::RunOptions &opt = ltpmg_host::options();
     if (benchmark == "TEST") {
         filepath = "../data/TEST.json";
         test_query_batch_size = batch_size / 2;
@@ -33,9 +31,8 @@ Param::Param()
         test_2_size = 24;
         table_cnt = 2;
         bitmap_table_cnt = 2;
-    } else if (is_oe()) {
+    } else if (ltpmg_host::is_oe(*this)) {
         filepath = "../data/TPCC.json";
-        oe_append_cap = FLAGS_append_cap;
         neworder_query_subtxn_cnt = 5 + 15 * 3;
         payment_query_subtxn_cnt = 7;
         orderstatus_query_subtxn_cnt = 3;
@@ -76,15 +73,12 @@ Param::Param()
         item_size = 100000;
         table_cnt = 9;
         bitmap_table_cnt = 2;
-    } else if (is_kv()) {
+    } else if (ltpmg_host::is_kv(*this)) {
         filepath = "";
         table_cnt = 1;
-        kv_size = FLAGS_kv_rows;
-        kv_fields = FLAGS_kv_fields;
-        kv_field_bytes = benchmark == "KV_C1" ? 4 : FLAGS_kv_field_bytes;
-        kv_ops = FLAGS_kv_ops;
-        uint32_t per_read = benchmark == "KV_C1" ? kv_fields : 1;
-        kv_sub_txn_size = batch_size * kv_ops * per_read;
+        if (benchmark == "KV_C1") opt.kv_field_bytes = 4;
+        uint32_t per_read = benchmark == "KV_C1" ? opt.kv_fields : 1;
+        opt.kv_sub_txn_size = batch_size * opt.kv_ops * per_read;
     } else {
         filepath = "../data/YCSB.json";
         ycsb_size = table_size * 1000000;
@@ -127,15 +121,17 @@ Param::~Param() {}
 void Param::print() {
     LOG(INFO) << "benchmark: " << benchmark;
     if (benchmark == "TPCC_ALL" || benchmark == "TPCC_PART") LOG(INFO) << "warehouse_size: " << table_size;
-    else if (is_kv()) LOG(INFO) << "table_size: " << kv_size << " x " << kv_fields << " x " << kv_field_bytes << " B";
+    else if (ltpmg_host::is_kv(*this))
+        LOG(INFO) << "table_size: " << ltpmg_host::options().kv_size << " x " << ltpmg_host::options().kv_fields
+                  << " x " << ltpmg_host::options().kv_field_bytes << " B";
     else LOG(INFO) << "table_size: " << ycsb_size;
     LOG(INFO) << "batch_size: " << batch_size;
     LOG(INFO) << "epoch_tp: " << epoch_tp;
     LOG(INFO) << "deviceIDs: " << deviceIDs;
     LOG(INFO) << "zipf_config: " << zipf_config;
     LOG(INFO) << "data_distribution: " << data_distribution;
-    LOG(INFO) << "cc_mode: " << (cc_mode == LTPMG_MODE_A ? "A (lowest TID wins)" : "O (ordered)");
-    LOG(INFO) << "seed: " << seed;
+    LOG(INFO) << "cc_mode: " << (ltpmg_host::options().cc_mode == LTPMG_MODE_A ? "A (lowest TID wins)" : "O (ordered)");
+    LOG(INFO) << "seed: " << ltpmg_host::options().seed;
 }
 
 void Param::analyse_deviceIDs() {
@@ -164,7 +160,7 @@ uint32_t Param::get_subtxn_kinds() {
     if (benchmark == "YCSB_C") return ycsb_c_query_subtxn_kinds;
     if (benchmark == "YCSB_D") return ycsb_d_query_subtxn_kinds;
     if (benchmark == "YCSB_E") return ycsb_e_query_subtxn_kinds;
-    if (is_kv()) return kv_ops;
+    if (ltpmg_host::is_kv(*this)) return ltpmg_host::options().kv_ops;
     return 0;
 }
 
@@ -177,7 +173,7 @@ uint32_t Param::get_sub_txn_size() {
     if (benchmark == "YCSB_C") return ycsb_c_sub_txn_size;
     if (benchmark == "YCSB_D") return ycsb_d_sub_txn_size;
     if (benchmark == "YCSB_E") return ycsb_e_sub_txn_size;
-    if (is_kv()) return kv_sub_txn_size;
+    if (ltpmg_host::is_kv(*this)) return ltpmg_host::options().kv_sub_txn_size;
     return 0;
 }
 
@@ -194,7 +190,7 @@ uint32_t Param::get_subtxn_cnt(const std::type_info &q) {
     if (q == typeid(YCSB_C_Query)) return ycsb_c_query_subtxn_cnt;
     if (q == typeid(YCSB_D_Query)) return ycsb_d_query_subtxn_cnt;
     if (q == typeid(YCSB_E_Query)) return ycsb_e_query_subtxn_cnt;
-    if (q == typeid(KV_Batch)) return kv_ops;
+    if (q == typeid(KV_Batch)) return ltpmg_host::options().kv_ops;
     return 0;
 }
 
@@ -214,3 +210,34 @@ uint32_t Param::get_txn_batch_size(const std::type_info &q) {
     if (q == typeid(KV_Batch)) return batch_size;
     return 0;
 }
+
+namespace ltpmg_host {
+
+RunOptions &options() {
+    static RunOptions o = [] {
+        RunOptions r;
+        r.cc_mode = (FLAGS_cc_mode == "A" || FLAGS_cc_mode == "a") ? LTPMG_MODE_A : LTPMG_MODE_O;
+        r.seed = FLAGS_seed;
+        r.kv_size = FLAGS_kv_rows;
+        r.kv_fields = FLAGS_kv_fields;
+        r.kv_field_bytes = FLAGS_kv_field_bytes;
+        r.kv_ops = FLAGS_kv_ops;
+        r.read_percent = FLAGS_read_percent;
+        r.oe_append_cap = FLAGS_append_cap;
+        return r;
+    }();
+    return o;
+}
+
+bool is_ycsb(const Param &p) { return p.benchmark.rfind("YCSB_", 0) == 0; }
+bool is_kv(const Param &p) { return p.benchmark == "KV" || p.benchmark == "KV_C1"; }
+bool is_oe(const Param &p) { return p.benchmark == "TPCC_PART" || p.benchmark == "TPCC_ALL" || p.benchmark == "OE"; }
+
+void export_gpu_options() {
+    const RunOptions &o = options();
+    setenv("LTPMG_CC_MODE", o.cc_mode == LTPMG_MODE_A ? "A" : "O", 1);
+    setenv("LTPMG_SEED", std::to_string(o.seed).c_str(), 1);
+    setenv("LTPMG_APPEND_CAP", std::to_string(o.oe_append_cap).c_str(), 1);
+}
+
+}  // namespace ltpmg_host

@@ -38,7 +38,7 @@ void Query::generate_txn(std::shared_ptr<Param> param) {
     else if (b == "YCSB_C") append(txn_generator_ycsb_c_query, param, param->ycsb_c_query_batch_size, transactions_batch);
     else if (b == "YCSB_D") append(txn_generator_ycsb_d_query, param, param->ycsb_d_query_batch_size, transactions_batch);
     else if (b == "YCSB_E") append(txn_generator_ycsb_e_query, param, param->ycsb_e_query_batch_size, transactions_batch);
-    else if (param->is_kv() || b == "OE") {
+    else if (ltpmg_host::is_kv(*param) || b == "OE") {
         Txn_Generator<KV_Batch> gen;
         gen.initialize(param);
         gen.generate_txn(param, param->batch_size);
@@ -49,12 +49,12 @@ void Query::generate_txn(std::shared_ptr<Param> param) {
         // interleave the kinds with a seeded Fisher-Yates shuffle (the
         // reference sorts by type-name pointer, src/query.cpp:115-116), then
         // TID = position, and each TID owns its append rows
-        ltpmg_host::Rng r(param->seed, 0x5348554646ull);
+        ltpmg_host::Rng r(ltpmg_host::options().seed, 0x5348554646ull);
         for (size_t i = transactions_batch.size(); i > 1; --i)
             std::swap(transactions_batch[i - 1], transactions_batch[r.below(i)]);
         const uint32_t start[4] = {0, 0, 0, 0};
         for (size_t t = 0; t < transactions_batch.size(); ++t) {
-            const oe::Slots sl = oe::slots_for(start, param->oe_append_cap, t);
+            const oe::Slots sl = oe::slots_for(start, ltpmg_host::options().oe_append_cap, t);
             if (auto *q = std::any_cast<Neworder_Query>(&transactions_batch[t])) {
                 q->TID = (UINT32)t;
                 oe::set_slots(*q, sl);

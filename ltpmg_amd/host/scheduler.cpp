@@ -7,6 +7,7 @@
 #include "ltpmg.h"
 #include "ltpmg_fixture.h"
 #include "ltpmg_oe.h"
+#include "pack.h"
 
 namespace {
 
@@ -17,20 +18,44 @@ void write_vec(const std::string &path, const std::vector<T> &v) {
     if (!f) LOG(FATAL) << "cannot write " << path;
 }
 
+// The engine the reference-interface entry points run on.
+ltpmg_engine *gpu_engine(const std::shared_ptr<Param> &param) {
+    const int32_t dev = param->device_IDs && param->device_cnt ? (int32_t)param->device_IDs[0] : 0;
+    ltpmg_engine *e = nullptr;
+    if (ltpmg_shared_engine(dev, &e) != LTPMG_OK) LOG(FATAL) << "no usable MI355X";
+    return e;
+}
+
+void check(ltpmg_engine *e, int rc, const char *what) {
+    if (rc != LTPMG_OK) LOG(FATAL) << what << " failed (" << rc << "): " << ltpmg_last_error(e);
+}
+
 // Epoch 0's batch as a binary fixture (include/ltpmg_fixture.h: the batch,
 // the fill seed, the mode and the table shapes) plus its outputs, so a
-// checker can replay the batch on the seeded initial image.
-void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info) {
-    Batch_Results r;
-    if (!fetch_results_from_GPU(param, &r)) LOG(FATAL) << "no results to dump";
+// checker can replay the batch on the seeded initial image.  The batch is
+// packed again from the same queries with the same rules the entry points
+// use (pack.h), and the outputs are read from the engine they ran on.
+void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info, const std::vector<std::any> &queries) {
+    const ltpmg_host::RunOptions &opt = ltpmg_host::options();
+    ltpmg_engine *e = gpu_engine(param);
+    KV_Batch batch;
+    std::string why;
+    if (!ltpmg_pack::pack_queries(queries, info[0], opt.seed, batch, &why)) LOG(FATAL) << "cannot pack " << why;
+    const uint32_t n_txns = (uint32_t)(batch.txn_off.size() - 1), n_ops = (uint32_t)batch.rows.size();
+    const uint32_t stride = ltpmg_result_stride(e);
+    std::vector<uint8_t> commit(n_txns), results((size_t)n_ops * stride);
+    check(e, ltpmg_fetch_results(e, 0, commit.data(), results.data(), results.size()), "ltpmg_fetch_results");
+    ltpmg_batch_stats st{};
+    check(e, ltpmg_stats(e, 0, &st), "ltpmg_stats");
+    if (st.n_txns != n_txns || st.n_ops != n_ops) LOG(FATAL) << "the engine ran another batch than the one packed";
     const std::string &p = FLAGS_dump_path;
     ltpmg_fixture_info fx;
     std::memset(&fx, 0, sizeof fx);
-    fx.seed = param->seed;
-    fx.mode = param->cc_mode;
+    fx.seed = opt.seed;
+    fx.mode = opt.cc_mode;
     fx.n_tables = param->table_cnt;
-    if (param->is_oe()) {
-        ltpmg_oe_schema(param->warehouse_size, param->oe_append_cap, fx.tables);
+    if (ltpmg_host::is_oe(*param)) {
+        ltpmg_oe_schema(param->warehouse_size, opt.oe_append_cap, fx.tables);
     } else {
         for (uint32_t t = 0; t < param->table_cnt; ++t) {
             const Global_Table_Info &ti = info[t];
@@ -41,15 +66,18 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info) {
             for (uint32_t c = 0; c < ti.string_size; ++c) d.col_bytes[d.n_cols++] = 4 * ti.string_length;
         }
     }
-    ltpmg_batch b{(uint32_t)(r.batch.txn_off.size() - 1), (uint32_t)r.batch.rows.size(), r.batch.txn_off.data(),
-                  r.batch.rows.data(), r.batch.meta.data(), r.batch.args.data()};
+    ltpmg_batch b{n_txns, n_ops, batch.txn_off.data(), batch.rows.data(), batch.meta.data(), batch.args.data()};
     if (ltpmg_fixture_write((p + ".batch").c_str(), &fx, &b) != LTPMG_OK) LOG(FATAL) << "cannot write " << p << ".batch";
-    write_vec(p + ".commit.u8", r.commit);
-    write_vec(p + ".results.u8", r.results);
+    write_vec(p + ".commit.u8", commit);
+    write_vec(p + ".results.u8", results);
     std::ofstream j(p + ".json");
-    j << "{\"benchmark\": \"" << param->benchmark << "\", \"result_stride\": " << r.result_stride
-      << ", \"committed\": " << param->result.committed << ", \"digests\": [";
-    for (uint32_t t = 0; t < param->table_cnt; ++t) j << (t ? ", " : "") << table_digest_on_GPU(param, t);
+    j << "{\"benchmark\": \"" << param->benchmark << "\", \"result_stride\": " << stride
+      << ", \"committed\": " << st.committed << ", \"digests\": [";
+    for (uint32_t t = 0; t < param->table_cnt; ++t) {
+        uint64_t d = 0;
+        check(e, ltpmg_table_digest(e, t, &d), "ltpmg_table_digest");
+        j << (t ? ", " : "") << d;
+    }
     j << "]}\n";
     LOG(INFO) << "dumped epoch 0 to " << p << ".*";
 }
@@ -86,13 +114,15 @@ void Scheduler::start() {
     for (uint32_t i = 0; i < param->epoch_tp; ++i) {
         query->transfer_to_GPU(param);
         this->execute(param);
-        if (i == 0 && !FLAGS_dump_path.empty()) dump(param, db->get_info());
+        if (i == 0 && !FLAGS_dump_path.empty()) dump(param, db->get_info(), query->get_batch());
+        ltpmg_engine *e = gpu_engine(param);
+        ltpmg_batch_stats st{};
+        check(e, ltpmg_stats(e, 0, &st), "ltpmg_stats");
         cost_sum += param->result.cost;
-        committed_sum += param->result.committed;
-        LOG(INFO) << "epoch " << i << ": " << param->result.committed << " committed, "
-                  << param->result.aborted << " aborted, " << param->result.cost * 1e3 << " ms, "
-                  << (param->result.cost > 0 ? param->result.committed / param->result.cost : 0.0)
-                  << " committed txn/s";
+        committed_sum += st.committed;
+        LOG(INFO) << "epoch " << i << ": " << st.committed << " committed, " << st.aborted << " aborted, "
+                  << param->result.cost * 1e3 << " ms, "
+                  << (param->result.cost > 0 ? st.committed / param->result.cost : 0.0) << " committed txn/s";
     }
     if (param->epoch_tp)
         LOG(INFO) << "TPS (committed, device time): " << (cost_sum > 0 ? committed_sum / cost_sum : 0.0);

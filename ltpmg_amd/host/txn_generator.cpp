@@ -31,7 +31,7 @@ void gen_ycsb(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, ui
             Random r;
             if (zipf) r.init_rand_zipf(rows, param->zipf_config);
             for (size_t i = w; i < size; i += n_threads) {
-                r.init_seed(param->seed ^ (type_tag << 56) ^ (uint64_t)i);
+                r.init_seed(ltpmg_host::options().seed ^ (type_tag << 56) ^ (uint64_t)i);
                 Q &q = out[i];
                 q.TID = (UINT32)i;
                 for (int j = 0; j < 10; ++j) {
@@ -54,8 +54,8 @@ void gen_ycsb(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, ui
 }  // namespace
 
 template <typename T>
-void Txn_Generator<T>::initialize(std::shared_ptr<Param> param) {
-    random.init_seed(param->seed);
+void Txn_Generator<T>::initialize(std::shared_ptr<Param>) {
+    random.init_seed(ltpmg_host::options().seed);
     transactions.clear();
 }
 
@@ -108,10 +108,10 @@ template <typename Q, typename Make>
 void gen_oe_queries(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, uint64_t tag,
                     Make make) {
     // 15 % of payments by last name, as src/txn_generator.cpp:346-351 draws them
-    const oe::Mix mix{param->warehouse_size, 5, 15, 100, param->oe_append_cap, 15};
+    const oe::Mix mix{param->warehouse_size, 5, 15, 100, ltpmg_host::options().oe_append_cap, 15};
     out.resize(size);
     for (size_t i = 0; i < size; ++i) {
-        ltpmg_host::Rng r(param->seed ^ (tag << 48), i);
+        ltpmg_host::Rng r(ltpmg_host::options().seed ^ (tag << 48), i);
         out[i] = make(r, mix);
         out[i].TID = (UINT32)i;
     }
@@ -150,10 +150,10 @@ void Txn_Generator<KV_Batch>::generate_txn(std::shared_ptr<Param> param, size_t 
         sp.lines_min = 5;
         sp.lines_max = 15;
         sp.remote_bp = 100;
-        sp.append_cap = param->oe_append_cap;
-        sp.seed = param->seed ^ 0x4F45ull;
+        sp.append_cap = ltpmg_host::options().oe_append_cap;
+        sp.seed = ltpmg_host::options().seed ^ 0x4F45ull;
         sp.byname_pct = 15;  // src/txn_generator.cpp:346-351
-        sp.table_seed = param->seed;  // the tables are filled from the run's seed
+        sp.table_seed = ltpmg_host::options().seed;  // the tables are filled from the run's seed
         KV_Batch b;
         uint64_t cap = ltpmg_gen_oe_max_ops(&sp);
         b.txn_off.resize(size + 1);
@@ -174,16 +174,16 @@ void Txn_Generator<KV_Batch>::generate_txn(std::shared_ptr<Param> param, size_t 
     const bool c1 = param->benchmark == "KV_C1";
     ltpmg_kv_spec sp{};
     sp.n_txns = (uint32_t)size;
-    sp.ops_min = sp.ops_max = param->kv_ops;
-    sp.n_rows = param->kv_size;
+    sp.ops_min = sp.ops_max = ltpmg_host::options().kv_ops;
+    sp.n_rows = ltpmg_host::options().kv_size;
     sp.zipf_theta = param->data_distribution == "zipf" ? std::min(param->zipf_config, 0.99) : 0.0;
-    sp.n_cols = param->kv_fields;
-    sp.read_pct = param->read_percent;
+    sp.n_cols = ltpmg_host::options().kv_fields;
+    sp.read_pct = ltpmg_host::options().read_percent;
     sp.write_kind = c1 ? LTPMG_OP_RMW_ADD : LTPMG_OP_UPDATE;
     sp.read_all_cols = c1 ? 1 : 0;
     sp.table_id = 0;
     sp.rmw_delta = 1;
-    sp.seed = param->seed ^ 0x4B56ull;
+    sp.seed = ltpmg_host::options().seed ^ 0x4B56ull;
     KV_Batch b;
     uint64_t cap = ltpmg_gen_kv_max_ops(&sp);
     b.txn_off.resize(size + 1);

@@ -50,6 +50,94 @@ def test_serving_and_order_entry_export_every_declared_symbol():
         assert decl == set(api)
 
 
+# The reference's GPU library is called through exactly these eight C++
+# functions (include/database.h:66-70, include/query.h:63-70,
+# include/scheduler.h:33-37); their signatures are the interface.
+REFERENCE_EXTERNS = {
+    "transfer_database_to_GPU(std::shared_ptr<Param>, Global_Table_Info*, Global_Table*, Global_Table_Index*)",
+    "transfer_database_to_CPU(std::shared_ptr<Param>, Global_Table_Info*, Global_Table*, Global_Table_Index*)",
+    "initial_query_on_GPU(std::shared_ptr<Param>, std::shared_ptr<std::vector<std::any, std::allocator<std::any> > >, Global_Txn_Info*)",
+    "transfer_query_to_GPU(std::shared_ptr<Param>, std::shared_ptr<std::vector<std::any, std::allocator<std::any> > >, Global_Txn_Info*)",
+    "transfer_query_to_CPU(std::shared_ptr<Param>, std::shared_ptr<std::vector<std::any, std::allocator<std::any> > >, Global_Txn_Info*)",
+    "initial_launcher_on_GPU(std::shared_ptr<Param>)",
+    "execute_on_GPU(std::shared_ptr<Param>)",
+    "free_launcher_on_GPU(std::shared_ptr<Param>)",
+}
+
+
+def test_gpu_library_exports_exactly_the_eight_reference_externs():
+    so = eng.LIB_DIR / "libltpmg_gpu.so"
+    out = subprocess.run(["nm", "-D", "--defined-only", "-C", str(so)], check=True,
+                         capture_output=True, text=True).stdout
+    names = {l.split(" ", 2)[2] for l in out.splitlines() if l.strip()}
+    assert names == REFERENCE_EXTERNS
+    for forbidden in ("Scheduler", "Database", "Query::", "Param::", "ltpmg_"):
+        assert not any(forbidden in n for n in names)
+
+
+def test_gpu_library_links_only_the_engine():
+    out = subprocess.run(["readelf", "-d", str(eng.LIB_DIR / "libltpmg_gpu.so")], check=True,
+                         capture_output=True, text=True).stdout
+    needed = set(re.findall(r"NEEDED.*\[(.+)\]", out))
+    ours = {n for n in needed if "ltpmg" in n}
+    assert ours == {"libltpmg_hip.so"}, needed
+
+
+# The reference's Param members in declaration order (include/param.h:13-96):
+# the object crosses the eight externs by shared_ptr, so this order, the
+# member types and Result (include/param.h:7-11) as the last member fix its
+# layout.
+PARAM_FIELDS = """benchmark table_size batch_size epoch_tp epoch_sync deviceIDs neworder_percent filepath
+table_cnt device_cnt device_IDs data_distribution zipf_config test_1_size test_2_size warehouse_size
+district_size customer_size neworder_size history_size order_size orderline_size stock_size item_size
+ycsb_size bitmap_row_cnt bitmap_table_cnt test_query_batch_size test_query_2_batch_size
+neworder_query_batch_size payment_query_batch_size orderstatus_query_batch_size
+delivery_query_batch_size stocklevel_query_batch_size ycsb_a_query_batch_size ycsb_b_query_batch_size
+ycsb_c_query_batch_size ycsb_d_query_batch_size ycsb_e_query_batch_size test_query_subtxn_cnt
+test_query_2_subtxn_cnt neworder_query_subtxn_cnt payment_query_subtxn_cnt orderstatus_query_subtxn_cnt
+delivery_query_subtxn_cnt stocklevel_query_subtxn_cnt ycsb_a_query_subtxn_cnt ycsb_b_query_subtxn_cnt
+ycsb_c_query_subtxn_cnt ycsb_d_query_subtxn_cnt ycsb_e_query_subtxn_cnt test_sub_txn_size
+tpcc_all_sub_txn_size tpcc_part_sub_txn_size ycsb_a_sub_txn_size ycsb_b_sub_txn_size ycsb_c_sub_txn_size
+ycsb_d_sub_txn_size ycsb_e_sub_txn_size test_query_subtxn_kinds tpcc_all_query_subtxn_kinds
+tpcc_part_query_subtxn_kinds ycsb_a_query_subtxn_kinds ycsb_b_query_subtxn_kinds ycsb_c_query_subtxn_kinds
+ycsb_d_query_subtxn_kinds ycsb_e_query_subtxn_kinds select_batch_size insert_batch_size update_batch_size
+scan_batch_size delete_batch_size result""".split()
+PARAM_TYPES = {"benchmark": "std::string", "deviceIDs": "std::string", "filepath": "std::string",
+               "data_distribution": "std::string", "zipf_config": "double", "device_IDs": "uint32_t *",
+               "result": "Result"}
+
+
+def test_param_has_the_reference_layout(tmp_path):
+    # a compiled probe: member types, declaration order and packing as the
+    # reference declares them, Result = {double cost; uint32_t epoch, batch_size}
+    # every member right after its predecessor (at the next offset its
+    # alignment allows), so nothing can be inserted or reordered unnoticed
+    checks = ["static_assert(offsetof(Param, benchmark) == 0, \"benchmark first\");"]
+    for a, b in zip(PARAM_FIELDS, PARAM_FIELDS[1:]):
+        end = f"(offsetof(Param, {a}) + sizeof(Param::{a}))"
+        al = f"alignof(decltype(Param::{b}))"
+        checks.append(f"static_assert(offsetof(Param, {b}) == ({end} + {al} - 1) / {al} * {al}, "
+                      f"\"{b} right after {a}\");")
+    for f in PARAM_FIELDS:
+        t = PARAM_TYPES.get(f, "uint32_t")
+        checks.append(f"static_assert(std::is_same<decltype(Param::{f}), {t}>::value, \"type of {f}\");")
+    src = tmp_path / "probe.cpp"
+    src.write_text("""
+#include <cstddef>
+#include <type_traits>
+#include "param.h"
+static_assert(sizeof(Result) == 16 && offsetof(Result, cost) == 0 && offsetof(Result, epoch) == 8 &&
+              offsetof(Result, batch_size) == 12, "Result is the reference's {double, uint32_t, uint32_t}");
+static_assert(std::is_same<decltype(Result::cost), double>::value, "Result::cost");
+static_assert(offsetof(Param, result) + sizeof(Result) == sizeof(Param), "result is the last member");
+static_assert(!std::is_polymorphic<Param>::value, "no vtable");
+""" + "\n".join(checks) + "\nint main() { return 0; }\n")
+    proc = subprocess.run(["g++", "-std=c++17", "-Wno-invalid-offsetof", "-fsyntax-only",
+                           f"-I{ROOT / 'include'}", f"-I{ROOT / 'include' / 'ltpmg_host'}", str(src)],
+                          capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr[-3000:]
+
+
 def test_concat_batches_keeps_each_batch():
     a = eng.Batch.from_txns([[(0, 0, 1, 5, 0)], [(1, 0, 2, 6, 9), (0, 0, 0, 7, 0)]])
     b = eng.Batch.from_txns([[(2, 1, 3, 8, 4)]])
