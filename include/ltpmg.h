@@ -29,6 +29,24 @@
  *   RMW_ADD   adds (uint32_t)arg to every little-endian u32 word of the cell.
  * Within one txn a cell that is written must not be accessed by another op.
  *
+ * Append tables (ltpmg_create_append_table) hold rows [0, cursor), each
+ * present or deleted, up to a capacity; two row-level op kinds act on them
+ * (kind LTPMG_OP_ROW, the column field names which):
+ *   INSERT    appends a row at the end of the batch: the committed INSERTs
+ *             into a table take rows cursor, cursor+1, ... in op (TID) order,
+ *             so aborted txns leave no holes; every column c gets the bytes of
+ *             stream(rowseed(arg, c), 0..), rowseed(arg, c) =
+ *             mix64(arg + (c+1) * 0xD1B54A32D192ED03), and the op's result
+ *             slot the row id (u32, rest zero).  The row field must be 0.
+ *   DELETE    marks row `row` deleted.
+ * An op on a row that is not present (deleted, or not yet inserted when the
+ * batch started: rows an INSERT appends become visible to the next batch)
+ * has no effect and its status is 0; a READ of one returns zeros.  A txn
+ * that deletes a row does not access that row with another op.  INSERT and
+ * DELETE are only valid on append tables; every row of an ordinary table is
+ * always present.  In the lowest-TID-wins mode a DELETE reserves its row like
+ * a write; an INSERT touches no existing row and reserves nothing.
+ *
  * Modes (both equal a serial replay in ascending TID of the committed txns):
  *   LTPMG_MODE_A  lowest TID wins: W[row] = min TID writing the row; a txn
  *                 commits iff no lower TID writes any row it touches (the
@@ -59,12 +77,18 @@ extern "C" {
 #define LTPMG_EHIP (-4)     /* HIP runtime error */
 #define LTPMG_ESTATE (-5)   /* call out of order (no table / no batch) */
 #define LTPMG_EDEVICE (-6)  /* no usable GPU */
+#define LTPMG_EFULL (-7)    /* the INSERTs of a batch may not fit an append table's capacity */
 
 #define LTPMG_OP_READ 0u
 #define LTPMG_OP_UPDATE 1u
 #define LTPMG_OP_RMW_ADD 2u
+#define LTPMG_OP_ROW 3u    /* row-level op on an append table; the column field says which: */
+#define LTPMG_ROW_INSERT 0u
+#define LTPMG_ROW_DELETE 1u
 #define LTPMG_META(kind, table, col) \
     ((uint16_t)((kind) | ((uint32_t)(col) << 2) | ((uint32_t)(table) << 8)))
+#define LTPMG_META_INSERT(table) LTPMG_META(LTPMG_OP_ROW, table, LTPMG_ROW_INSERT)
+#define LTPMG_META_DELETE(table) LTPMG_META(LTPMG_OP_ROW, table, LTPMG_ROW_DELETE)
 
 #define LTPMG_MODE_A 0u
 #define LTPMG_MODE_O 1u
@@ -111,6 +135,7 @@ typedef struct {
     char kernel_name[LTPMG_MAX_KSTATS][32];
     float kernel_ms[LTPMG_MAX_KSTATS]; /* with LTPMG_CFG_PROFILE or LTPMG_EXEC_PROFILE */
     uint64_t hbm_allocated;
+    uint32_t inserted;  /* rows the batch appended (committed INSERTs) */
 } ltpmg_batch_stats;
 
 /* Engine lifecycle.  Replaces GPUScheduler's global singleton
@@ -138,6 +163,18 @@ int ltpmg_load_column(ltpmg_engine *e, uint32_t table_id, uint32_t col, uint32_t
                       uint32_t n_rows, const void *src); /* packed, col_bytes per row */
 int ltpmg_read_column(ltpmg_engine *e, uint32_t table_id, uint32_t col, uint32_t row0,
                       uint32_t n_rows, void *dst);
+/* An append table (see INSERT / DELETE above): capacity desc->n_rows rows,
+ * of which rows [0, n_present) exist at creation (filled by
+ * ltpmg_fill_table / ltpmg_load_column like any table).  Replaces the
+ * reference's fixed-size insert targets (insert_executor at host-chosen rows,
+ * cuda/src/gpulauncher.cu:205-260, 52-79) for append-only tables. */
+int ltpmg_create_append_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *desc,
+                              uint32_t n_present);
+/* The rows an append table has allocated so far (its cursor); n_rows of an
+ * ordinary table. */
+int ltpmg_table_rows(ltpmg_engine *e, uint32_t table_id, uint32_t *out);
+/* Presence bytes (1 present, 0 deleted or unallocated) of rows [row0, row0 + n_rows). */
+int ltpmg_read_presence(ltpmg_engine *e, uint32_t table_id, uint32_t row0, uint32_t n_rows, uint8_t *dst);
 /* Order-independent digest of a table: sum over cells of a hash of
  * (table, row, col, cell bytes).  Replaces nothing in the reference (it has
  * no readback, gpudatabase.cu:240-339 frees without copying back). */
@@ -185,6 +222,11 @@ int ltpmg_wait(ltpmg_engine *e, uint32_t slot);
 int ltpmg_fetch_results(ltpmg_engine *e, uint32_t slot, uint8_t *commit, void *results,
                         uint64_t results_bytes);
 int ltpmg_stats(ltpmg_engine *e, uint32_t slot, ltpmg_batch_stats *out);
+/* Per op of the slot's last batch (n_ops bytes): 1 if the op took effect --
+ * its txn committed and its row was present (an INSERT: its row was
+ * appended) -- else 0.  No counterpart in the reference (its operators return
+ * nothing). */
+int ltpmg_fetch_status(ltpmg_engine *e, uint32_t slot, uint8_t *status);
 
 /* Building-block self-tests, the counterparts of the reference's in-binary
  * LTPMG_GPULAUNCHER_TEST_MERGESORT and _PREFIXSUM checks
@@ -222,7 +264,8 @@ int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t iters, doubl
  * so op fields are checked by the kernels (invalid ops are skipped), and a
  * malformed txn_off only leaves the affected commit flags undefined (every
  * access stays inside the batch and the tables); the batch must otherwise
- * follow the rules of ltpmg_submit_batch. */
+ * follow the rules of ltpmg_submit_batch.  An engine with an append table
+ * is refused (LTPMG_ESTATE): row ops need ltpmg_execute. */
 #define LTPMG_STATIC_MAX_OPS (1u << 18)
 int ltpmg_execute_static(ltpmg_engine *e, uint32_t mode, const ltpmg_batch *dev, uint8_t *dev_commit,
                          void *dev_results, void *hip_stream);

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", f"-I{INCLUDE}",
 ]
-HIP_SOURCES = ["engine.hip", "kernels_table.hip", "kernels_cc.hip", "kernels_order.hip"]
+HIP_SOURCES = ["engine.hip", "kernels_table.hip", "kernels_cc.hip", "kernels_order.hip", "kernels_append.hip"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-pthread", f"-I{INCLUDE}",
              f"-I{INCLUDE / 'ltpmg_host'}"]

@@ -54,6 +54,14 @@ struct Slot {
     uint64_t *d_args = nullptr;
     uint8_t *d_commit = nullptr, *d_results = nullptr;
     OrderBufs ob{};
+    // row-op buffers (allocated on the first batch that touches an append
+    // table, for cap_ops ops) and what the current batch does with rows
+    uint8_t *d_found = nullptr;
+    uint16_t *d_meta_eff = nullptr;
+    uint32_t *d_ins_cnt = nullptr, *d_ins_base = nullptr;
+    bool has_row_bufs = false;
+    RowBufs rb{};
+    uint32_t inserts[LTPMG_MAX_TABLES] = {};  // INSERT ops of the batch per table
     uint8_t *h_stage = nullptr;
     size_t stage_bytes = 0;
     uint32_t n_txns = 0, n_ops = 0, mode = 0;
@@ -71,6 +79,11 @@ struct Table {
     uint32_t width[LTPMG_MAX_COLS] = {}, slot[LTPMG_MAX_COLS] = {};
     uint8_t *col[LTPMG_MAX_COLS] = {};
     unsigned long long *lock = nullptr;
+    // append tables: presence bytes, and an upper bound on the device cursor
+    // (the cursor itself is d_cursor[table]: aborted INSERTs take no row, so
+    // the host only knows a bound without reading it back)
+    uint8_t *present = nullptr;
+    uint64_t cursor_bound = 0;
 };
 
 }  // namespace
@@ -92,6 +105,8 @@ struct ltpmg_engine {
     TabDesc *d_tabs = nullptr;
     unsigned long long *d_digest = nullptr;
     uint8_t *static_scratch = nullptr;  // this device's copy of `scratch` (ltpmg_execute_static)
+    uint32_t *d_cursor = nullptr;       // [LTPMG_MAX_TABLES] append cursors
+    uint32_t n_append = 0;              // live append tables
     std::vector<Slot> slots;
     std::string err;
     std::mutex mu;
@@ -250,8 +265,8 @@ int sync_descriptors(ltpmg_engine *e) {
         const Table &tb = e->tables[t];
         if (!tb.live) continue;
         const uint32_t stride = key_col_stride(tb, pad);
-        tabs[t] = TabDesc{tb.lock, tb.n_rows, tb.n_cols, (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u,
-                          stride};
+        tabs[t] = TabDesc{tb.lock, tb.present, tb.n_rows, tb.n_cols,
+                          (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu), 1u, stride};
         cell_base += (uint64_t)tb.n_rows * stride;
         kd.tab[kd.n] = t;
         kd.end[kd.n] = (uint32_t)std::min<uint64_t>(cell_base, 0xFFFFFFFFu);
@@ -308,6 +323,99 @@ int alloc_order_bufs(ltpmg_engine *e, OrderBufs &ob, size_t ops) {
     return LTPMG_OK;
 }
 
+// Row-op buffers for batches of up to ops ops (found, meta_eff, insert
+// counts per tile and table, the cursor at the batch start per table).
+int alloc_row_bufs(ltpmg_engine *e, size_t ops, uint8_t *&found, uint16_t *&meta_eff, uint32_t *&ins_cnt,
+                   uint32_t *&ins_base) {
+    int rc = dev_alloc_n(e, &found, std::max<size_t>(ops, 1));
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &meta_eff, std::max<size_t>(ops, 1));
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &ins_cnt, (size_t)std::max(ins_tiles((uint32_t)ops), 1u) * LTPMG_MAX_TABLES);
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &ins_base, LTPMG_MAX_TABLES);
+    return rc;
+}
+void free_row_bufs(ltpmg_engine *e, size_t ops, uint8_t *&found, uint16_t *&meta_eff, uint32_t *&ins_cnt,
+                   uint32_t *&ins_base) {
+    dev_free(e, found, std::max<size_t>(ops, 1));
+    dev_free(e, meta_eff, std::max<size_t>(ops, 1) * 2);
+    dev_free(e, ins_cnt, (size_t)std::max(ins_tiles((uint32_t)ops), 1u) * LTPMG_MAX_TABLES * 4);
+    dev_free(e, ins_base, LTPMG_MAX_TABLES * 4);
+    found = nullptr;
+    meta_eff = nullptr;
+    ins_cnt = ins_base = nullptr;
+}
+
+// What a batch does with rows: any op on an append table, DELETEs, and the
+// INSERTs per table (a pass over meta, split over host threads for large
+// batches; nothing to do while no append table exists).
+void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint32_t inserts[LTPMG_MAX_TABLES]) {
+    rb = RowBufs{};
+    std::fill(inserts, inserts + LTPMG_MAX_TABLES, 0u);
+    if (e->n_append == 0 || b->n_ops == 0) return;
+    uint32_t append_mask = 0;
+    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
+        if (e->tables[t].live && e->tables[t].present) append_mask |= 1u << t;
+    const uint32_t n = b->n_ops;
+    const uint32_t n_threads = n < (1u << 20) ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    struct Acc {
+        uint32_t ins[LTPMG_MAX_TABLES] = {};
+        bool app = false, del = false;
+    };
+    std::vector<Acc> acc(n_threads);
+    auto work = [&](uint32_t w) {
+        Acc &a = acc[w];
+        const uint32_t lo = (uint32_t)((uint64_t)n * w / n_threads), hi = (uint32_t)((uint64_t)n * (w + 1) / n_threads);
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint16_t m = b->meta[i];
+            const uint32_t tb = (m >> 8) & 15u;
+            if (!((append_mask >> tb) & 1u)) continue;
+            a.app = true;
+            if ((m & 3u) == LTPMG_OP_ROW) {
+                if (((m >> 2) & 63u) == LTPMG_ROW_INSERT) ++a.ins[tb];
+                else a.del = true;
+            }
+        }
+    };
+    if (n_threads == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (uint32_t w = 0; w < n_threads; ++w) th.emplace_back(work, w);
+        for (auto &t : th) t.join();
+    }
+    for (const Acc &a : acc) {
+        rb.has_append |= a.app;
+        rb.has_delete |= a.del;
+        for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) inserts[t] += a.ins[t];
+    }
+    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
+        if (inserts[t]) rb.insert_tables |= 1u << t;
+    rb.has_insert = rb.insert_tables != 0;
+}
+
+// The INSERTs of a batch must fit: each append table's bound on its cursor
+// plus the batch's INSERTs stays within its capacity.  When the bound says
+// no, the exact cursor is read back (after every executed batch) and
+// checked again.  Then the bounds move on by the batch's INSERTs.
+int check_capacity(ltpmg_engine *e, const uint32_t inserts[LTPMG_MAX_TABLES]) {
+    bool synced = false;
+    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
+        Table &tb = e->tables[t];
+        if (!inserts[t] || tb.cursor_bound + inserts[t] <= tb.n_rows) continue;
+        if (!synced) {
+            for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+            synced = true;
+        }
+        uint32_t cur = 0;
+        HIP_TRY(e, hipMemcpy(&cur, e->d_cursor + t, 4, hipMemcpyDeviceToHost));
+        tb.cursor_bound = cur;
+        if ((uint64_t)cur + inserts[t] > tb.n_rows)
+            return fail(e, LTPMG_EFULL, "table %u: the batch inserts up to %u rows, %u of %u rows are left", t,
+                        inserts[t], tb.n_rows - cur, tb.n_rows);
+    }
+    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) e->tables[t].cursor_bound += inserts[t];
+    return LTPMG_OK;
+}
+
 void free_slot_buffers(ltpmg_engine *e, Slot &s) {
     const size_t ops = s.cap_ops, txns = s.cap_txns;
     dev_free(e, s.d_txn_off, (txns + 1) * 4);
@@ -319,6 +427,8 @@ void free_slot_buffers(ltpmg_engine *e, Slot &s) {
     dev_free(e, s.d_tile_t0, (ops / kTileOps + 2) * 4);
     dev_free(e, s.d_counters, 16);
     free_order_bufs(e, s.ob, ops);
+    if (s.has_row_bufs) free_row_bufs(e, ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+    s.has_row_bufs = false;
     if (s.h_stage) (void)hipHostFree(s.h_stage);
     s.d_txn_off = s.d_rows = s.d_tile_t0 = s.d_counters = nullptr;
     s.d_meta = nullptr;
@@ -417,7 +527,11 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ul
         uint16_t m = b->meta[i];
         uint32_t kind = m & 3u, col = (m >> 2) & 63u;
         const Table &t = e->tables[(m >> 8) & 15u];
-        return (m >> 12) || kind > LTPMG_OP_RMW_ADD || !t.live || col >= t.n_cols || b->rows[i] >= t.n_rows;
+        if ((m >> 12) || !t.live) return true;
+        if (kind == LTPMG_OP_ROW)
+            return !t.present || (col == LTPMG_ROW_DELETE ? b->rows[i] >= t.n_rows
+                                                          : col != LTPMG_ROW_INSERT || b->rows[i] != 0);
+        return col >= t.n_cols || b->rows[i] >= t.n_rows;
     };
     uint32_t i = first_bad_parallel(b->n_ops, [&](uint32_t lo, uint32_t hi) {
         for (uint32_t k = lo; k < hi; ++k)
@@ -428,35 +542,60 @@ int validate_batch(ltpmg_engine *e, const ltpmg_batch *b, uint64_t max_ops = 1ul
         uint16_t m = b->meta[i];
         uint32_t kind = m & 3u, col = (m >> 2) & 63u, tab = (m >> 8) & 15u;
         const Table &t = e->tables[tab];
-        if ((m >> 12) || kind > LTPMG_OP_RMW_ADD) return fail(e, LTPMG_EINVAL, "op %u: bad meta 0x%04x", i, m);
+        if (m >> 12) return fail(e, LTPMG_EINVAL, "op %u: bad meta 0x%04x", i, m);
         if (!t.live) return fail(e, LTPMG_EINVAL, "op %u: table %u does not exist", i, tab);
+        if (kind == LTPMG_OP_ROW) {
+            if (!t.present) return fail(e, LTPMG_EINVAL, "op %u: INSERT / DELETE on table %u, not an append table", i, tab);
+            if (col == LTPMG_ROW_INSERT) return fail(e, LTPMG_EINVAL, "op %u: an INSERT names row 0, not %u", i, b->rows[i]);
+            if (col != LTPMG_ROW_DELETE) return fail(e, LTPMG_EINVAL, "op %u: bad row op %u", i, col);
+            return fail(e, LTPMG_EINVAL, "op %u: DELETE of row %u >= %u rows of table %u", i, b->rows[i], t.n_rows, tab);
+        }
         if (col >= t.n_cols) return fail(e, LTPMG_EINVAL, "op %u: column %u >= %u", i, col, t.n_cols);
         return fail(e, LTPMG_EINVAL, "op %u: row %u >= %u rows of table %u", i, b->rows[i], t.n_rows, tab);
     }
     // Within a txn, a cell that an op writes is not accessed by any later op
-    // of the txn (a READ before the write is fine).  The kernels rely on it:
-    // lowest-TID-wins reads see the batch-start snapshot and applies a txn's
-    // writes to one cell in parallel.  Only pairs whose first op writes are
-    // compared, so a 16-op txn with 8 writes costs at most 8 x 15 compares.
+    // of the txn (a READ before the write is fine), and a row the txn
+    // DELETEs is not accessed by any other op of the txn.  The kernels rely
+    // on it: lowest-TID-wins reads see the batch-start snapshot and apply a
+    // txn's writes to one cell in parallel.  Only pairs whose first op writes
+    // (or either op deletes) are compared, so a 16-op txn with 8 writes costs
+    // at most 8 x 15 compares.
     const uint32_t *off = b->txn_off;
+    auto is_del = [&](uint32_t p) { return (b->meta[p] & 3u) == LTPMG_OP_ROW && ((b->meta[p] >> 2) & 63u) == LTPMG_ROW_DELETE; };
+    auto is_row = [&](uint32_t p) { return (b->meta[p] & 3u) == LTPMG_OP_ROW; };
+    // the first op (index) after p in txn [lo, hi) that conflicts with p, or hi
+    auto clash = [&](uint32_t p, uint32_t lo, uint32_t hi) -> uint32_t {
+        const uint32_t tab = (b->meta[p] >> 8) & 15u;
+        if (is_del(p)) {  // every other op of the txn on that row
+            for (uint32_t q = lo; q < hi; ++q)
+                if (q != p && !(is_row(q) && !is_del(q)) && b->rows[q] == b->rows[p] && ((b->meta[q] >> 8) & 15u) == tab)
+                    return q;
+            return hi;
+        }
+        if ((b->meta[p] & 3u) == LTPMG_OP_READ || is_row(p)) return hi;
+        const uint16_t cell = b->meta[p] & (uint16_t)~3u;
+        for (uint32_t q = p + 1; q < hi; ++q)
+            if (!is_row(q) && b->rows[q] == b->rows[p] && (b->meta[q] & (uint16_t)~3u) == cell) return q;
+        return hi;
+    };
     uint32_t bad_t = first_bad_parallel(b->n_txns, [&](uint32_t lo, uint32_t hi) {
         for (uint32_t t = lo; t < hi; ++t)
-            for (uint32_t p = off[t]; p + 1 < off[t + 1]; ++p) {
-                if ((b->meta[p] & 3u) == LTPMG_OP_READ) continue;
-                const uint16_t cell = b->meta[p] & (uint16_t)~3u;
-                for (uint32_t q = p + 1; q < off[t + 1]; ++q)
-                    if (b->rows[q] == b->rows[p] && (b->meta[q] & (uint16_t)~3u) == cell) return t;
-            }
+            for (uint32_t p = off[t]; p < off[t + 1]; ++p)
+                if (clash(p, off[t], off[t + 1]) != off[t + 1]) return t;
         return UINT32_MAX;
     });
     if (bad_t != UINT32_MAX) {
-        for (uint32_t p = off[bad_t]; p + 1 < off[bad_t + 1]; ++p)
-            for (uint32_t q = p + 1; (b->meta[p] & 3u) != LTPMG_OP_READ && q < off[bad_t + 1]; ++q)
-                if (b->rows[q] == b->rows[p] && (b->meta[q] & (uint16_t)~3u) == (b->meta[p] & (uint16_t)~3u))
-                    return fail(e, LTPMG_EINVAL,
-                                "txn %u: op %u accesses the cell (table %u, row %u, column %u) that op %u of "
-                                "the same txn writes",
-                                bad_t, q, (b->meta[p] >> 8) & 15u, b->rows[p], (b->meta[p] >> 2) & 63u, p);
+        for (uint32_t p = off[bad_t]; p < off[bad_t + 1]; ++p) {
+            const uint32_t q = clash(p, off[bad_t], off[bad_t + 1]);
+            if (q == off[bad_t + 1]) continue;
+            if (is_del(p))
+                return fail(e, LTPMG_EINVAL, "txn %u: op %u accesses row %u of table %u, which op %u of the same txn "
+                            "deletes", bad_t, q, b->rows[p], (b->meta[p] >> 8) & 15u, p);
+            return fail(e, LTPMG_EINVAL,
+                        "txn %u: op %u accesses the cell (table %u, row %u, column %u) that op %u of "
+                        "the same txn writes",
+                        bad_t, q, (b->meta[p] >> 8) & 15u, b->rows[p], (b->meta[p] >> 2) & 63u, p);
+        }
     }
     return LTPMG_OK;
 }
@@ -490,14 +629,17 @@ struct StageRec {
 
 // Lowest-TID-wins on the ops of ov with the three-kernel pipeline.
 // wait_tables() runs before the first access to lock words or cells.
+// rb: the batch's row-op buffers, or nullptr when it touches no append
+// table (then the kernels never look at presence).
 template <class Rec, class Wait>
 int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *tile_t0,
                   uint32_t *counters, uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec,
-                  Wait wait_tables) {
+                  Wait wait_tables, RowBufs *rb = nullptr) {
     const uint32_t n_tiles = std::max(1u, (ov.n_ops + kTileOps - 1) / kTileOps);
     int rc = wait_tables();
     if (rc != LTPMG_OK) return rc;
     DevTables dt = dev_tables(e);
+    uint8_t *found = rb ? rb->found : nullptr;
     rec.begin("tile_bounds");
     launch_zero_u32(st, counters, 4);
     launch_tile_bounds(st, ov.txn_off, ov.n_txns, n_tiles, tile_t0, e->d_sync);
@@ -506,11 +648,19 @@ int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *t
     launch_cc_reserve(st, dt, ov, tile_t0, n_tiles, e->d_sync);
     rec.end();
     rec.begin("cc_validate_read");
-    launch_cc_validate_read(st, dt, ov, tile_t0, n_tiles, e->d_sync, commit, results, rs, counters);
+    launch_cc_validate_read(st, dt, ov, tile_t0, n_tiles, e->d_sync, commit, results, rs, counters, found);
     rec.end();
     rec.begin("cc_write");
-    launch_cc_write(st, dt, ov, tile_t0, n_tiles, commit, rs);
+    launch_cc_write(st, dt, ov, tile_t0, n_tiles, commit, rs, found);
     rec.end();
+    if (rb) {
+        rec.begin("row_deletes");
+        launch_row_deletes(st, dt, ov, *rb, commit);
+        rec.end();
+        rec.begin("inserts");
+        launch_inserts(st, dt, ov, *rb, commit, results, rs);
+        rec.end();
+    }
     return LTPMG_OK;
 }
 
@@ -533,13 +683,27 @@ int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
 // Ordered mode on the ops of ov (results indexed by op within ov).  Keys,
 // sort and scan touch only the pipeline's own buffers; wait_tables() runs
 // before the first cell access.
+// rb as for launch_mode_a.  A batch that touches append tables reads the
+// presence bytes before its sort, so it waits for the previous batch first
+// (then its sort does not overlap that batch), and sorts by a private copy of
+// the meta words in which ops on absent rows are invalid.
 template <class Rec, class Wait>
-int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &ob,
-                  uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables) {
+int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBufs &ob,
+                  uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables,
+                  RowBufs *rb = nullptr) {
     uint32_t key_bits = 0;
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
     DevTables dt = dev_tables(e);
+    OpView ov = ov_in;
+    if (rb) {
+        rc = wait_tables();
+        if (rc != LTPMG_OK) return rc;
+        rec.begin("row_presence");
+        launch_row_presence_o(st, dt, ov_in, *rb, e->d_sync, results, rs);
+        rec.end();
+        ov.meta = rb->meta_eff;
+    }
     int sorted = 0;
     if (ov.n_ops > 1 && key_bits > 0) {
         // digit 0 straight from the ops: the unsorted keys are never stored
@@ -574,8 +738,10 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     rec.begin("mo_seg_scan");
     launch_mo_seg_scan(st, ov, ob, sorted);
     rec.end();
-    rc = wait_tables();
-    if (rc != LTPMG_OK) return rc;
+    if (!rb) {
+        rc = wait_tables();
+        if (rc != LTPMG_OK) return rc;
+    }
     rec.begin("mo_warm_seeds");
     launch_mo_warm_seeds(st, ov, ob);
     rec.end();
@@ -585,6 +751,14 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov, OrderBufs &
     rec.begin("mo_final_writes");
     launch_mo_final_writes(st, dt, ov, ob, rs, commit);
     rec.end();
+    if (rb) {
+        rec.begin("row_deletes");
+        launch_row_deletes(st, dt, ov_in, *rb, commit);
+        rec.end();
+        rec.begin("inserts");
+        launch_inserts(st, dt, ov_in, *rb, commit, results, rs);
+        rec.end();
+    }
     return LTPMG_OK;
 }
 
@@ -622,6 +796,8 @@ int ltpmg_open(const ltpmg_cfg *cfg, ltpmg_engine **out) {
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_digest, 1);
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_sync, 4);
     if (rc == LTPMG_OK && hipMemset(e->d_sync, 0, 16) != hipSuccess) rc = LTPMG_EHIP;
+    if (rc == LTPMG_OK) rc = dev_alloc_n(e, &e->d_cursor, LTPMG_MAX_TABLES);
+    if (rc == LTPMG_OK && hipMemset(e->d_cursor, 0, 4 * LTPMG_MAX_TABLES) != hipSuccess) rc = LTPMG_EHIP;
     if (rc == LTPMG_OK &&
         hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess)
         rc = LTPMG_EHIP;
@@ -673,7 +849,9 @@ int ltpmg_close(ltpmg_engine *e) {
     for (Table &t : e->tables) {
         for (uint32_t c = 0; c < t.n_cols; ++c) (void)hipFree(t.col[c]);
         if (t.lock) (void)hipFree(t.lock);
+        if (t.present) (void)hipFree(t.present);
     }
+    if (e->d_cursor) (void)hipFree(e->d_cursor);
     if (e->d_cols) (void)hipFree(e->d_cols);
     if (e->d_tabs) (void)hipFree(e->d_tabs);
     if (e->d_digest) (void)hipFree(e->d_digest);
@@ -711,9 +889,13 @@ const char *ltpmg_last_error(const ltpmg_engine *e) { return e ? e->err.c_str() 
 
 uint32_t ltpmg_result_stride(const ltpmg_engine *e) { return e ? max_slot_all(e) : 0; }
 
-int ltpmg_create_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *d) {
-    if (!e || !d) return LTPMG_EINVAL;
-    std::lock_guard<std::mutex> g(e->mu);
+}  // extern "C"
+
+namespace {
+// Creates table table_id; an append table (append set) also gets presence
+// bytes, 1 for rows [0, n_present), and its cursor at n_present.
+int create_table_locked(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *d, bool append,
+                        uint32_t n_present) {
     HIP_TRY(e, hipSetDevice(e->device));
     if (table_id >= LTPMG_MAX_TABLES) return fail(e, LTPMG_EINVAL, "table id %u >= %d", table_id, LTPMG_MAX_TABLES);
     if (e->tables[table_id].live) return fail(e, LTPMG_ESTATE, "table %u already exists", table_id);
@@ -738,14 +920,76 @@ int ltpmg_create_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_des
     if (rc == LTPMG_OK) rc = dev_alloc_n(e, &t.lock, t.n_rows);
     if (rc == LTPMG_OK && hipMemset(t.lock, 0, sizeof(unsigned long long) * t.n_rows) != hipSuccess)
         rc = fail(e, LTPMG_EHIP, "hipMemset of lock words failed");
+    if (append) {
+        if (rc == LTPMG_OK) rc = dev_alloc_n(e, &t.present, t.n_rows);
+        if (rc == LTPMG_OK && (hipMemset(t.present, 0, t.n_rows) != hipSuccess ||
+                               (n_present && hipMemset(t.present, 1, n_present) != hipSuccess) ||
+                               hipMemcpy(e->d_cursor + table_id, &n_present, 4, hipMemcpyHostToDevice) != hipSuccess))
+            rc = fail(e, LTPMG_EHIP, "initialising the presence bytes failed");
+        t.cursor_bound = n_present;
+    }
     if (rc != LTPMG_OK) {
         for (uint32_t c = 0; c < t.n_cols; ++c) dev_free(e, t.col[c], (size_t)t.slot[c] * t.n_rows);
         dev_free(e, t.lock, 8ull * t.n_rows);
+        dev_free(e, t.present, t.n_rows);
         return rc;
     }
     t.live = true;
     e->tables[table_id] = t;
+    if (append) ++e->n_append;
     return sync_descriptors(e);
+}
+}  // namespace
+
+extern "C" {
+
+int ltpmg_create_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *d) {
+    if (!e || !d) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    return create_table_locked(e, table_id, d, false, 0);
+}
+
+int ltpmg_create_append_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_table_desc *d, uint32_t n_present) {
+    if (!e || !d) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    if (n_present > d->n_rows) return fail(e, LTPMG_EINVAL, "%u present rows > capacity %u", n_present, d->n_rows);
+    return create_table_locked(e, table_id, d, true, n_present);
+}
+
+int ltpmg_table_rows(ltpmg_engine *e, uint32_t table_id, uint32_t *out) {
+    if (!e || !out) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (table_id >= LTPMG_MAX_TABLES || !e->tables[table_id].live)
+        return fail(e, LTPMG_ESTATE, "table %u does not exist", table_id);
+    Table &t = e->tables[table_id];
+    if (!t.present) {
+        *out = t.n_rows;
+        return LTPMG_OK;
+    }
+    for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+    HIP_TRY(e, hipMemcpy(out, e->d_cursor + table_id, 4, hipMemcpyDeviceToHost));
+    t.cursor_bound = *out;
+    return LTPMG_OK;
+}
+
+int ltpmg_read_presence(ltpmg_engine *e, uint32_t table_id, uint32_t row0, uint32_t n_rows, uint8_t *dst) {
+    if (!e || (!dst && n_rows)) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (table_id >= LTPMG_MAX_TABLES || !e->tables[table_id].live)
+        return fail(e, LTPMG_ESTATE, "table %u does not exist", table_id);
+    const Table &t = e->tables[table_id];
+    if ((uint64_t)row0 + n_rows > t.n_rows)
+        return fail(e, LTPMG_EINVAL, "rows [%u, %llu) outside the table", row0, (unsigned long long)row0 + n_rows);
+    if (!n_rows) return LTPMG_OK;
+    if (!t.present) {
+        std::memset(dst, 1, n_rows);
+        return LTPMG_OK;
+    }
+    for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+    HIP_TRY(e, hipMemcpy(dst, t.present + row0, n_rows, hipMemcpyDeviceToHost));
+    return LTPMG_OK;
 }
 
 int ltpmg_fill_table(ltpmg_engine *e, uint32_t table_id, uint64_t seed) {
@@ -866,6 +1110,26 @@ static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
         int rc = trusted ? validate_structure(e, &b, 1ull << 29) : validate_batch(e, &b);
         if (rc != LTPMG_OK) return rc;
     }
+    {
+        ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};
+        scan_row_ops(e, &b, s.rb, s.inserts);
+    }
+    if (s.rb.has_append) {
+        if (!s.has_row_bufs) {
+            int rc = alloc_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+            if (rc != LTPMG_OK) {
+                free_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+                return rc;
+            }
+            s.has_row_bufs = true;
+        }
+        s.rb.found = s.d_found;
+        s.rb.meta_eff = s.d_meta_eff;
+        s.rb.ins_cnt = s.d_ins_cnt;
+        s.rb.ins_base = s.d_ins_base;
+        s.rb.cursor = e->d_cursor;
+        s.rb.inserted = s.d_counters + 1;
+    }
     HIP_TRY(e, hipEventRecord(s.up0, s.stream));
     HIP_TRY(e, hipMemcpyAsync(s.d_txn_off, st.txn_off, ((size_t)n_txns + 1) * 4, hipMemcpyHostToDevice, s.stream));
     if (n_ops) {
@@ -946,6 +1210,11 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
     const uint32_t rs = max_slot_all(e);
     if (rs > s.cap_res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the batch was submitted");
     OpView ov{s.d_txn_off, s.d_rows, s.d_meta, s.d_args, s.n_txns, s.n_ops};
+    RowBufs *rb = s.rb.has_append ? &s.rb : nullptr;
+    if (rb) {
+        int rc = check_capacity(e, s.inserts);
+        if (rc != LTPMG_OK) return rc;
+    }
     StageRec rec{e, s, profile};
     s.n_stages = 0;
     s.mode = mode;
@@ -958,8 +1227,8 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
     };
     int rc = mode == LTPMG_MODE_A
                  ? launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit, s.d_results, rs,
-                                 rec, wait_tail)
-                 : launch_mode_o(e, s.stream, ov, s.ob, s.d_commit, s.d_results, rs, rec, wait_tail);
+                                 rec, wait_tail, rb)
+                 : launch_mode_o(e, s.stream, ov, s.ob, s.d_commit, s.d_results, rs, rec, wait_tail, rb);
     if (rc != LTPMG_OK) return rc;
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(s.done, s.stream));
@@ -1009,6 +1278,27 @@ int ltpmg_fetch_results(ltpmg_engine *e, uint32_t slot, uint8_t *commit, void *r
     return LTPMG_OK;
 }
 
+int ltpmg_fetch_status(ltpmg_engine *e, uint32_t slot, uint8_t *status) {
+    if (!e) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    Slot *s = get_slot(e, slot);
+    if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
+    if (!s->executed) return fail(e, LTPMG_ESTATE, "slot %u has not executed a batch", slot);
+    if (!s->n_ops) return LTPMG_OK;
+    if (!status) return fail(e, LTPMG_EINVAL, "null status buffer");
+    HIP_TRY(e, hipStreamSynchronize(s->stream));
+    // the batch's offsets are still in the slot's staging buffer
+    const uint32_t *off = stage_ptrs(*s, s->n_txns, s->n_ops).txn_off;
+    std::vector<uint8_t> commit(s->n_txns);
+    HIP_TRY(e, hipMemcpy(commit.data(), s->d_commit, s->n_txns, hipMemcpyDeviceToHost));
+    if (s->rb.has_append) HIP_TRY(e, hipMemcpy(status, s->d_found, s->n_ops, hipMemcpyDeviceToHost));
+    else std::memset(status, 1, s->n_ops);
+    for (uint32_t t = 0; t < s->n_txns; ++t)
+        if (!commit[t]) std::memset(status + off[t], 0, off[t + 1] - off[t]);
+    return LTPMG_OK;
+}
+
 int ltpmg_stats(ltpmg_engine *e, uint32_t slot, ltpmg_batch_stats *out) {
     if (!e || !out) return LTPMG_EINVAL;
     std::lock_guard<std::mutex> g(e->mu);
@@ -1030,6 +1320,7 @@ int ltpmg_stats(ltpmg_engine *e, uint32_t slot, ltpmg_batch_stats *out) {
         out->committed = s->n_txns;
     }
     out->aborted = s->n_txns - out->committed;
+    if (s->rb.has_insert) HIP_TRY(e, hipMemcpy(&out->inserted, s->d_counters + 1, 4, hipMemcpyDeviceToHost));
     HIP_TRY(e, hipEventElapsedTime(&out->exec_ms, s->x0, s->x1));
     HIP_TRY(e, hipEventElapsedTime(&out->upload_ms, s->up0, s->up1));
     out->n_kernels = std::min<uint32_t>(s->n_stages, LTPMG_MAX_KSTATS);
@@ -1160,6 +1451,8 @@ extern "C" int ltpmg_execute_static(ltpmg_engine *e, uint32_t mode, const ltpmg_
     if (!dev->txn_off || !dev->rows || !dev->meta || !dev->args || !dev_commit || !dev_results)
         return fail(e, LTPMG_EINVAL, "null device array");
     if (!e->static_scratch) return fail(e, LTPMG_ESTATE, "no static scratch on device %d", e->device);
+    if (e->n_append)
+        return fail(e, LTPMG_ESTATE, "append tables need ltpmg_execute (their row ops use per-slot buffers)");
     StaticBufs sb;
     if (!carve_static(e->static_scratch, sb))
         return fail(e, LTPMG_ESTATE, "the static scratch is too small for this build's sort tiles");
@@ -1205,6 +1498,14 @@ struct ltpmg_stream {
     ltpmg_stream_plan graph_plan{};
     bool tuned = false, ran = false;
     ltpmg_stream_plan tuned_plan{};
+    // row ops per batch (scan_row_ops), the INSERTs of the whole stream per
+    // table, and the row-op scratch of one launch group
+    std::vector<RowBufs> rows;
+    uint32_t stream_inserts[LTPMG_MAX_TABLES] = {};
+    uint8_t *d_found = nullptr;
+    uint16_t *d_meta_eff = nullptr;
+    uint32_t *d_ins_cnt = nullptr, *d_ins_base = nullptr;
+    uint64_t row_cap_ops = 0;
 };
 
 namespace {
@@ -1213,6 +1514,7 @@ struct Group {
     uint32_t b0, b1, mode;
     bool fused;
     uint32_t grid;
+    RowBufs rb;  // the row ops of batches [b0, b1) together
 };
 
 const uint32_t kTiles[3] = {1024, 2048, 4096};
@@ -1239,17 +1541,28 @@ std::vector<Group> plan_groups(ltpmg_engine *e, const ltpmg_stream &s, const ltp
     const bool each = (p.flags & LTPMG_PLAN_SYNC_EACH) != 0;
     for (uint32_t b = 0; b < s.n_batches; ++b) {
         const uint32_t ops = s.op_base[b + 1] - s.op_base[b];
+        const RowBufs &rb = s.rows[b];
         if (s.modes[b] == LTPMG_MODE_O) {
-            if (!each && !g.empty() && g.back().mode == LTPMG_MODE_O &&
+            // A merged group runs as one ordered batch, which is the same as
+            // its batches one by one -- except that rows an INSERT appends
+            // become visible only after the group, so a batch with INSERTs
+            // ends its group.
+            if (!each && !g.empty() && g.back().mode == LTPMG_MODE_O && !g.back().rb.has_insert &&
                 (uint64_t)s.op_base[b + 1] - s.op_base[g.back().b0] <= p.merge_max_ops) {
-                g.back().b1 = b + 1;
+                Group &last = g.back();
+                last.b1 = b + 1;
+                last.rb.has_append |= rb.has_append;
+                last.rb.has_delete |= rb.has_delete;
+                last.rb.insert_tables |= rb.insert_tables;
+                last.rb.has_insert |= rb.has_insert;
                 continue;
             }
-            g.push_back(Group{b, b + 1, LTPMG_MODE_O, false, 0});
+            g.push_back(Group{b, b + 1, LTPMG_MODE_O, false, 0, rb});
         } else {
+            // the fused kernel has no row-op stages
             const uint32_t n_tiles = (ops + p.fused_tile_ops - 1) / p.fused_tile_ops;
-            const bool fused = cap && ops <= p.fused_max_ops && n_tiles <= (uint64_t)kFusedMaxTiles * cap;
-            g.push_back(Group{b, b + 1, LTPMG_MODE_A, fused, fused ? cc_fused_grid(ops, p.fused_tile_ops, cap) : 0});
+            const bool fused = cap && !rb.has_append && ops <= p.fused_max_ops && n_tiles <= (uint64_t)kFusedMaxTiles * cap;
+            g.push_back(Group{b, b + 1, LTPMG_MODE_A, fused, fused ? cc_fused_grid(ops, p.fused_tile_ops, cap) : 0, rb});
         }
     }
     return g;
@@ -1266,16 +1579,29 @@ void stream_free_scratch(ltpmg_engine *e, ltpmg_stream &s) {
     s.tile_cap_ops = 0;
     free_order_bufs(e, s.ob, s.ob_cap_ops);
     s.ob_cap_ops = 0;
+    if (s.row_cap_ops) free_row_bufs(e, s.row_cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+    s.row_cap_ops = 0;
 }
 
 // Scratch for the largest group of the plan.
 int stream_scratch(ltpmg_engine *e, ltpmg_stream &s, const std::vector<Group> &groups) {
-    uint64_t max_o = 0, max_a = 0, max_any = 0;
+    uint64_t max_o = 0, max_a = 0, max_any = 0, max_row = 0;
     for (const Group &g : groups) {
         const uint64_t ops = (uint64_t)s.op_base[g.b1] - s.op_base[g.b0];
         max_any = std::max(max_any, ops);
         if (g.mode == LTPMG_MODE_O) max_o = std::max(max_o, ops);
         else if (!g.fused) max_a = std::max(max_a, ops);
+        if (g.rb.has_append) max_row = std::max(max_row, ops);
+    }
+    if (max_row > s.row_cap_ops) {
+        if (s.row_cap_ops) free_row_bufs(e, s.row_cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+        s.row_cap_ops = 0;
+        int rc = alloc_row_bufs(e, max_row, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+        if (rc != LTPMG_OK) {
+            free_row_bufs(e, max_row, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+            return rc;
+        }
+        s.row_cap_ops = max_row;
     }
     if (!(s.flags & LTPMG_STREAM_KEEP_RESULTS) && max_any > s.res_cap_ops) {
         dev_free(e, s.d_results, s.res_cap_ops * s.res_stride);
@@ -1311,9 +1637,18 @@ int issue_group(ltpmg_engine *e, ltpmg_stream &s, const Group &g, const ltpmg_st
     OpView ov{s.d_txn_off + t0 + g.b0, s.d_rows + op0, s.d_meta + op0, s.d_args + op0, txns, ops};
     NoRec rec;
     auto now = []() { return LTPMG_OK; };
-    if (g.mode == LTPMG_MODE_O) return launch_mode_o(e, s.stream, ov, s.ob, s.d_commit + t0, results, rs, rec, now);
+    RowBufs rb = g.rb;
+    rb.found = s.d_found;
+    rb.meta_eff = s.d_meta_eff;
+    rb.ins_cnt = s.d_ins_cnt;
+    rb.ins_base = s.d_ins_base;
+    rb.cursor = e->d_cursor;
+    rb.inserted = s.d_counters + 1;
+    RowBufs *rbp = g.rb.has_append ? &rb : nullptr;
+    if (g.mode == LTPMG_MODE_O)
+        return launch_mode_o(e, s.stream, ov, s.ob, s.d_commit + t0, results, rs, rec, now, rbp);
     if (!g.fused)
-        return launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit + t0, results, rs, rec, now);
+        return launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit + t0, results, rs, rec, now, rbp);
     GridBar bar{e->d_sync + 1, e->d_sync + 2, e->d_sync + 3, 1u << 24};
     DevTables dt = dev_tables(e);
     HIP_TRY(e, launch_cc_fused(s.stream, dt, ov, p.fused_tile_ops, g.grid, e->d_sync, s.d_commit + t0, results,
@@ -1353,6 +1688,18 @@ int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan 
     if (max_slot_all(e) > s.res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the stream was created");
     for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
     e->tail_slot = -1;
+    {  // every INSERT of the stream must fit (the run is synchronous: exact cursors)
+        uint32_t cur[LTPMG_MAX_TABLES];
+        HIP_TRY(e, hipMemcpy(cur, e->d_cursor, sizeof cur, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
+            Table &tb = e->tables[t];
+            if (!s.stream_inserts[t]) continue;
+            if ((uint64_t)cur[t] + s.stream_inserts[t] > tb.n_rows)
+                return fail(e, LTPMG_EFULL, "table %u: the stream inserts up to %u rows, %u of %u rows are left", t,
+                            s.stream_inserts[t], tb.n_rows - cur[t], tb.n_rows);
+            tb.cursor_bound = (uint64_t)cur[t] + s.stream_inserts[t];
+        }
+    }
     std::vector<Group> groups = plan_groups(e, s, p);
     int rc = stream_scratch(e, s, groups);
     if (rc != LTPMG_OK) return rc;
@@ -1468,6 +1815,15 @@ int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *bat
         return fail(e, LTPMG_EINVAL, "batch sizes add up to %llu txns, %u given", (unsigned long long)t, all->n_txns);
     }
     s->res_stride = max_slot_all(e);
+    s->rows.resize(n_batches);
+    for (uint32_t b = 0; b < n_batches; ++b) {
+        const uint32_t o0 = s->op_base[b];
+        ltpmg_batch one{s->txn_base[b + 1] - s->txn_base[b], s->op_base[b + 1] - o0, nullptr,
+                        all->rows + o0, all->meta + o0, all->args + o0};
+        uint32_t ins[LTPMG_MAX_TABLES];
+        scan_row_ops(e, &one, s->rows[b], ins);
+        for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) s->stream_inserts[t] += ins[t];
+    }
     const uint64_t n_ops = all->n_ops, n_txns = all->n_txns;
     // per-batch offsets, each batch starting at 0
     std::vector<uint32_t> rel(n_txns + n_batches);

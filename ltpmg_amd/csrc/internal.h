@@ -95,6 +95,7 @@ struct ColDesc {
 
 struct TabDesc {
     unsigned long long *lock;
+    uint8_t *present;    // append tables: one byte per row, 1 = present (nullptr otherwise)
     uint32_t n_rows;
     uint32_t n_cols;
     uint32_t cell_base;  // ordered-mode key of (row 0, col 0): cells are numbered
@@ -118,13 +119,29 @@ __device__ inline uint32_t op_kind(uint16_t m) { return m & 3u; }
 __device__ inline uint32_t op_col(uint16_t m) { return (m >> 2) & 63u; }
 __device__ inline uint32_t op_tab(uint16_t m) { return (m >> 8) & 15u; }
 
-// An op the kernels may act on: reserved meta bits clear, known kind, and
-// row and column inside a live table.  The host checks every submitted op;
-// this keeps an unchecked (trusted) submit from touching memory outside the
-// tables.
+// A cell op the kernels may act on: reserved meta bits clear, a cell kind
+// (READ / UPDATE / RMW_ADD), and row and column inside a live table.  The
+// host checks every submitted op; this keeps an unchecked (trusted) submit
+// from touching memory outside the tables.
 __device__ inline bool op_valid(const TabDesc *tabs, uint16_t m, uint32_t row) {
     const TabDesc &td = tabs[op_tab(m)];
     return (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows && op_col(m) < td.n_cols;
+}
+// Any op the kernels may act on, cell or row op, from the table's row and
+// column counts (n_rows 0 for a dead table) and whether it is an append
+// table: an INSERT names row 0, a DELETE a row of the table.
+__device__ inline bool op_check(uint16_t m, uint32_t row, uint32_t n_rows, uint32_t n_cols, bool append) {
+    if ((m >> 12) != 0) return false;
+    if (op_kind(m) != LTPMG_OP_ROW) return row < n_rows && op_col(m) < n_cols;
+    if (!append) return false;
+    return op_col(m) == LTPMG_ROW_DELETE ? row < n_rows : (op_col(m) == LTPMG_ROW_INSERT && row == 0 && n_rows > 0);
+}
+__device__ inline bool is_insert(uint16_t m) { return op_kind(m) == LTPMG_OP_ROW && op_col(m) == LTPMG_ROW_INSERT; }
+__device__ inline bool is_delete(uint16_t m) { return op_kind(m) == LTPMG_OP_ROW && op_col(m) == LTPMG_ROW_DELETE; }
+// rowseed(arg, c): the payload seed of column c of a row an INSERT appends
+constexpr uint64_t kRowGolden = 0xD1B54A32D192ED03ull;
+__host__ __device__ inline uint64_t rowseed(uint64_t arg, uint32_t col) {
+    return mix64(arg + (uint64_t)(col + 1) * kRowGolden);
 }
 
 // popcount of the bits of m below this lane (wave64)
@@ -249,11 +266,13 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_stream_copy(hipStream_t s, const void *src, void *dst, uint64_t n16);
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                        uint32_t n_tiles, const uint32_t *epoch_ctr);
+// found (nullable): per op, 1 if its row is present at the batch start
+// (validate writes it, write skips cell writes of absent rows)
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                              uint32_t n_tiles, const uint32_t *epoch_ctr, uint8_t *commit,
-                             uint8_t *results, uint32_t res_stride, uint32_t *counters);
+                             uint8_t *results, uint32_t res_stride, uint32_t *counters, uint8_t *found);
 void launch_cc_write(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                     uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot);
+                     uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot, const uint8_t *found);
 
 // Grid barrier state in device memory (zero at start; reset by the last
 // arriver, so it is reused by every launch).  spin_limit bounds each wait.
@@ -271,6 +290,32 @@ uint32_t cc_fused_grid(uint32_t n_ops, uint32_t tile_ops, uint32_t grid);
 hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile_ops,
                            uint32_t grid, uint32_t *epoch_ctr, uint8_t *commit, uint8_t *results,
                            uint32_t res_stride, uint32_t *counters, GridBar bar);
+
+// Per-batch buffers of a batch that touches append tables (INSERT, DELETE,
+// or any op on an append table); absent (has_append false) otherwise.
+struct RowBufs {
+    bool has_append = false, has_delete = false, has_insert = false;
+    uint32_t insert_tables = 0;  // bit t: the batch INSERTs into table t
+    uint8_t *found = nullptr;    // [n_ops]: 1 = the op's row was present (an INSERT: appended)
+    uint16_t *meta_eff = nullptr;  // [n_ops]: ordered mode's meta, ops on absent rows marked invalid
+    uint32_t *ins_cnt = nullptr;   // [ins_tiles(n_ops) * LTPMG_MAX_TABLES] per tile, then its offsets
+    uint32_t *ins_base = nullptr;  // [LTPMG_MAX_TABLES] cursor of each table at the batch start
+    uint32_t *cursor = nullptr;    // [LTPMG_MAX_TABLES] the engine's append cursors
+    uint32_t *inserted = nullptr;  // [1] rows the batch appended
+};
+constexpr uint32_t kInsTile = 2048;  // ops per tile of the insert kernels
+inline uint32_t ins_tiles(uint32_t n_ops) { return (n_ops + kInsTile - 1) / kInsTile; }
+// ordered mode, before the sort: the lowest op that deletes each row (lock
+// words, tagged with a new epoch), then found / meta_eff for every op (a READ
+// of an absent row gets a zeroed result slot).  After the sort and apply:
+// row_deletes clears the presence of rows whose DELETE found them (commit,
+// if given, gates each op by its txn's flag); inserts appends the committed
+// INSERTs.
+void launch_row_presence_o(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, uint32_t *epoch_ctr,
+                           uint8_t *results, uint32_t res_stride);
+void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit);
+void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit,
+                    uint8_t *results, uint32_t res_stride);
 
 struct SortBufs {
     uint32_t *keys[2];

@@ -170,12 +170,18 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
                                     uint32_t *tids, uint32_t n_slots) {
     constexpr unsigned long long kEmpty = ~0ull;
     const unsigned long long ep = (unsigned long long)epoch << 32;
-    // {rows (0 for a dead table), columns, lock words} per table, in LDS
+    // {rows (0 for a dead table), columns, lock words} per table, in LDS,
+    // and a bit per append table
     __shared__ uint4 tc[LTPMG_MAX_TABLES];
+    __shared__ uint32_t s_append;
     if (threadIdx.x < LTPMG_MAX_TABLES) {
         const TabDesc &td = dt.tabs[threadIdx.x];
         const uint64_t lp = reinterpret_cast<uint64_t>(td.lock);
         tc[threadIdx.x] = make_uint4(td.live ? td.n_rows : 0u, td.n_cols, (uint32_t)lp, (uint32_t)(lp >> 32));
+    }
+    if (threadIdx.x < 64) {  // wave 0
+        const uint64_t b = __ballot(threadIdx.x < LTPMG_MAX_TABLES && dt.tabs[threadIdx.x].present != nullptr);
+        if (threadIdx.x == 0) s_append = (uint32_t)b;
     }
     auto offer = [&](uint32_t tab, uint32_t row, uint32_t tid) {
         const uint4 c = tc[tab];
@@ -209,10 +215,12 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
             const uint32_t i = base + j * blockDim.x + threadIdx.x;
             const uint16_t m = (uint16_t)mm[j];
             const uint32_t row = rr[j];
-            if (i >= t.nops || op_kind(m) == LTPMG_OP_READ) continue;
+            // writes (UPDATE, RMW_ADD, DELETE) reserve their row; an INSERT
+            // appends a new row and reserves nothing
+            if (i >= t.nops || op_kind(m) == LTPMG_OP_READ || is_insert(m)) continue;
             const uint4 c = tc[op_tab(m)];
             // never touch memory outside the tables
-            if ((m >> 12) != 0 || op_kind(m) > LTPMG_OP_RMW_ADD || row >= c.x || op_col(m) >= c.y) continue;
+            if (!op_check(m, row, c.x, c.y, (s_append >> op_tab(m)) & 1u)) continue;
             const unsigned long long key = ((unsigned long long)op_tab(m) << 32) | row;
             const uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (n_slots - 1);
             const unsigned long long prev = atomicCAS(&keys[slot], kEmpty, key);
@@ -228,15 +236,21 @@ __device__ inline void tile_reserve(const DevTables &dt, const OpView &ov, const
 }
 
 // Pass 2 on one tile: commit flags into ok[] and commit[], committed READs
-// copied to their result slots.  Returns the tile's committed count.
+// copied to their result slots.  Returns the tile's committed count.  With
+// found (a batch that touches append tables), found[o] gets whether op o's
+// row is present at the batch start -- what a committed op sees, since a
+// committed DELETE excludes every higher TID from its row -- and a committed
+// READ of an absent row gets a zeroed slot instead of the cell.
 template <int G, bool VEC16>
 __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView &ov, const Tile &t,
                                               const uint16_t *owner, uint32_t epoch, uint8_t *ok,
                                               uint32_t *list, uint8_t *commit, uint8_t *results,
-                                              uint32_t res_stride) {
+                                              uint32_t res_stride, uint8_t *found) {
     __shared__ uint32_t n_list, n_commit;
-    // {rows (0 for a dead table), columns, lock words} per table
+    // {rows (0 for a dead table), columns, lock words} per table, and the
+    // presence bytes of append tables
     __shared__ uint4 tc[LTPMG_MAX_TABLES];
+    __shared__ const uint8_t *s_pres[LTPMG_MAX_TABLES];
     __syncthreads();  // the previous pass's readers of ok[] and list[] are done
     for (uint32_t j = threadIdx.x; j < t.nt; j += blockDim.x) ok[j] = 1;
     if (threadIdx.x == 0) n_list = n_commit = 0;
@@ -244,6 +258,7 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
         const TabDesc &td = dt.tabs[threadIdx.x];
         const uint64_t lp = reinterpret_cast<uint64_t>(td.lock);
         tc[threadIdx.x] = make_uint4(td.live ? td.n_rows : 0u, td.n_cols, (uint32_t)lp, (uint32_t)(lp >> 32));
+        s_pres[threadIdx.x] = td.present;
     }
     __syncthreads();
     // kBatch ops per thread at a time, every load of a batch issued before
@@ -265,11 +280,23 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
         for (uint32_t j = 0; j < kBatch; ++j) {
             const uint16_t m = (uint16_t)mm[j];
             const uint4 c = tc[op_tab(m)];
-            valid[j] = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && rr[j] < c.x && op_col(m) < c.y;
+            valid[j] = op_check(m, rr[j], c.x, c.y, s_pres[op_tab(m)] != nullptr);
+            // an INSERT touches no existing row: no lock to check
             const unsigned long long *p =
-                valid[j] ? reinterpret_cast<const unsigned long long *>(((uint64_t)c.w << 32) | c.z) + rr[j]
-                         : &g_no_lock;
+                valid[j] && !is_insert(m)
+                    ? reinterpret_cast<const unsigned long long *>(((uint64_t)c.w << 32) | c.z) + rr[j]
+                    : &g_no_lock;
             lk[j] = *(const LTPMG_GLOBAL unsigned long long *)p;
+        }
+        if (found) {  // presence at the batch start (INSERTs: decided when appended)
+#pragma unroll
+            for (uint32_t j = 0; j < kBatch; ++j) {
+                const uint32_t i = base + j * blockDim.x + threadIdx.x;
+                if (i >= t.nops) continue;
+                const uint16_t m = (uint16_t)mm[j];
+                const uint8_t *pres = s_pres[op_tab(m)];
+                found[t.op0 + i] = (valid[j] && pres && !is_insert(m)) ? pres[rr[j]] : (uint8_t)1;
+            }
         }
 #pragma unroll
         for (uint32_t j = 0; j < kBatch; ++j) {
@@ -291,11 +318,17 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
     }
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if (__lane_id() == 0 && mine) atomicAdd(&n_commit, mine);
-    // committed READs of this tile -> LDS list (loop bound uniform per wave)
+    // committed READs of this tile -> LDS list (loop bound uniform per wave);
+    // a READ of an absent row gets zeros (found[] of op i was written by this
+    // same thread above: both loops give op i to thread i % blockDim.x)
     for (uint32_t base = 0; base < t.nops; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool take = false;
         if (i < t.nops) take = op_kind(ov.meta[t.op0 + i]) == LTPMG_OP_READ && ok[owner[i]];
+        if (take && found && !found[t.op0 + i]) {
+            for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)(t.op0 + i) * res_stride + b, 0u);
+            take = false;
+        }
         wave_append(take, t.op0 + i, list, &n_list);
     }
     __syncthreads();
@@ -306,10 +339,12 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
 
 // Pass 3 on one tile: committed writes.  ok[] holds the tile's commit
 // flags, or is filled here from commit[] when fill_ok is set.
+// With found, a write of a row absent at the batch start is skipped (row
+// ops are applied by their own kernels afterwards).
 template <int G, bool VEC16>
 __device__ inline void tile_write(const DevTables &dt, const OpView &ov, const Tile &t,
                                   const uint16_t *owner, uint8_t *ok, bool fill_ok,
-                                  const uint8_t *commit, uint32_t *list) {
+                                  const uint8_t *commit, uint32_t *list, const uint8_t *found) {
     __shared__ uint32_t n_list;
     __syncthreads();  // the previous pass's readers of ok[] and list[] are done
     if (fill_ok)
@@ -321,7 +356,8 @@ __device__ inline void tile_write(const DevTables &dt, const OpView &ov, const T
         bool take = false;
         if (i < t.nops && ok[owner[i]]) {
             uint32_t o = t.op0 + i;
-            take = op_kind(ov.meta[o]) != LTPMG_OP_READ && op_valid(dt.tabs, ov.meta[o], ov.rows[o]);
+            take = op_kind(ov.meta[o]) != LTPMG_OP_READ && op_valid(dt.tabs, ov.meta[o], ov.rows[o]) &&
+                   (!found || found[o]);
         }
         wave_append(take, t.op0 + i, list, &n_list);
     }
@@ -435,26 +471,26 @@ __global__ __launch_bounds__(kBlock) void k_cc_validate_read(DevTables dt, OpVie
                                                              const uint32_t *epoch_ctr,
                                                              uint8_t *commit, uint8_t *results,
                                                              uint32_t res_stride,
-                                                             uint32_t *counters) {
+                                                             uint32_t *counters, uint8_t *found) {
     __shared__ uint16_t owner[kTileOpsMax];
     __shared__ uint8_t ok[kTileOps];
     __shared__ uint32_t list[kTileOpsMax];
     const uint32_t epoch = *epoch_ctr;
     Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
     uint32_t c = tile_validate_read<G, VEC16>(dt, ov, t, owner, epoch, ok, list, commit, results,
-                                              res_stride);
+                                              res_stride, found);
     if (threadIdx.x == 0 && c) atomicAdd(&counters[0], c);
 }
 
 template <int G, bool VEC16>
 __global__ __launch_bounds__(kBlock) void k_cc_write(DevTables dt, OpView ov,
                                                      const uint32_t *tile_t0,
-                                                     const uint8_t *commit) {
+                                                     const uint8_t *commit, const uint8_t *found) {
     __shared__ uint16_t owner[kTileOpsMax];
     __shared__ uint8_t ok[kTileOps];
     __shared__ uint32_t list[kTileOpsMax];
     Tile t = fill_owner(ov, tile_t0[blockIdx.x], tile_t0[blockIdx.x + 1], owner);
-    tile_write<G, VEC16>(dt, ov, t, owner, ok, true, commit, list);
+    tile_write<G, VEC16>(dt, ov, t, owner, ok, true, commit, list, found);
 }
 
 // All three passes in one persistent launch.  Workgroup b owns tiles
@@ -509,13 +545,13 @@ __global__ __launch_bounds__(kBlock) void k_cc_fused(DevTables dt, OpView ov, ui
     for (uint32_t j = 0; j < nk; ++j) {
         if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner, TILE, TILE + LTPMG_MAX_TXN_OPS);
         committed += tile_validate_read<G, VEC16>(dt, ov, t, owner, epoch, ok, list, commit,
-                                                  results, res_stride);
+                                                  results, res_stride, nullptr);
     }
     if (threadIdx.x == 0 && committed) atomicAdd(&counters[0], committed);
     if (!grid_sync(bar)) return;
     for (uint32_t j = 0; j < nk; ++j) {
         if (nk > 1) t = fill_owner(ov, s_t0[j], s_t0[j + 1], owner, TILE, TILE + LTPMG_MAX_TXN_OPS);
-        tile_write<G, VEC16>(dt, ov, t, owner, ok, nk > 1, commit, list);
+        tile_write<G, VEC16>(dt, ov, t, owner, ok, nk > 1, commit, list, nullptr);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *epoch_ctr = epoch;
 }
@@ -550,16 +586,16 @@ void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *t
 
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                              uint32_t n_tiles, const uint32_t *epoch_ctr, uint8_t *commit,
-                             uint8_t *results, uint32_t res_stride, uint32_t *counters) {
+                             uint8_t *results, uint32_t res_stride, uint32_t *counters, uint8_t *found) {
     // the result stride is the largest slot, so it selects the vector width
     uint32_t max_slot = res_stride;
     LTPMG_CC_DISPATCH(k_cc_validate_read, dim3(n_tiles), dt, ov, tile_t0, epoch_ctr, commit,
-                      results, res_stride, counters);
+                      results, res_stride, counters, found);
 }
 
 void launch_cc_write(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
-                     uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot) {
-    LTPMG_CC_DISPATCH(k_cc_write, dim3(n_tiles), dt, ov, tile_t0, commit);
+                     uint32_t n_tiles, const uint8_t *commit, uint32_t max_slot, const uint8_t *found) {
+    LTPMG_CC_DISPATCH(k_cc_write, dim3(n_tiles), dt, ov, tile_t0, commit, found);
 }
 
 namespace {

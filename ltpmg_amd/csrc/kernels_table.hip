@@ -114,28 +114,4 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
     hipLaunchKernelGGL(k_zero_u32, dim3(blocks), dim3(kBlock), 0, s, p, n);
 }
 
-// tile_t0[k] = first txn whose first op index is >= k * kTileOps, k in [0, n_tiles].
-// Every txn has >= 1 op, so a tile owns <= kTileOps txns and
-// < kTileOps + LTPMG_MAX_TXN_OPS ops.
-__global__ void k_tile_bounds(const uint32_t *txn_off, uint32_t n_txns, uint32_t n_tiles,
-                              uint32_t *tile_t0) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > n_tiles) return;
-    uint64_t target = (uint64_t)k * kTileOps;
-    uint32_t lo = 0, hi = n_txns;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if ((uint64_t)txn_off[mid] < target) lo = mid + 1;
-        else hi = mid;
-    }
-    tile_t0[k] = lo;
-}
-
-void launch_tile_bounds(hipStream_t s, const uint32_t *txn_off, uint32_t n_txns,
-                        uint32_t n_tiles, uint32_t *tile_t0) {
-    uint32_t blocks = (n_tiles + 1 + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_tile_bounds, dim3(blocks), dim3(kBlock), 0, s, txn_off, n_txns,
-                       n_tiles, tile_t0);
-}
-
 }  // namespace ltpmg

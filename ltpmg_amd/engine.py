@@ -18,13 +18,14 @@ MAX_TABLES = 16
 MAX_COLS = 64
 MAX_TXN_OPS = 256
 MAX_SLOTS = 8
-OP_READ, OP_UPDATE, OP_RMW_ADD = 0, 1, 2
+OP_READ, OP_UPDATE, OP_RMW_ADD, OP_ROW = 0, 1, 2, 3
+ROW_INSERT, ROW_DELETE = 0, 1  # column field of an OP_ROW op
 MODE_A, MODE_O = 0, 1
 EXEC_PROFILE = 0x100
 CFG_PROFILE = 1
 
 STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EBUDGET", -4: "EHIP", -5: "ESTATE",
-          -6: "EDEVICE"}
+          -6: "EDEVICE", -7: "EFULL"}
 
 
 def meta(kind: int, table: int, col: int) -> int:
@@ -58,7 +59,7 @@ class _Stats(C.Structure):
                 ("aborted", C.c_uint32), ("mode", C.c_uint32), ("exec_ms", C.c_float),
                 ("upload_ms", C.c_float), ("n_kernels", C.c_uint32),
                 ("kernel_name", (C.c_char * 32) * 24), ("kernel_ms", C.c_float * 24),
-                ("hbm_allocated", C.c_uint64)]
+                ("hbm_allocated", C.c_uint64), ("inserted", C.c_uint32)]
 
 
 class _Staging(C.Structure):
@@ -81,6 +82,11 @@ ENGINE_API = {
     "ltpmg_shared_engine": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "ltpmg_last_error": (C.c_char_p, [C.c_void_p]),
     "ltpmg_create_table": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_TableDesc)]),
+    "ltpmg_create_append_table": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_TableDesc),
+                                            C.c_uint32]),
+    "ltpmg_table_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "ltpmg_read_presence": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_void_p]),
     "ltpmg_fill_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
     "ltpmg_load_column": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_void_p]),
@@ -98,6 +104,7 @@ ENGINE_API = {
     "ltpmg_wait": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ltpmg_fetch_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                       C.c_uint64]),
+    "ltpmg_fetch_status": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "ltpmg_stats": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Stats)]),
     "ltpmg_selftest_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                       C.c_uint32]),
@@ -432,6 +439,30 @@ class Engine:
         self._check("ltpmg_create_table", self._lib.ltpmg_create_table(self._h, table_id, C.byref(d)))
         self.tables[table_id] = (n_rows, list(col_bytes))
 
+    def create_append_table(self, table_id: int, capacity: int, col_bytes: list[int],
+                            n_present: int = 0) -> None:
+        """A table of `capacity` rows whose rows [0, n_present) are present;
+        INSERT ops append after them, DELETE ops clear presence."""
+        d = _TableDesc(capacity, len(col_bytes))
+        for i, w in enumerate(col_bytes):
+            d.col_bytes[i] = w
+        self._check("ltpmg_create_append_table",
+                    self._lib.ltpmg_create_append_table(self._h, table_id, C.byref(d), n_present))
+        self.tables[table_id] = (capacity, list(col_bytes))
+
+    def table_rows(self, table_id: int) -> int:
+        """An append table's cursor: rows allocated so far."""
+        v = C.c_uint32(0)
+        self._check("ltpmg_table_rows", self._lib.ltpmg_table_rows(self._h, table_id, C.byref(v)))
+        return v.value
+
+    def read_presence(self, table_id: int, row0: int = 0, n_rows: int | None = None) -> np.ndarray:
+        n = self.tables[table_id][0] - row0 if n_rows is None else n_rows
+        out = np.empty(n, np.uint8)
+        self._check("ltpmg_read_presence",
+                    self._lib.ltpmg_read_presence(self._h, table_id, row0, n, _ptr(out)))
+        return out
+
     def fill_table(self, table_id: int, seed: int) -> None:
         self._check("ltpmg_fill_table", self._lib.ltpmg_fill_table(self._h, table_id, seed))
 
@@ -507,6 +538,13 @@ class Engine:
                                                   res.nbytes if res is not None else 0))
         return commit, res
 
+    def fetch_status(self, slot: int = 0) -> np.ndarray:
+        """Per op: 1 when it took effect (committed, and its row present or
+        its INSERT given a row), else 0."""
+        out = np.empty(self._batches[slot].n_ops, np.uint8)
+        self._check("ltpmg_fetch_status", self._lib.ltpmg_fetch_status(self._h, slot, _ptr(out)))
+        return out
+
     def stats(self, slot: int = 0) -> dict:
         s = _Stats()
         self._check("ltpmg_stats", self._lib.ltpmg_stats(self._h, slot, C.byref(s)))
@@ -518,7 +556,8 @@ class Engine:
         return {"n_txns": s.n_txns, "n_ops": s.n_ops, "committed": s.committed,
                 "aborted": s.aborted, "mode": s.mode, "exec_ms": float(s.exec_ms),
                 "upload_ms": float(s.upload_ms), "kernels": kernels,
-                "kernel_launches": launches, "hbm_allocated": int(s.hbm_allocated)}
+                "kernel_launches": launches, "hbm_allocated": int(s.hbm_allocated),
+                "inserted": int(s.inserted)}
 
     def run(self, batch: Batch, mode: int, slot: int = 0, results: bool = True):
         self.submit(batch, slot)

@@ -85,6 +85,8 @@ extern "C" double orc_cpu_pipeline(orc_table *tables, uint32_t n_tables, uint32_
     std::vector<uint64_t> base(n_tables + 1, 0);
     for (uint32_t t = 0; t < n_tables; ++t) base[t + 1] = base[t] + tables[t].n_rows;
     const uint64_t total_rows = base[n_tables];
+    for (uint32_t t = 0; t < n_tables; ++t)
+        if (tables[t].present) return -1.0;  // append tables: orc_replay only
     for (uint32_t i = 0; i < n_ops; ++i) {
         OpRef o = decode(meta[i]);
         if (o.kind > ORC_KIND_RMW_ADD || o.tab >= n_tables || rows[i] >= tables[o.tab].n_rows ||

@@ -51,10 +51,19 @@ static int op_fields(const orc_table *tables, uint32_t n_tables, uint32_t row, u
     *kind = m & 3u;
     *col = (m >> 2) & 63u;
     *tab = (m >> 8) & 15u;
-    if ((m >> 12) != 0 || *kind > ORC_KIND_RMW_ADD) return -1;
-    if (*tab >= n_tables) return -1;
+    if ((m >> 12) != 0) return -1;
+    if (*tab >= n_tables || tables[*tab].n_rows == 0) return -1;
+    if (*kind == ORC_KIND_ROW) {
+        if (!tables[*tab].present) return -1;
+        if (*col == ORC_ROW_DELETE) return row < tables[*tab].n_rows ? 0 : -1;
+        return (*col == ORC_ROW_INSERT && row == 0) ? 0 : -1;
+    }
     if (*col >= tables[*tab].n_cols || row >= tables[*tab].n_rows) return -1;
     return 0;
+}
+
+static uint64_t rowseed(uint64_t arg, uint32_t col) {
+    return orc_mix64(arg + (uint64_t)(col + 1) * 0xD1B54A32D192ED03ull);
 }
 
 /* Apply one op to the current image, in place (the serial step). */
@@ -83,7 +92,7 @@ static void apply_op(orc_table *t, uint32_t kind, uint32_t col, uint32_t row, ui
 int orc_replay(orc_table *tables, uint32_t n_tables, uint32_t n_txns,
                const uint32_t *txn_off, const uint32_t *rows, const uint16_t *meta,
                const uint64_t *args, uint32_t mode, uint8_t *commit,
-               uint8_t *results, uint32_t res_stride) {
+               uint8_t *results, uint32_t res_stride, uint8_t *found) {
     if (mode != ORC_MODE_A && mode != ORC_MODE_O) return -1;
     if (txn_off[0] != 0) return -1;
     for (uint32_t t = 0; t < n_txns; ++t)
@@ -93,7 +102,9 @@ int orc_replay(orc_table *tables, uint32_t n_tables, uint32_t n_txns,
         uint32_t k, tb, c;
         if (op_fields(tables, n_tables, rows[i], meta[i], &k, &tb, &c)) return -2;
         if (k == ORC_KIND_READ && tables[tb].width[c] > res_stride) return -3;
+        if (k == ORC_KIND_ROW && res_stride < 4) return -3;
     }
+    if (found) memset(found, 0, n_ops);
 
     if (mode == ORC_MODE_O) {
         for (uint32_t t = 0; t < n_txns; ++t) commit[t] = 1;
@@ -109,13 +120,15 @@ int orc_replay(orc_table *tables, uint32_t n_tables, uint32_t n_txns,
             for (uint32_t i = txn_off[t]; i < txn_off[t + 1]; ++i) {
                 uint32_t k, tb, c;
                 op_fields(tables, n_tables, rows[i], meta[i], &k, &tb, &c);
-                if (k != ORC_KIND_READ && W[tb][rows[i]] > t) W[tb][rows[i]] = t;
+                if (k == ORC_KIND_READ || (k == ORC_KIND_ROW && c == ORC_ROW_INSERT)) continue;
+                if (W[tb][rows[i]] > t) W[tb][rows[i]] = t;
             }
         for (uint32_t t = 0; t < n_txns; ++t) {
             uint8_t ok = 1;
             for (uint32_t i = txn_off[t]; i < txn_off[t + 1]; ++i) {
                 uint32_t k, tb, c;
                 op_fields(tables, n_tables, rows[i], meta[i], &k, &tb, &c);
+                if (k == ORC_KIND_ROW && c == ORC_ROW_INSERT) continue;  /* no existing row */
                 if (W[tb][rows[i]] < t) ok = 0;
             }
             commit[t] = ok;
@@ -123,17 +136,59 @@ int orc_replay(orc_table *tables, uint32_t n_tables, uint32_t n_txns,
         for (uint32_t tb = 0; tb < n_tables; ++tb) free(W[tb]);
     }
 
-    /* Serial replay of the committed txns in ascending TID, ops in txn order. */
+    /* Serial replay of the committed txns in ascending TID, ops in txn order.
+     * INSERTs are collected and appended after the last txn, in op order. */
+    uint32_t n_ins = 0;
     for (uint32_t t = 0; t < n_txns; ++t) {
         if (!commit[t]) continue;
         for (uint32_t i = txn_off[t]; i < txn_off[t + 1]; ++i) {
             uint32_t k, tb, c;
             op_fields(tables, n_tables, rows[i], meta[i], &k, &tb, &c);
-            apply_op(&tables[tb], k, c, rows[i], args[i],
-                     results + (uint64_t)i * res_stride, res_stride);
+            orc_table *tt = &tables[tb];
+            uint8_t *slot = results + (uint64_t)i * res_stride;
+            if (k == ORC_KIND_ROW) {
+                if (c == ORC_ROW_INSERT) {
+                    ++n_ins;
+                } else if (tt->present[rows[i]]) {
+                    tt->present[rows[i]] = 0;
+                    if (found) found[i] = 1;
+                }
+                continue;
+            }
+            if (tt->present && !tt->present[rows[i]]) {
+                if (k == ORC_KIND_READ) memset(slot, 0, res_stride);
+                continue;
+            }
+            apply_op(tt, k, c, rows[i], args[i], slot, res_stride);
+            if (found) found[i] = 1;
         }
     }
-    return 0;
+    int rc = 0;
+    for (uint32_t t = 0; t < n_txns && n_ins; ++t) {
+        if (!commit[t]) continue;
+        for (uint32_t i = txn_off[t]; i < txn_off[t + 1]; ++i) {
+            uint32_t k, tb, c;
+            op_fields(tables, n_tables, rows[i], meta[i], &k, &tb, &c);
+            if (k != ORC_KIND_ROW || c != ORC_ROW_INSERT) continue;
+            orc_table *tt = &tables[tb];
+            if (tt->cursor >= tt->n_rows) {
+                rc = -5;
+                continue;
+            }
+            const uint32_t row = tt->cursor++;
+            for (uint32_t col = 0; col < tt->n_cols; ++col)
+                orc_bytes_from_stream(rowseed(args[i], col), tt->width[col], tt->col[col] + (uint64_t)row * tt->width[col]);
+            tt->present[row] = 1;
+            uint8_t *slot = results + (uint64_t)i * res_stride;
+            memset(slot, 0, res_stride);
+            slot[0] = (uint8_t)row;
+            slot[1] = (uint8_t)(row >> 8);
+            slot[2] = (uint8_t)(row >> 16);
+            slot[3] = (uint8_t)(row >> 24);
+            if (found) found[i] = 1;
+        }
+    }
+    return rc;
 }
 
 int orc_commit_pairwise(uint32_t n_txns, const uint32_t *txn_off, const uint32_t *rows,
@@ -142,10 +197,12 @@ int orc_commit_pairwise(uint32_t n_txns, const uint32_t *txn_off, const uint32_t
         uint8_t ok = 1;
         for (uint32_t s = 0; s < t && ok; ++s)
             for (uint32_t i = txn_off[s]; i < txn_off[s + 1] && ok; ++i) {
-                if ((meta[i] & 3u) == ORC_KIND_READ) continue;
+                /* READs and INSERTs write no existing row */
+                if ((meta[i] & 3u) == ORC_KIND_READ || (meta[i] & 0xFFu) == ORC_KIND_ROW) continue;
                 uint32_t tab_s = (meta[i] >> 8) & 15u;
                 for (uint32_t j = txn_off[t]; j < txn_off[t + 1]; ++j)
-                    if (rows[j] == rows[i] && ((meta[j] >> 8) & 15u) == tab_s) { ok = 0; break; }
+                    if ((meta[j] & 0xFFu) != ORC_KIND_ROW && rows[j] == rows[i] &&
+                        ((meta[j] >> 8) & 15u) == tab_s) { ok = 0; break; }
             }
         commit[t] = ok;
     }

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ltpmg_amd.engine import OP_READ, OP_RMW_ADD, OP_UPDATE, Batch, meta
+from ltpmg_amd.engine import OP_READ, OP_RMW_ADD, OP_ROW, OP_UPDATE, ROW_DELETE, ROW_INSERT, Batch, meta
 
 
 def random_batch(rng: np.random.Generator, n_txns: int, n_rows: int, n_cols: int, *,
@@ -52,4 +52,36 @@ def op_txn(batch: Batch) -> np.ndarray:
     return np.repeat(np.arange(batch.n_txns), np.diff(batch.txn_off.astype(np.int64)))
 
 
-__all__ = ["random_batch", "concat", "op_txn", "meta", "OP_READ", "OP_UPDATE", "OP_RMW_ADD"]
+def random_row_batch(rng: np.random.Generator, n_txns: int, n_rows: int, n_cols: int, *,
+                     table: int = 0, max_len: int = 4, insert_pct: int = 20, delete_pct: int = 15,
+                     kinds=(OP_READ, OP_UPDATE, OP_RMW_ADD)) -> Batch:
+    """Random txns on an append table of n_rows capacity: INSERTs (row 0,
+    a random arg), DELETEs and cell ops, each non-INSERT op of a txn on its
+    own row (so no txn accesses a row it deletes or a cell it wrote)."""
+    txns = []
+    for _ in range(n_txns):
+        length = int(rng.integers(1, max_len + 1))
+        ops, used = [], set()
+        for _ in range(length):
+            u = int(rng.integers(0, 100))
+            if u < insert_pct:
+                ops.append((OP_ROW, table, ROW_INSERT, 0, int(rng.integers(0, 2**63))))
+                continue
+            row = int(rng.integers(0, n_rows))
+            if row in used:
+                continue
+            used.add(row)
+            if u < insert_pct + delete_pct:
+                ops.append((OP_ROW, table, ROW_DELETE, row, 0))
+                continue
+            kind = int(rng.choice(kinds))
+            arg = int(rng.integers(0, 2**63)) if kind == OP_UPDATE else int(rng.integers(-3, 4)) & 0xFFFFFFFFFFFFFFFF
+            ops.append((kind, table, int(rng.integers(0, n_cols)), row, arg))
+        if not ops:
+            ops.append((OP_ROW, table, ROW_INSERT, 0, int(rng.integers(0, 2**63))))
+        txns.append(ops)
+    return Batch.from_txns(txns)
+
+
+__all__ = ["random_batch", "random_row_batch", "concat", "op_txn", "meta", "OP_READ", "OP_UPDATE",
+           "OP_RMW_ADD", "OP_ROW", "ROW_INSERT", "ROW_DELETE"]

@@ -19,7 +19,8 @@ MODE_A, MODE_O = 0, 1
 
 class _OrcTable(C.Structure):
     _fields_ = [("n_rows", C.c_uint32), ("n_cols", C.c_uint32),
-                ("width", C.c_uint32 * MAX_COLS), ("col", C.c_void_p * MAX_COLS)]
+                ("width", C.c_uint32 * MAX_COLS), ("col", C.c_void_p * MAX_COLS),
+                ("present", C.c_void_p), ("cursor", C.c_uint32)]
 
 
 _lib = None
@@ -49,7 +50,7 @@ def lib() -> C.CDLL:
     batch_args = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                   C.c_void_p, C.c_void_p, C.c_uint32]
     L.orc_replay.restype = C.c_int
-    L.orc_replay.argtypes = [C.POINTER(_OrcTable), C.c_uint32, *batch_args]
+    L.orc_replay.argtypes = [C.POINTER(_OrcTable), C.c_uint32, *batch_args, C.c_void_p]
     L.orc_commit_pairwise.restype = C.c_int
     L.orc_commit_pairwise.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.orc_cpu_pipeline.restype = C.c_double
@@ -77,12 +78,20 @@ def payload(arg: int, width: int) -> np.ndarray:
 
 
 class Image:
-    """Host image of the tables: images[t][c] is an (n_rows, width) uint8 array."""
+    """Host image of the tables: images[t][c] is an (n_rows, width) uint8 array.
+    append={t: n_present} makes table t an append table: presence bytes
+    present[t] (1 for rows [0, n_present)) and cursor[t] = n_present."""
 
-    def __init__(self, schema: dict[int, tuple[int, list[int]]]):
+    def __init__(self, schema: dict[int, tuple[int, list[int]]], append: dict[int, int] | None = None):
         self.schema = schema
         self.cols = {t: [np.zeros((n, w), np.uint8) for w in widths]
                      for t, (n, widths) in schema.items()}
+        self.present, self.cursor = {}, {}
+        for t, k in (append or {}).items():
+            n = schema[t][0]
+            self.present[t] = np.zeros(n, np.uint8)
+            self.present[t][:k] = 1
+            self.cursor[t] = k
 
     @property
     def n_tables(self) -> int:
@@ -113,6 +122,8 @@ class Image:
     def copy(self) -> "Image":
         im = Image(self.schema)
         im.cols = {t: [a.copy() for a in cs] for t, cs in self.cols.items()}
+        im.present = {t: a.copy() for t, a in self.present.items()}
+        im.cursor = dict(self.cursor)
         return im
 
     def tables(self):
@@ -123,7 +134,15 @@ class Image:
             for c, w in enumerate(widths):
                 arr[t].width[c] = w
                 arr[t].col[c] = self.cols[t][c].ctypes.data
+            if t in self.present:
+                arr[t].present = self.present[t].ctypes.data
+                arr[t].cursor = self.cursor[t]
         return arr
+
+    def sync(self, arr) -> None:
+        """Reads the append cursors back from the struct array a call changed."""
+        for t in self.cursor:
+            self.cursor[t] = arr[t].cursor
 
     def digest(self) -> int:
         """Same definition as the engine's table digest, restated with numpy."""
@@ -209,17 +228,27 @@ def sparse_image(schema: dict, seed: int, batches) -> tuple[Image, dict]:
     return im, cells
 
 
-def replay(image: Image, batch, mode: int, res_stride: int):
-    """Serial replay in TID order; mutates image.  Returns (commit, results)."""
+class AppendFull(ValueError):
+    """The INSERTs of a batch did not all fit their append tables."""
+
+
+def replay(image: Image, batch, mode: int, res_stride: int, with_found: bool = False):
+    """Serial replay in TID order; mutates image.  Returns (commit, results),
+    or (commit, results, found) with with_found."""
     n_txns = len(batch.txn_off) - 1
     commit = np.zeros(n_txns, np.uint8)
     results = np.zeros((len(batch.rows), res_stride), np.uint8)
+    found = np.zeros(len(batch.rows), np.uint8)
     tabs = image.tables()
     rc = lib().orc_replay(tabs, image.n_tables, n_txns, _p(batch.txn_off), _p(batch.rows),
-                          _p(batch.meta), _p(batch.args), mode, _p(commit), _p(results), res_stride)
+                          _p(batch.meta), _p(batch.args), mode, _p(commit), _p(results), res_stride,
+                          _p(found))
+    image.sync(tabs)
+    if rc == -5:
+        raise AppendFull("orc_replay: an append table is full")
     if rc != 0:
         raise ValueError(f"orc_replay rejected the batch ({rc})")
-    return commit, results
+    return (commit, results, found) if with_found else (commit, results)
 
 
 def commit_pairwise(batch) -> np.ndarray:
@@ -243,7 +272,8 @@ def cpu_pipeline(image: Image, batch, mode: int, res_stride: int, n_threads: int
 
 
 def read_mask(batch, commit: np.ndarray) -> np.ndarray:
-    """Ops whose result slot is defined: READs of committed txns."""
+    """Ops whose result slot is defined: READs and INSERTs of committed txns."""
     n_txns = len(batch.txn_off) - 1
     op_txn = np.repeat(np.arange(n_txns), np.diff(batch.txn_off.astype(np.int64)))
-    return ((batch.meta & 3) == 0) & (commit[op_txn] != 0)
+    defined = ((batch.meta & 3) == 0) | ((batch.meta & 0xFF) == 3)
+    return defined & (commit[op_txn] != 0)
