@@ -19,8 +19,10 @@ Beside the headline line's fields it reports, from the same run:
                        vs plain per-batch launches
   serving_large_schema served vs plain on W=1024 order entry + a 100M-row
                        key-value table, arrivals from the order-entry mix
+  order_entry_plain    one W=64 order-entry batch per launch, both modes
   cpu_baseline         the threaded CPU restatement of the reference's
-                       pipeline on a sample of the same batch
+                       pipeline (all cores, 1 core) and the serial oracle,
+                       on this batch and on the reference's CPU config
 
 Prints one JSON line (see the field list at the end of main()).
 """
@@ -212,20 +214,21 @@ def batching_compare(args) -> dict:
     Both modes are run over the whole stream.  Value: committed txns of one
     pass over the stream / device time of that pass."""
     from ltpmg_amd import MODE_A, MODE_O, Engine, Stream, StreamPlan
-    from ltpmg_amd.engine import OeStream, oe_schema
+    from ltpmg_amd.engine import OeStream, create_oe_tables
     rng = np.random.default_rng(BATCH_SEED + 4)
     nb = args.batching_batches
     sizes = np.floor(np.exp(rng.uniform(np.log(args.batching_min_txns),
                                         np.log(args.batching_max_txns), nb))).astype(np.int64)
-    W, cap, kv_tab, n_cols, width = 64, int(sizes.max()), 9, 10, 100
+    # the order-entry batches append rows; every run of the stream rewinds
+    # the append tables to their preloaded rows, so one pass needs room for
+    # the inserts of the order-entry half of the stream
+    W, cap, kv_tab, n_cols, width = 64, int(sizes[1::2].sum()), 9, 10, 100
     e = Engine()
-    schema = dict(oe_schema(W, cap))
-    schema[kv_tab] = (args.rows, [width] * n_cols)
-    for t, (n, widths) in sorted(schema.items()):
-        e.create_table(t, n, widths)
-        e.fill_table(t, TABLE_SEED)
+    create_oe_tables(e, W, cap, TABLE_SEED)
+    e.create_table(kv_tab, args.rows, [width] * n_cols)
+    e.fill_table(kv_tab, TABLE_SEED)
     from ltpmg_amd import OP_UPDATE, gen_kv
-    oe = OeStream(W, cap, seed=BATCH_SEED + 5)
+    oe = OeStream(W, seed=BATCH_SEED + 5)
     batches, kinds = [], []
     t0 = time.perf_counter()
     for i, n in enumerate(sizes):
@@ -248,7 +251,7 @@ def batching_compare(args) -> dict:
            "batch_txns_min": int(sizes.min()), "batch_txns_max": int(sizes.max()),
            "order_entry_partitions": W, "kv_rows": args.rows, "modes": {}}
     for name, mode in (("ordered", MODE_O), ("lowest_tid_wins", MODE_A)):
-        with Stream(e, batches, mode) as st:
+        with Stream(e, batches, mode, rewind=True) as st:
             plain_ms = [st.run(StreamPlan.plain()) for _ in range(2)]
             committed = int(st.committed().sum())
             plan, trials = st.tune()
@@ -280,32 +283,36 @@ def serve_c5(args) -> dict:
     transactions drawn from the order-entry mix (45% NewOrder 5-15 lines,
     43% Payment, 12% read-only; 15% of Payments by last name) over W=1024
     partitions, with a 100M-row x 10 x 100 B key-value table also resident
-    (about 190 GB of tables with 128-byte slots).  Ordered mode.  The
-    engine's HBM budget bounds the slots, which are sized before the clock
-    starts.  Forms: plain (fixed batches, one in flight) at two batch sizes,
-    served (continuous batching, two in flight)."""
+    (about 185 GB of tables with 128-byte slots).  NewOrder and Payment
+    append rows; the append tables hold 100 orders per district at the start
+    (the ordered mode numbers at most 2^32 cells) and room for 2^22
+    transactions' inserts, and are truncated back to the preloaded rows
+    before each run.  Ordered mode.  The engine's HBM budget bounds the
+    slots, which are sized before the clock starts.  Forms: plain (fixed
+    batches, one in flight) at two batch sizes, served (continuous batching,
+    two in flight)."""
     from ltpmg_amd import MODE_O, Engine
-    from ltpmg_amd.engine import OeStream, oe_schema
-    W, cap, kv_rows = 1024, 1 << 20, args.c5_kv_rows
+    from ltpmg_amd.engine import OeStream, create_oe_tables
+    W, cap, pre, kv_rows = 1024, 1 << 22, 100, args.c5_kv_rows
     budget = int(args.c5_hbm_budget_gb * (1 << 30))
     e = Engine(n_slots=2, hbm_budget=budget)
-    schema = dict(oe_schema(W, cap))
-    schema[9] = (kv_rows, [100] * 10)
     t0 = time.perf_counter()
-    for t, (n, widths) in sorted(schema.items()):
-        e.create_table(t, n, widths)
-        e.fill_table(t, TABLE_SEED)
+    _, append = create_oe_tables(e, W, cap, TABLE_SEED, preload=pre)
+    e.create_table(9, kv_rows, [100] * 10)
+    e.fill_table(9, TABLE_SEED)
     tables_gb = e.stats_hbm() / (1 << 30)
     log(f"serving (large schema): {tables_gb:.1f} GiB of tables filled in {time.perf_counter() - t0:.1f} s")
-    pool = OeStream(W, cap, seed=BATCH_SEED + 9, byname_pct=15, table_seed=TABLE_SEED).next(args.c5_pool)
+    pool = OeStream(W, seed=BATCH_SEED + 9, byname_pct=15, table_seed=TABLE_SEED, preload=pre).next(args.c5_pool)
     out = {"partitions": W, "kv_rows": kv_rows, "tables_gib": tables_gb, "hbm_budget_gib": args.c5_hbm_budget_gb,
            "pool_txns": pool.n_txns, "pool_ops": pool.n_ops, "mode": "ordered", "rates": []}
     for rate in args.c5_rates:
-        n = int(max(1 << 17, min(1 << 23, rate * args.c5_seconds)))
+        n = int(max(1 << 17, min(1 << 22, rate * args.c5_seconds)))
         row = {"arrival_rate": rate, "arrivals": n}
         forms = [(f"plain_{b}", dict(max_batch=b, min_batch=b, n_inflight=1)) for b in args.c5_plain_batches]
         forms.append(("served", dict(max_batch=args.c5_served_max, max_wait_us=args.c5_wait_us, n_inflight=2)))
         for name, kw in forms:
+            for t, k in append.items():
+                e.truncate_table(t, k)
             r = e.serve(pool, arrival_rate=rate, n_arrivals=n, mode=MODE_O, seed=11, **kw)
             row[name] = {k: r[k] for k in ("committed_per_s", "p50_ms", "p99_ms", "p999_ms",
                                            "mean_batch_txns", "batches", "host_pack_s", "device_ms")}
@@ -323,32 +330,138 @@ def pct(xs, q):
     return xs[k]
 
 
-def cpu_baseline(batch, rows: int, width: int, n_cols: int, sample_txns: int, threads: int,
-                 runs: int) -> dict:
-    import oracle_ffi as orc
+def host_cpu() -> dict:
+    """The host's CPU model, core counts and this process's CPU quota."""
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            quota = None if q == "max" else int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "cgroup_cpus": quota}
+
+
+def _time_cpu(fn, runs: int) -> tuple[float, float]:
+    times = [fn() for _ in range(runs)]
+    return statistics.median(times), sum(times)
+
+
+def _sample(batch, s: int):
     from ltpmg_amd.engine import Batch
-    s = min(sample_txns, batch.n_txns)
     n_ops = int(batch.txn_off[s])
-    sb = Batch(batch.txn_off[: s + 1].copy(), batch.rows[:n_ops].copy(), batch.meta[:n_ops].copy(),
-               batch.args[:n_ops].copy())
-    im = orc.Image({0: (rows, [width] * n_cols)})  # untouched pages stay unmapped
-    col = (sb.meta >> 2) & 63
+    return Batch(batch.txn_off[: s + 1].copy(), batch.rows[:n_ops].copy(), batch.meta[:n_ops].copy(),
+                 batch.args[:n_ops].copy())
+
+
+def _cpu_legs(orc, im, batch, mode: int, stride: int, threads: int, one_core_txns: int,
+              serial_txns: int, runs: int) -> dict:
+    """The threaded restatement on `threads` threads (whole batch) and on 1
+    thread, and the oracle's serial replay, each on its own sample."""
+    out = {}
+    for name, s, th in (("all_cores", batch.n_txns, threads), ("one_core", min(one_core_txns, batch.n_txns), 1)):
+        sb = _sample(batch, s)
+        med, tot = _time_cpu(lambda: orc.cpu_pipeline(im, sb, mode, stride,
+                                                      n_threads=th)[2], runs)
+        out[name] = {"value": s / med, "unit": "txn/s", "threads": th, "txns": s, "ops": sb.n_ops,
+                     "median_s": med, "total_s": tot, "runs": runs}
+    s = min(serial_txns, batch.n_txns)
+    sb = _sample(batch, s)
+
+    def serial():
+        t0 = time.perf_counter()
+        orc.replay(im, sb, mode, stride)
+        return time.perf_counter() - t0
+    med, tot = _time_cpu(serial, runs)
+    out["serial_oracle"] = {"value": s / med, "unit": "txn/s", "threads": 1, "txns": s, "ops": sb.n_ops,
+                            "median_s": med, "total_s": tot, "runs": runs,
+                            "note": "orc_replay: commit rule then serial replay in TID order "
+                                    "(wall time of the ctypes call)"}
+    return out
+
+
+def cpu_baseline(batch, rows: int, width: int, n_cols: int, runs: int) -> dict:
+    """CPU baselines on the GPU box's host (rank 0, N=1), bounded samples:
+      C2  the headline key-value batch, ordered mode: the threaded C++
+          restatement of the reference pipeline on every core this process
+          may use (whole batch) and on 1 core (first 2^16 txns), and the
+          oracle's serial replay (first 2^16 txns);
+      C1  the reference's CPU configuration (100k rows x 10 int32, 8192 txns
+          x 10 ops, uniform keys, 50% READ of the row / 50% RMW +1, lowest
+          TID wins): the same three legs on the whole batch.
+    The reference's own CPU path cannot run here (its CPU executor is CUDA
+    host code, cuda/src/gpulauncher.cu:708-753); "port" is the restatement."""
+    import oracle_ffi as orc
+    from ltpmg_amd import MODE_A, OP_RMW_ADD, gen_kv
+    cpu = host_cpu()
+    threads = cpu["affinity"]  # every core this process may run on, within its CPU quota
+    if cpu["cgroup_cpus"]:
+        threads = max(1, min(threads, int(np.ceil(cpu["cgroup_cpus"]))))
+    # C2: only the cells the batch touches are filled (untouched pages stay unmapped)
+    im = orc.Image({0: (rows, [width] * n_cols)})
+    col = (batch.meta >> 2) & 63
     for c in range(n_cols):
-        r = np.unique(sb.rows[col == c])
-        im.fill_cells(TABLE_SEED, 0, c, r)
-    times = []
-    for _ in range(runs):
-        _, _, secs = orc.cpu_pipeline(im, sb, orc.MODE_O, 128, n_threads=threads)
-        times.append(secs)
-    secs = statistics.median(times)
-    what = "the whole batch" if s == batch.n_txns else f"the first {s} txns of the batch"
-    return {"value": s / secs, "unit": "txn/s", "cores": threads, "kind": "port",
-            "sample": f"{what} ({s} txns, {n_ops} ops), ordered mode, the same table seed; "
-                      f"C++ restatement of the reference pipeline (histogram, scan, scatter, "
-                      f"per-row TID sort, per-row apply) timed around CC + execute only; "
-                      f"median of {runs} runs ({sum(times):.1f} s of CPU pipeline time), "
-                      f"{platform.processor() or 'host'}",
-            "seconds": secs}
+        im.fill_cells(TABLE_SEED, 0, c, np.unique(batch.rows[col == c]))
+    c2 = _cpu_legs(orc, im, batch, orc.MODE_O, 128, threads, 1 << 16, 1 << 16, runs)
+    del im
+    c1_batch = gen_kv(8192, 100_000, 10, ops=10, read_pct=50, write_kind=OP_RMW_ADD, read_all_cols=True,
+                      rmw_delta=1, seed=BATCH_SEED + 11)
+    im1 = orc.Image({0: (100_000, [4] * 10)}).fill(TABLE_SEED)
+    c1 = _cpu_legs(orc, im1, c1_batch, MODE_A, 4, threads, 8192, 8192, max(runs, 9))
+    head = c2["all_cores"]
+    return {"value": head["value"], "unit": "txn/s", "cores": threads, "kind": "port",
+            "sample": f"C2: the whole headline batch ({head['txns']} txns, {head['ops']} ops), ordered "
+                      f"mode, same table seed; C++ restatement of the reference pipeline (histogram, "
+                      f"scan, scatter, per-row TID sort, per-row apply) on {threads} threads, timed "
+                      f"around CC + execute only, median of {runs} runs; one-core and serial-oracle "
+                      f"legs and the C1 configuration beside it",
+            **cpu, "c2": c2, "c1": c1}
+
+
+def order_entry_c3(args) -> dict:
+    """Plain batch on the order-entry schema: W = 64 partitions, one batch of
+    --c3-txns transactions of the 45% NewOrder (5-15 lines, 1% remote) / 43%
+    Payment / 12% read-only mix (15% of customer lookups by last name), both
+    modes.  NewOrder and Payment append rows, so before each step the append
+    tables are truncated back to their preloaded rows (outside the timed
+    execute + wait)."""
+    from ltpmg_amd import MODE_A, MODE_O, Engine
+    from ltpmg_amd.engine import OeStream, create_oe_tables
+    W, n = 64, args.c3_txns
+    e = Engine()
+    _, append = create_oe_tables(e, W, n, TABLE_SEED)
+    batch = OeStream(W, seed=BATCH_SEED + 3, byname_pct=15, table_seed=TABLE_SEED).next(n)
+    e.submit(batch)
+    out = {"partitions": W, "txns": batch.n_txns, "ops": batch.n_ops, "modes": {}}
+    for name, mode in (("ordered", MODE_O), ("lowest_tid_wins", MODE_A)):
+        lat = []
+        for k in range(args.warmup + args.c3_steps):
+            for t, rows in append.items():
+                e.truncate_table(t, rows)
+            t0 = time.perf_counter()
+            e.execute(mode)
+            e.wait()
+            if k >= args.warmup:
+                lat.append(time.perf_counter() - t0)
+        st = e.stats()
+        med = statistics.median(lat)
+        out["modes"][name] = {"committed": st["committed"], "inserted_rows": st["inserted"],
+                              "committed_txn_per_s": st["committed"] / med, "step_ms_median": 1e3 * med,
+                              "device_ms": st["exec_ms"], "steps": len(lat)}
+        log(f"C3 {name}: {st['committed']}/{batch.n_txns} committed, {st['inserted']} rows appended, "
+            f"{1e3 * med:.3f} ms per step, {st['committed'] / med:.4g} committed txn/s")
+    e.close()
+    return out
 
 
 def main() -> int:
@@ -360,7 +473,6 @@ def main() -> int:
     ap.add_argument("--txns", type=int, default=1 << 20)
     ap.add_argument("--ops", type=int, default=16)
     ap.add_argument("--theta", type=float, default=0.99)
-    ap.add_argument("--cpu-sample-txns", type=int, default=1 << 20)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8, 2e8],
@@ -379,6 +491,9 @@ def main() -> int:
     ap.add_argument("--c5-plain-batches", type=int, nargs="*", default=[1 << 15, 1 << 17])
     ap.add_argument("--c5-served-max", type=int, default=1 << 17)
     ap.add_argument("--c5-wait-us", type=float, default=1000.0)
+    ap.add_argument("--no-c3", action="store_true", help="skip the order-entry plain batch")
+    ap.add_argument("--c3-txns", type=int, default=1 << 20)
+    ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-batching", action="store_true",
                     help="skip the batching-form stream comparison")
     ap.add_argument("--batching-batches", type=int, default=200)
@@ -472,6 +587,9 @@ def main() -> int:
         serving = serve_compare(e, batch, args)
 
     e.close()
+    order_entry = None
+    if not args.no_c3:
+        order_entry = order_entry_c3(args)
     batching = None
     if not args.no_batching:
         batching = batching_compare(args)
@@ -481,9 +599,12 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(batch, args.rows, width, n_cols, args.cpu_sample_txns, threads, args.cpu_runs)
+        log(f"cpu baseline: {host_cpu()}")
+        cpu = cpu_baseline(batch, args.rows, width, n_cols, args.cpu_runs)
+        log(f"cpu baseline: C2 {cpu['c2']['all_cores']['value']:.4g} txn/s on {cpu['cores']} threads, "
+            f"{cpu['c2']['one_core']['value']:.4g} on 1, serial oracle {cpu['c2']['serial_oracle']['value']:.4g}; "
+            f"C1 {cpu['c1']['all_cores']['value']:.4g} / {cpu['c1']['one_core']['value']:.4g} / "
+            f"{cpu['c1']['serial_oracle']['value']:.4g}")
 
     modes = {}
     for name, r in res.items():
@@ -512,7 +633,8 @@ def main() -> int:
         "upload_ms": upload_ms, "upload_ms_first": uploads[0][1], "upload_wall_s": upload_wall,
         "upload_bytes": int(batch.n_ops * 14 + (batch.n_txns + 1) * 4),
         "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
-        "batching": batching, "serving_large_schema": serving_large, "batch_counts": counts,
+        "order_entry_plain": order_entry, "batching": batching, "serving_large_schema": serving_large,
+        "batch_counts": counts,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -175,6 +175,11 @@ int ltpmg_create_append_table(ltpmg_engine *e, uint32_t table_id, const ltpmg_ta
 int ltpmg_table_rows(ltpmg_engine *e, uint32_t table_id, uint32_t *out);
 /* Presence bytes (1 present, 0 deleted or unallocated) of rows [row0, row0 + n_rows). */
 int ltpmg_read_presence(ltpmg_engine *e, uint32_t table_id, uint32_t row0, uint32_t n_rows, uint8_t *dst);
+/* Sets an append table's cursor to n_rows (<= capacity) and clears the
+ * presence of rows [n_rows, capacity): the rows appended after that point
+ * are gone and the next INSERT takes row n_rows.  Waits for every slot.  No
+ * counterpart in the reference (its tables are fixed-size). */
+int ltpmg_truncate_table(ltpmg_engine *e, uint32_t table_id, uint32_t n_rows);
 /* Order-independent digest of a table: sum over cells of a hash of
  * (table, row, col, cell bytes).  Replaces nothing in the reference (it has
  * no readback, gpudatabase.cu:240-339 frees without copying back). */

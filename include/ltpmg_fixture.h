@@ -10,15 +10,17 @@
  *
  * Layout (little-endian):
  *   magic "LTPMGFX\0" | u32 version | u32 header bytes
- *   u64 seed | u32 mode | u32 n_tables
+ *   u64 seed | u32 mode | u32 n_tables | u32 append_mask (version 2 on)
  *   per table: u32 n_rows | u32 n_cols | u32 col_bytes[n_cols]
+ *              | u32 n_present (version 2 on, tables in append_mask only)
  *   u32 n_txns | u32 n_ops | u64 checksum of the arrays
  *   txn_off u32[n_txns + 1] | rows u32[n_ops] | meta u16[n_ops] | zeros
  *   until the meta bytes are a multiple of 8 | args u64[n_ops]
  * The checksum chains mix64 over the arrays' bytes as little-endian 64-bit
  * words (the last word zero padded), starting from 0x9E3779B97F4A7C15.
- * A reader rejects another magic, a newer version, a short file or a
- * checksum mismatch with LTPMG_EINVAL.
+ * Version 1 files (no append tables) read with append_mask 0.  A reader
+ * rejects another magic, a newer version, a short file or a checksum
+ * mismatch with LTPMG_EINVAL.
  */
 #ifndef LTPMG_FIXTURE_H
 #define LTPMG_FIXTURE_H
@@ -31,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LTPMG_FIXTURE_VERSION 1
+#define LTPMG_FIXTURE_VERSION 2
 #define LTPMG_FIXTURE_NO_MODE 0xFFFFFFFFu
 
 typedef struct {
@@ -39,6 +41,8 @@ typedef struct {
     uint32_t mode;     /* LTPMG_MODE_A, LTPMG_MODE_O or LTPMG_FIXTURE_NO_MODE */
     uint32_t n_tables; /* table ids 0 .. n_tables-1 */
     ltpmg_table_desc tables[LTPMG_MAX_TABLES];
+    uint32_t append_mask;                 /* bit t: table t is an append table */
+    uint32_t n_present[LTPMG_MAX_TABLES]; /* an append table's rows at creation */
 } ltpmg_fixture_info;
 
 int ltpmg_fixture_write(const char *path, const ltpmg_fixture_info *info, const ltpmg_batch *b);

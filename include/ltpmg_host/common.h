@@ -26,6 +26,7 @@ DECLARE_uint32(kv_ops);         // KV benchmarks: ops per txn
 DECLARE_uint32(read_percent);   // KV benchmarks: percent READ
 DECLARE_string(dump_path);      // if set: write batch, results and image digest of epoch 0
 DECLARE_uint32(append_cap);     // order-entry append tables, in transactions
+DECLARE_uint32(oe_preload);     // order-entry orders per district at table creation
 DECLARE_string(schema_dir);     // where <benchmark>.json schemas are looked up
 
 void initialize_dependency(int argc, char **argv);

@@ -7,7 +7,7 @@
 // then hands this build's GPU library an object it reads correctly.  The
 // settings the reference has no field for live here instead, filled from the
 // flags of common.h, and reach the GPU library through the environment
-// (LTPMG_CC_MODE, LTPMG_SEED, LTPMG_APPEND_CAP), which the driver sets from
+// (LTPMG_CC_MODE, LTPMG_SEED, LTPMG_APPEND_CAP, LTPMG_OE_PRELOAD), which the driver sets from
 // the same flags.
 #pragma once
 
@@ -27,6 +27,7 @@ struct RunOptions {
     uint32_t read_percent = 50;
     uint32_t kv_sub_txn_size = 0;       // upper bound on device ops per batch
     uint32_t oe_append_cap = 1u << 20;  // order-entry append tables, in transactions
+    uint32_t oe_preload = 3000;         // order-entry orders per district at table creation
 };
 
 // The settings of this run: taken from the flags on first use.
@@ -39,7 +40,7 @@ bool is_kv(const Param &p);
 bool is_oe(const Param &p);
 
 // Exports the settings the GPU library reads (LTPMG_CC_MODE, LTPMG_SEED,
-// LTPMG_APPEND_CAP) into the environment.
+// LTPMG_APPEND_CAP, LTPMG_OE_PRELOAD) into the environment.
 void export_gpu_options();
 
 }  // namespace ltpmg_host

@@ -49,6 +49,11 @@ typedef struct ltpmg_stream ltpmg_stream;
  * HBM).  Without it only the commit flags are kept, and results land in a
  * buffer that each launch group overwrites. */
 #define LTPMG_STREAM_KEEP_RESULTS 0x1u
+/* Every run first truncates (ltpmg_truncate_table) each append table the
+ * stream INSERTs into back to its row count when the stream was created, so
+ * repeated runs (timing, ltpmg_stream_tune) append the same rows instead of
+ * filling the table.  Rows below that count keep their presence. */
+#define LTPMG_STREAM_REWIND 0x2u
 
 /* Plan flags */
 #define LTPMG_PLAN_SYNC_EACH 0x1u /* host waits after every batch (plain form) */

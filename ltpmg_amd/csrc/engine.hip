@@ -973,6 +973,35 @@ int ltpmg_table_rows(ltpmg_engine *e, uint32_t table_id, uint32_t *out) {
     return LTPMG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Sets an append table's cursor to n_rows and clears the presence of every
+// row from there on (caller holds e->mu, the slots are idle).
+int truncate_locked(ltpmg_engine *e, uint32_t table_id, uint32_t n_rows) {
+    Table &t = e->tables[table_id];
+    if (n_rows < t.n_rows) HIP_TRY(e, hipMemset(t.present + n_rows, 0, t.n_rows - n_rows));
+    HIP_TRY(e, hipMemcpy(e->d_cursor + table_id, &n_rows, 4, hipMemcpyHostToDevice));
+    t.cursor_bound = n_rows;
+    return LTPMG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ltpmg_truncate_table(ltpmg_engine *e, uint32_t table_id, uint32_t n_rows) {
+    if (!e) return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    if (table_id >= LTPMG_MAX_TABLES || !e->tables[table_id].live)
+        return fail(e, LTPMG_ESTATE, "table %u does not exist", table_id);
+    Table &t = e->tables[table_id];
+    if (!t.present) return fail(e, LTPMG_EINVAL, "table %u is not an append table", table_id);
+    if (n_rows > t.n_rows) return fail(e, LTPMG_EINVAL, "%u rows > capacity %u", n_rows, t.n_rows);
+    for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+    return truncate_locked(e, table_id, n_rows);
+}
+
 int ltpmg_read_presence(ltpmg_engine *e, uint32_t table_id, uint32_t row0, uint32_t n_rows, uint8_t *dst) {
     if (!e || (!dst && n_rows)) return LTPMG_EINVAL;
     std::lock_guard<std::mutex> g(e->mu);
@@ -1502,6 +1531,7 @@ struct ltpmg_stream {
     // table, and the row-op scratch of one launch group
     std::vector<RowBufs> rows;
     uint32_t stream_inserts[LTPMG_MAX_TABLES] = {};
+    uint32_t rewind_rows[LTPMG_MAX_TABLES] = {};  // LTPMG_STREAM_REWIND: cursors at creation
     uint8_t *d_found = nullptr;
     uint16_t *d_meta_eff = nullptr;
     uint32_t *d_ins_cnt = nullptr, *d_ins_base = nullptr;
@@ -1694,6 +1724,13 @@ int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan 
         for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
             Table &tb = e->tables[t];
             if (!s.stream_inserts[t]) continue;
+            if ((s.flags & LTPMG_STREAM_REWIND) && cur[t] != s.rewind_rows[t]) {
+                if (s.rewind_rows[t] > tb.n_rows)
+                    return fail(e, LTPMG_ESTATE, "table %u changed since the stream was created", t);
+                int rc = truncate_locked(e, t, s.rewind_rows[t]);
+                if (rc != LTPMG_OK) return rc;
+                cur[t] = s.rewind_rows[t];
+            }
             if ((uint64_t)cur[t] + s.stream_inserts[t] > tb.n_rows)
                 return fail(e, LTPMG_EFULL, "table %u: the stream inserts up to %u rows, %u of %u rows are left", t,
                             s.stream_inserts[t], tb.n_rows - cur[t], tb.n_rows);
@@ -1781,7 +1818,7 @@ int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *bat
     *out = nullptr;
     std::lock_guard<std::mutex> g(e->mu);
     HIP_TRY(e, hipSetDevice(e->device));
-    if (flags & ~LTPMG_STREAM_KEEP_RESULTS) return fail(e, LTPMG_EINVAL, "stream flags 0x%x", flags);
+    if (flags & ~(LTPMG_STREAM_KEEP_RESULTS | LTPMG_STREAM_REWIND)) return fail(e, LTPMG_EINVAL, "stream flags 0x%x", flags);
     int rc = validate_batch(e, all, 0xFFFFFFFFull);
     if (rc != LTPMG_OK) return rc;
     ltpmg_stream *s = new ltpmg_stream();
@@ -1823,6 +1860,18 @@ int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *bat
         uint32_t ins[LTPMG_MAX_TABLES];
         scan_row_ops(e, &one, s->rows[b], ins);
         for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) s->stream_inserts[t] += ins[t];
+    }
+    if (flags & LTPMG_STREAM_REWIND) {  // the cursors every run starts from
+        for (Slot &sl : e->slots) {
+            if (hipStreamSynchronize(sl.stream) != hipSuccess) {
+                delete s;
+                return fail(e, LTPMG_EHIP, "hipStreamSynchronize failed");
+            }
+        }
+        if (hipMemcpy(s->rewind_rows, e->d_cursor, sizeof s->rewind_rows, hipMemcpyDeviceToHost) != hipSuccess) {
+            delete s;
+            return fail(e, LTPMG_EHIP, "reading the append cursors failed");
+        }
     }
     const uint64_t n_ops = all->n_ops, n_txns = all->n_txns;
     // per-batch offsets, each batch starting at 0

@@ -87,6 +87,7 @@ ENGINE_API = {
     "ltpmg_table_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "ltpmg_read_presence": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_void_p]),
+    "ltpmg_truncate_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "ltpmg_fill_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
     "ltpmg_load_column": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_void_p]),
@@ -124,6 +125,7 @@ class _StreamPlan(C.Structure):
 
 
 STREAM_KEEP_RESULTS = 0x1
+STREAM_REWIND = 0x2
 PLAN_SYNC_EACH = 0x1
 
 # Every symbol include/ltpmg_stream.h declares (also in libltpmg_hip.so).
@@ -173,14 +175,17 @@ WORKLOAD_API = {
 
 class _OeSpec(C.Structure):
     _fields_ = [("n_txns", C.c_uint32), ("warehouses", C.c_uint32), ("neworder_pct", C.c_uint32),
-                ("payment_pct", C.c_uint32), ("lines_min", C.c_uint32), ("lines_max", C.c_uint32),
-                ("remote_bp", C.c_uint32), ("append_cap", C.c_uint32),
-                ("append_next", C.c_uint32 * 4), ("seed", C.c_uint64),
-                ("byname_pct", C.c_uint32), ("table_seed", C.c_uint64)]
+                ("payment_pct", C.c_uint32), ("delivery_pct", C.c_uint32), ("lines_min", C.c_uint32),
+                ("lines_max", C.c_uint32), ("remote_bp", C.c_uint32), ("preload", C.c_uint32),
+                ("byname_pct", C.c_uint32), ("seed", C.c_uint64), ("table_seed", C.c_uint64)]
+
+
+OE_APPEND_MASK = 0x78  # LTPMG_OE_APPEND_MASK: neworder, history, order, orderline
+OE_PRELOAD = 3000      # the reference's orders per district
 
 
 OE_API = {
-    "ltpmg_oe_schema": (C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
+    "ltpmg_oe_schema": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "ltpmg_oe_last_name_index": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p]),
     "ltpmg_gen_oe_max_ops": (C.c_uint64, [C.POINTER(_OeSpec)]),
     "ltpmg_gen_oe": (C.c_int, [C.POINTER(_OeSpec), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -189,7 +194,8 @@ OE_API = {
 
 class _FixtureInfo(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("mode", C.c_uint32), ("n_tables", C.c_uint32),
-                ("tables", _TableDesc * MAX_TABLES)]
+                ("tables", _TableDesc * MAX_TABLES), ("append_mask", C.c_uint32),
+                ("n_present", C.c_uint32 * MAX_TABLES)]
 
 
 FIXTURE_NO_MODE = 0xFFFFFFFF
@@ -304,26 +310,55 @@ def gen_kv(n_txns: int, n_rows: int, n_cols: int, *, ops: int | tuple[int, int] 
     return Batch(off, rows[:k].copy(), metas[:k].copy(), args[:k].copy())
 
 
-def oe_schema(warehouses: int, append_cap: int) -> dict[int, tuple[int, list[int]]]:
-    """Order-entry tables (include/ltpmg_oe.h): {table id: (rows, column bytes)}."""
+def _oe_descs(warehouses: int, append_cap: int, preload: int):
     descs = (_TableDesc * 9)()
-    rc = host_lib().ltpmg_oe_schema(warehouses, append_cap, C.cast(descs, C.c_void_p))
+    present = (C.c_uint32 * 9)()
+    rc = host_lib().ltpmg_oe_schema(warehouses, append_cap, preload, C.cast(descs, C.c_void_p),
+                                    C.cast(present, C.c_void_p))
     if rc != 0:
         raise LtpmgError("ltpmg_oe_schema", rc, "bad order-entry sizes")
+    return descs, present
+
+
+def oe_schema(warehouses: int, append_cap: int, preload: int = OE_PRELOAD) -> dict[int, tuple[int, list[int]]]:
+    """Order-entry tables (include/ltpmg_oe.h): {table id: (rows, column bytes)}."""
+    descs, _ = _oe_descs(warehouses, append_cap, preload)
     return {t: (d.n_rows, list(d.col_bytes[:d.n_cols])) for t, d in enumerate(descs)}
 
 
-class OeStream:
-    """Successive order-entry batches; append rows continue from batch to batch."""
+def oe_append(warehouses: int, append_cap: int, preload: int = OE_PRELOAD) -> dict[int, int]:
+    """The append tables of the order-entry schema: {table id: rows present at creation}."""
+    _, present = _oe_descs(warehouses, append_cap, preload)
+    return {t: int(present[t]) for t in range(9) if (OE_APPEND_MASK >> t) & 1}
 
-    def __init__(self, warehouses: int, append_cap: int, *, neworder_pct: int = 45,
-                 payment_pct: int = 43, lines: tuple[int, int] = (5, 15), remote_bp: int = 100,
-                 seed: int = 1, byname_pct: int = 0, table_seed: int = 0):
+
+def create_oe_tables(e: "Engine", warehouses: int, append_cap: int, seed: int,
+                     preload: int = OE_PRELOAD, first_table: int = 0):
+    """Creates and fills the nine order-entry tables on an engine; returns
+    (schema, append) as oe_schema and oe_append give them."""
+    schema = oe_schema(warehouses, append_cap, preload)
+    append = oe_append(warehouses, append_cap, preload)
+    for t, (n, widths) in sorted(schema.items()):
+        if t in append:
+            e.create_append_table(first_table + t, n, widths, append[t])
+        else:
+            e.create_table(first_table + t, n, widths)
+        e.fill_table(first_table + t, seed)
+    return schema, append
+
+
+class OeStream:
+    """Successive order-entry batches (each with its own seed)."""
+
+    def __init__(self, warehouses: int, *, neworder_pct: int = 45, payment_pct: int = 43,
+                 delivery_pct: int = 0, lines: tuple[int, int] = (5, 15), remote_bp: int = 100,
+                 preload: int = OE_PRELOAD, seed: int = 1, byname_pct: int = 0, table_seed: int = 0):
         """byname_pct of Payments and Orderstatus select the customer by last
-        name, resolved against tables filled from table_seed."""
-        self.spec = _OeSpec(0, warehouses, neworder_pct, payment_pct, lines[0], lines[1], remote_bp,
-                            append_cap, (C.c_uint32 * 4)(0, 0, 0, 0), seed & 0xFFFFFFFFFFFFFFFF,
-                            byname_pct, table_seed & 0xFFFFFFFFFFFFFFFF)
+        name, resolved against tables filled from table_seed; preload is the
+        orders per district the tables were created with."""
+        self.spec = _OeSpec(0, warehouses, neworder_pct, payment_pct, delivery_pct, lines[0], lines[1],
+                            remote_bp, preload, byname_pct, seed & 0xFFFFFFFFFFFFFFFF,
+                            table_seed & 0xFFFFFFFFFFFFFFFF)
         self.batches = 0
 
     def next(self, n_txns: int, n_threads: int = 0) -> Batch:
@@ -355,8 +390,9 @@ def oe_last_name_index(warehouses: int, table_seed: int) -> np.ndarray:
 
 
 def write_fixture(path: str, batch: Batch, *, seed: int, schema: dict[int, tuple[int, list[int]]],
-                  mode: int | None = None) -> None:
-    """Writes a versioned binary batch fixture (include/ltpmg_fixture.h)."""
+                  mode: int | None = None, append: dict[int, int] | None = None) -> None:
+    """Writes a versioned binary batch fixture (include/ltpmg_fixture.h);
+    append = {table id: rows present at creation} of the append tables."""
     info = _FixtureInfo()
     info.seed = seed & 0xFFFFFFFFFFFFFFFF
     info.mode = FIXTURE_NO_MODE if mode is None else mode
@@ -366,6 +402,9 @@ def write_fixture(path: str, batch: Batch, *, seed: int, schema: dict[int, tuple
         info.tables[t].n_cols = len(widths)
         for c, w in enumerate(widths):
             info.tables[t].col_bytes[c] = w
+    for t, k in (append or {}).items():
+        info.append_mask |= 1 << t
+        info.n_present[t] = k
     b = batch.normalized()
     rc = host_lib().ltpmg_fixture_write(str(path).encode(), C.byref(info), C.byref(b.c_struct()))
     if rc != 0:
@@ -373,7 +412,7 @@ def write_fixture(path: str, batch: Batch, *, seed: int, schema: dict[int, tuple
 
 
 def read_fixture(path: str) -> tuple[Batch, dict]:
-    """(batch, {"seed", "mode", "schema"}) from a binary batch fixture."""
+    """(batch, {"seed", "mode", "schema", "append"}) from a binary batch fixture."""
     lib = host_lib()
     info = _FixtureInfo()
     nt, no = C.c_uint32(0), C.c_uint32(0)
@@ -388,7 +427,8 @@ def read_fixture(path: str) -> tuple[Batch, dict]:
     schema = {t: (info.tables[t].n_rows, list(info.tables[t].col_bytes[:info.tables[t].n_cols]))
               for t in range(info.n_tables) if info.tables[t].n_cols}
     mode = None if info.mode == FIXTURE_NO_MODE else info.mode
-    return b, {"seed": info.seed, "mode": mode, "schema": schema}
+    append = {t: int(info.n_present[t]) for t in range(info.n_tables) if (info.append_mask >> t) & 1}
+    return b, {"seed": info.seed, "mode": mode, "schema": schema, "append": append}
 
 
 def zeta(n: int, theta: float) -> float:
@@ -455,6 +495,10 @@ class Engine:
         v = C.c_uint32(0)
         self._check("ltpmg_table_rows", self._lib.ltpmg_table_rows(self._h, table_id, C.byref(v)))
         return v.value
+
+    def truncate_table(self, table_id: int, n_rows: int) -> None:
+        """An append table's cursor back to n_rows; rows from there on are gone."""
+        self._check("ltpmg_truncate_table", self._lib.ltpmg_truncate_table(self._h, table_id, n_rows))
 
     def read_presence(self, table_id: int, row0: int = 0, n_rows: int | None = None) -> np.ndarray:
         n = self.tables[table_id][0] - row0 if n_rows is None else n_rows
@@ -674,7 +718,9 @@ class Stream:
     """A stream of batches resident in HBM (the batching form)."""
 
     def __init__(self, engine: "Engine", batches: list[Batch], modes: list[int] | int,
-                 keep_results: bool = False):
+                 keep_results: bool = False, rewind: bool = False):
+        """rewind: every run first truncates the append tables the stream
+        inserts into back to their rows at creation (LTPMG_STREAM_REWIND)."""
         self.engine = engine
         self.batch_txns = np.array([b.n_txns for b in batches], np.uint32)
         self.batch_ops = np.array([b.n_ops for b in batches], np.int64)
@@ -687,7 +733,8 @@ class Stream:
         engine._check("ltpmg_stream_create",
                       lib.ltpmg_stream_create(engine._h, len(batches), _ptr(self.batch_txns),
                                               _ptr(self.modes), C.byref(allb.c_struct()),
-                                              STREAM_KEEP_RESULTS if keep_results else 0,
+                                              (STREAM_KEEP_RESULTS if keep_results else 0) |
+                                              (STREAM_REWIND if rewind else 0),
                                               C.byref(h)))
         self._h = h
 

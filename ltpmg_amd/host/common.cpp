@@ -25,7 +25,8 @@ DEFINE_uint32(kv_field_bytes, 100, "KV: bytes per field (KV_C1 uses 4).");
 DEFINE_uint32(kv_ops, 16, "KV / KV_C1: logical ops per txn.");
 DEFINE_uint32(read_percent, 50, "KV / KV_C1: percent of ops that READ.");
 DEFINE_string(dump_path, "", "write epoch 0's batch, results and image digest under this prefix.");
-DEFINE_uint32(append_cap, 1u << 20, "TPCC_PART / OE: capacity of the append-only tables, in transactions.");
+DEFINE_uint32(append_cap, 1u << 20, "order entry: rows the append-only tables can take, in transactions.");
+DEFINE_uint32(oe_preload, 3000, "order entry: orders per district at table creation (at most 3000).");
 DEFINE_string(schema_dir, "", "directory of <benchmark>.json table schemas when Param's path is absent "
                              "(default: the schemas/ directory beside the library).");
 DEFINE_uint32(minloglevel, 0, "drop log lines below this severity (0 INFO .. 3 FATAL).");

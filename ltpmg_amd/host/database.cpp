@@ -97,9 +97,11 @@ void Database::malloc_global_row(std::shared_ptr<Param> param) {
                 LOG(FATAL) << filepath << ": table " << i << " is " << rows[i].get_table_name() << ", expected "
                            << kNames[i];
         ltpmg_table_desc d[LTPMG_OE_TABLES];
-        if (n_tables != LTPMG_OE_TABLES || ltpmg_oe_schema(param->warehouse_size, ltpmg_host::options().oe_append_cap, d))
+        const ltpmg_host::RunOptions &opt = ltpmg_host::options();
+        if (n_tables != LTPMG_OE_TABLES ||
+            ltpmg_oe_schema(param->warehouse_size, opt.oe_append_cap, opt.oe_preload, d, nullptr))
             LOG(FATAL) << "order-entry schema: " << param->warehouse_size << " partitions, append capacity "
-                       << ltpmg_host::options().oe_append_cap;
+                       << opt.oe_append_cap << ", " << opt.oe_preload << " orders per district";
         for (uint32_t i = 0; i < n_tables; ++i) {
             tables_for_GPU_info[i].table_size = d[i].n_rows;
             tables_for_GPU_info[i].int_size = d[i].n_cols;

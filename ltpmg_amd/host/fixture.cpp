@@ -71,10 +71,13 @@ bool read_header(std::FILE *f, ltpmg_fixture_info *info, uint32_t *n_txns, uint3
     if (!get(f, &in.seed, 8) || !get(f, &in.mode, 4) || !get(f, &in.n_tables, 4) ||
         in.n_tables > LTPMG_MAX_TABLES)
         return false;
+    if (version >= 2 && (!get(f, &in.append_mask, 4) || (in.append_mask >> in.n_tables) != 0)) return false;
     for (uint32_t t = 0; t < in.n_tables; ++t) {
         ltpmg_table_desc &d = in.tables[t];
         if (!get(f, &d.n_rows, 4) || !get(f, &d.n_cols, 4) || d.n_cols > LTPMG_MAX_COLS ||
             !get(f, d.col_bytes, (size_t)d.n_cols * 4))
+            return false;
+        if (((in.append_mask >> t) & 1u) && (!get(f, &in.n_present[t], 4) || in.n_present[t] > d.n_rows))
             return false;
     }
     uint32_t nt = 0, no = 0;
@@ -92,12 +95,15 @@ bool read_header(std::FILE *f, ltpmg_fixture_info *info, uint32_t *n_txns, uint3
 
 extern "C" int ltpmg_fixture_write(const char *path, const ltpmg_fixture_info *info, const ltpmg_batch *b) {
     if (!path || !info || !b || !b->txn_off || (b->n_ops && (!b->rows || !b->meta || !b->args)) ||
-        info->n_tables > LTPMG_MAX_TABLES)
+        info->n_tables > LTPMG_MAX_TABLES || (info->append_mask >> info->n_tables) != 0)
         return LTPMG_EINVAL;
     for (uint32_t t = 0; t < info->n_tables; ++t)
-        if (info->tables[t].n_cols > LTPMG_MAX_COLS) return LTPMG_EINVAL;
-    uint32_t header = 8 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8;
-    for (uint32_t t = 0; t < info->n_tables; ++t) header += 8 + 4 * info->tables[t].n_cols;
+        if (info->tables[t].n_cols > LTPMG_MAX_COLS ||
+            (((info->append_mask >> t) & 1u) && info->n_present[t] > info->tables[t].n_rows))
+            return LTPMG_EINVAL;
+    uint32_t header = 8 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 8;
+    for (uint32_t t = 0; t < info->n_tables; ++t)
+        header += 8 + 4 * info->tables[t].n_cols + (((info->append_mask >> t) & 1u) ? 4 : 0);
     File fl;
     fl.f = std::fopen(path, "wb");
     if (!fl.f) return LTPMG_EINVAL;
@@ -105,10 +111,12 @@ extern "C" int ltpmg_fixture_write(const char *path, const ltpmg_fixture_info *i
     const uint64_t sum = arrays_checksum(b);
     static const uint8_t zeros[8] = {};
     bool ok = put(fl.f, kMagic, 8) && put(fl.f, &version, 4) && put(fl.f, &header, 4) &&
-              put(fl.f, &info->seed, 8) && put(fl.f, &info->mode, 4) && put(fl.f, &info->n_tables, 4);
+              put(fl.f, &info->seed, 8) && put(fl.f, &info->mode, 4) && put(fl.f, &info->n_tables, 4) &&
+              put(fl.f, &info->append_mask, 4);
     for (uint32_t t = 0; ok && t < info->n_tables; ++t) {
         const ltpmg_table_desc &d = info->tables[t];
         ok = put(fl.f, &d.n_rows, 4) && put(fl.f, &d.n_cols, 4) && put(fl.f, d.col_bytes, (size_t)d.n_cols * 4);
+        if (ok && ((info->append_mask >> t) & 1u)) ok = put(fl.f, &info->n_present[t], 4);
     }
     ok = ok && put(fl.f, &b->n_txns, 4) && put(fl.f, &b->n_ops, 4) && put(fl.f, &sum, 8) &&
          put(fl.f, b->txn_off, ((size_t)b->n_txns + 1) * 4) && put(fl.f, b->rows, (size_t)b->n_ops * 4) &&

@@ -13,14 +13,17 @@
 //   LTPMG_SEED        table fill and payload seed (decimal or 0x hex;
 //                     default 0x4C54504D47000001)
 //   LTPMG_APPEND_CAP  order-entry append tables, in transactions (2^20)
+//   LTPMG_OE_PRELOAD  order-entry orders per district at creation (3000)
 // The reference's GPUScheduler singleton (gpuscheduler.cuh:54) becomes the
 // process-wide engine of the device (ltpmg_shared_engine), which host code
 // beside these functions can also open to read what they ran.  Failures keep
 // the reference's behaviour of its CHECK macro (print and exit,
 // cuda/include/define.cuh:57-65).
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "database.h"
 #include "ltpmg.h"
@@ -40,6 +43,7 @@ struct ShimState {
     uint32_t mode = LTPMG_MODE_O;
     uint64_t seed = 0x4C54504D47000001ull;
     uint32_t append_cap = 1u << 20;
+    uint32_t preload = 3000;
 } g;
 
 [[noreturn]] void die(const char *fmt, ...) {
@@ -67,6 +71,7 @@ void read_options() {
     }
     if (const char *s = std::getenv("LTPMG_SEED")) g.seed = std::strtoull(s, nullptr, 0);
     if (const char *c = std::getenv("LTPMG_APPEND_CAP")) g.append_cap = (uint32_t)std::strtoul(c, nullptr, 0);
+    if (const char *c = std::getenv("LTPMG_OE_PRELOAD")) g.preload = (uint32_t)std::strtoul(c, nullptr, 0);
 }
 
 bool is_oe(const Param &p) { return p.benchmark == "TPCC_PART" || p.benchmark == "TPCC_ALL" || p.benchmark == "OE"; }
@@ -107,9 +112,13 @@ void transfer_database_to_GPU(std::shared_ptr<Param> param, Global_Table_Info *i
     g.info.assign(info, info + param->table_cnt);
     if (is_oe(*param)) {  // include/ltpmg_oe.h: created and filled on the device
         ltpmg_table_desc d[LTPMG_OE_TABLES];
-        check(oe::schema(param->warehouse_size, g.append_cap, d), "order-entry schema");
+        uint32_t present[LTPMG_OE_TABLES];
+        check(oe::schema(param->warehouse_size, g.append_cap, g.preload, d, present), "order-entry schema");
         for (uint32_t t = 0; t < LTPMG_OE_TABLES; ++t) {
-            check(ltpmg_create_table(g.e, t, &d[t]), "ltpmg_create_table");
+            if ((LTPMG_OE_APPEND_MASK >> t) & 1u)
+                check(ltpmg_create_append_table(g.e, t, &d[t], present[t]), "ltpmg_create_append_table");
+            else
+                check(ltpmg_create_table(g.e, t, &d[t]), "ltpmg_create_table");
             check(ltpmg_fill_table(g.e, t, g.seed), "ltpmg_fill_table");
         }
         return;
@@ -119,19 +128,31 @@ void transfer_database_to_GPU(std::shared_ptr<Param> param, Global_Table_Info *i
         d.n_rows = info[t].table_size;
         d.n_cols = n_cols(info[t]);
         for (uint32_t c = 0; c < d.n_cols; ++c) d.col_bytes[c] = host_col(info[t], tables[t], c).width;
-        check(ltpmg_create_table(g.e, t, &d), "ltpmg_create_table");
+        if (t == 0 && param->benchmark == "YCSB_E") {
+            // E's insert appends a row per query and epoch
+            // (cuda/src/gpuquery.cu:3243-3264): rows [0, table_size) are
+            // present, the rest of the capacity takes the inserts
+            const uint32_t cap = ltpmg_pack::ycsb_e_capacity(*param, d.n_rows);
+            if (cap == 0) die("YCSB_E: the table and its inserts exceed 2^32 rows");
+            const uint32_t present = d.n_rows;
+            d.n_rows = cap;
+            check(ltpmg_create_append_table(g.e, t, &d, present), "ltpmg_create_append_table");
+        } else {
+            check(ltpmg_create_table(g.e, t, &d), "ltpmg_create_table");
+        }
         const bool host_data = tables[t].int_data || tables[t].double_data || tables[t].string_data;
         if (!host_data) {
             check(ltpmg_fill_table(g.e, t, g.seed), "ltpmg_fill_table");
             continue;
         }
         std::vector<uint8_t> packed;
+        const uint32_t host_rows = info[t].table_size;
         for (uint32_t c = 0; c < d.n_cols; ++c) {
             HostCol hc = host_col(info[t], tables[t], c);
-            packed.resize((size_t)hc.width * d.n_rows);
-            for (size_t r = 0; r < d.n_rows; ++r)
+            packed.resize((size_t)hc.width * host_rows);
+            for (size_t r = 0; r < host_rows; ++r)
                 std::memcpy(&packed[r * hc.width], hc.base + r * hc.row_stride + hc.offset, hc.width);
-            check(ltpmg_load_column(g.e, t, c, 0, d.n_rows, packed.data()), "ltpmg_load_column");
+            check(ltpmg_load_column(g.e, t, c, 0, host_rows, packed.data()), "ltpmg_load_column");
         }
     }
 }
@@ -169,7 +190,7 @@ void transfer_query_to_GPU(std::shared_ptr<Param> param,
     ensure_open(param);
     if (g.info.empty()) die("transfer_query_to_GPU before transfer_database_to_GPU");
     std::string why;
-    if (!ltpmg_pack::pack_queries(*batch, g.info[0], g.seed, g.last, &why))
+    if (!ltpmg_pack::pack_queries(*batch, g.info[0], g.seed, g.preload, g.last, &why))
         die("query type %s is not handled by this engine", why.c_str());
     ltpmg_batch b{(uint32_t)(g.last.txn_off.size() - 1), (uint32_t)g.last.rows.size(),
                   g.last.txn_off.data(), g.last.rows.data(), g.last.meta.data(), g.last.args.data()};

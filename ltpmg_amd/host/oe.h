@@ -35,14 +35,22 @@ namespace i { enum : uint32_t { ID, IM_ID, PRICE, NAME, DATA }; }
 constexpr uint32_t kDistricts = 10, kCustomers = 3000, kStock = 100000, kItems = 100000;
 constexpr uint32_t kMaxLines = 15;
 constexpr uint32_t kLastNames = LTPMG_OE_LAST_NAMES;
+constexpr uint32_t kPreLines = LTPMG_OE_PRELOAD_LINES;
+constexpr uint32_t kDeliveries = 10;  // entries of a Delivery_Query
 
 inline uint32_t district_row(uint32_t w, uint32_t dd) { return w * kDistricts + dd; }
 inline uint32_t customer_row(uint32_t w, uint32_t dd, uint32_t cc) { return district_row(w, dd) * kCustomers + cc; }
 inline uint32_t stock_row(uint32_t w, uint32_t item) { return w * kStock + item; }
+// preloaded order k of district (w, dd), and line l of a preloaded order
+inline uint32_t order_row(uint32_t w, uint32_t dd, uint32_t k, uint32_t preload) {
+    return district_row(w, dd) * preload + k;
+}
+inline uint32_t line_row(uint32_t order, uint32_t l) { return order * kPreLines + l; }
 
-// The nine table descriptors for W partitions and append tables of cap
-// transactions (the body of ltpmg_oe_schema).
-int schema(uint32_t W, uint32_t cap, ltpmg_table_desc out[LTPMG_OE_TABLES]);
+// The nine table descriptors and rows present at creation (the body of
+// ltpmg_oe_schema).
+int schema(uint32_t W, uint32_t cap, uint32_t preload, ltpmg_table_desc out[LTPMG_OE_TABLES],
+           uint32_t n_present[LTPMG_OE_TABLES]);
 
 // Last-name code of a customer row: the first little-endian u64 of its
 // c_last cell, as ltpmg_fill_table(table_seed) writes it, mod kLastNames.
@@ -52,7 +60,7 @@ uint32_t last_name_code(uint64_t table_seed, uint32_t customer_row);
 // reference's index scan, cuda/src/gpuquery.cu:2641-2650).
 uint32_t customer_by_last_name(uint64_t table_seed, uint32_t w, uint32_t dd, uint32_t k);
 
-// Collects ops of one batch in SoA form.  begin() marks the start of a
+// Collects ops of one batch in SoA form.  begin(tid) marks the start of a
 // transaction: within it, a second RMW_ADD of a cell is folded into the
 // first one's delta (a query may name one stock row on two order lines, as
 // the reference's generator allows, src/txn_generator.cpp:262-276), because
@@ -61,10 +69,15 @@ struct Sink {
     std::vector<uint32_t> &rows;
     std::vector<uint16_t> &meta;
     std::vector<uint64_t> &args;
-    uint64_t seed;        // payload seeds of inserted cells
+    uint64_t seed;        // payload seeds of written and inserted rows
     uint64_t table_seed;  // the tables' fill seed (last-name lookups)
+    uint32_t preload;     // preloaded orders per district
     size_t txn_first = 0;
-    void begin() { txn_first = rows.size(); }
+    uint32_t tid = 0;
+    void begin(uint32_t t) {
+        txn_first = rows.size();
+        tid = t;
+    }
     void put(uint32_t kind, uint32_t table, uint32_t col, uint32_t row, uint64_t arg) {
         rows.push_back(row);
         meta.push_back(LTPMG_META(kind, table, col));
@@ -80,41 +93,39 @@ struct Sink {
             }
         put(LTPMG_OP_RMW_ADD, t, col, row, (uint64_t)(uint32_t)delta);
     }
-    // an inserted cell gets a payload seeded by (seed, table, row, col)
-    void insert(uint32_t t, uint32_t col, uint32_t row) {
-        uint64_t id = ((uint64_t)row << 10) | ((uint64_t)t << 6) | col;
+    // a written cell gets a payload seeded by (seed, table, row, col)
+    void set(uint32_t t, uint32_t col, uint32_t row) {
+        put(LTPMG_OP_UPDATE, t, col, row, mixed(((uint64_t)row << 10) | ((uint64_t)t << 6) | col));
+    }
+    // an appended row gets payloads seeded by (seed, TID, table, line)
+    void insert(uint32_t t, uint32_t line = 0) {
+        put(LTPMG_OP_ROW, t, LTPMG_ROW_INSERT, 0, mixed(((uint64_t)tid << 12) | ((uint64_t)t << 8) | line | (1ull << 63)));
+    }
+    uint64_t mixed(uint64_t id) const {
         uint64_t z = seed + 0x9E3779B97F4A7C15ull * (id + 1);
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        put(LTPMG_OP_UPDATE, t, col, row, z ^ (z >> 31));
+        return z ^ (z >> 31);
     }
 };
 
-// Append rows are carried in the query's own id fields: Neworder O_ID (order
-// row), N_O_ID (neworder row), O_OL_ID (first orderline row); Payment H_ID
-// (history row); Orderstatus O_ID / OL_ID; Stocklevel O_OL_ID[k].
+// The transaction's TID is the query's TID field.
 void decompose(const Neworder_Query &q, Sink &out);
 void decompose(const Payment_Query &q, Sink &out);
 void decompose(const Orderstatus_Query &q, Sink &out);
+void decompose(const Delivery_Query &q, Sink &out);
 void decompose(const Stocklevel_Query &q, Sink &out);
-
-// Append rows owned by the transaction at position t of a batch: one row of
-// every append table and 15 orderlines, whatever the transaction's kind.
-struct Slots {
-    uint32_t order, neworder, history, orderline;
-};
-Slots slots_for(const uint32_t append_next[4], uint32_t cap, uint64_t t);
-void set_slots(Neworder_Query &q, const Slots &s);
-void set_slots(Payment_Query &q, const Slots &s);
 
 // Query builders drawing from a transaction's own random stream.
 struct Mix {
-    uint32_t W, lines_min, lines_max, remote_bp, cap;
+    uint32_t W, lines_min, lines_max, remote_bp;
+    uint32_t preload;     // preloaded orders per district (read-only queries and deliveries pick among them)
     uint32_t byname_pct;  // Payment / Orderstatus: percent that select the customer by last name
 };
 Neworder_Query make_neworder(ltpmg_host::Rng &r, const Mix &m);
 Payment_Query make_payment(ltpmg_host::Rng &r, const Mix &m);
 Orderstatus_Query make_orderstatus(ltpmg_host::Rng &r, const Mix &m);
+Delivery_Query make_delivery(ltpmg_host::Rng &r, const Mix &m);
 Stocklevel_Query make_stocklevel(ltpmg_host::Rng &r, const Mix &m);
 
 }  // namespace oe

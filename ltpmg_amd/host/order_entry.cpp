@@ -12,20 +12,6 @@ using ltpmg_host::Rng;
 
 namespace oe {
 
-Slots slots_for(const uint32_t next[4], uint32_t cap, uint64_t t) {
-    return Slots{(uint32_t)((next[2] + t) % cap), (uint32_t)((next[0] + t) % cap),
-                 (uint32_t)((next[1] + t) % cap),
-                 (uint32_t)((next[3] + t * kMaxLines) % ((uint64_t)cap * kMaxLines))};
-}
-
-void set_slots(Neworder_Query &q, const Slots &s) {
-    q.O_ID = s.order;
-    q.N_O_ID = s.neworder;
-    q.O_OL_ID = s.orderline;
-}
-
-void set_slots(Payment_Query &q, const Slots &s) { q.H_ID = s.history; }
-
 Neworder_Query make_neworder(Rng &r, const Mix &m) {
     Neworder_Query q{};
     q.W_ID = (uint32_t)r.below(m.W);
@@ -69,11 +55,31 @@ Orderstatus_Query make_orderstatus(Rng &r, const Mix &m) {
     q.W_ID = (uint32_t)r.below(m.W);
     q.D_ID = (uint32_t)r.below(kDistricts);
     q.C_ID = (uint32_t)r.nurand(1023, 0, kCustomers - 1);
-    q.O_ID = (uint32_t)r.below(m.cap);
-    q.OL_ID = (uint32_t)r.below((uint64_t)m.cap * kMaxLines - 5);
+    q.O_ID = (uint32_t)r.below(m.preload);  // a preloaded order of the district
     if (m.byname_pct) {
         q.isName = r.below(100) < m.byname_pct ? 1u : 0u;
         q.C_LAST = (uint32_t)r.below(kLastNames);
+    }
+    return q;
+}
+
+// Ten entries as the reference draws them (src/txn_generator.cpp:466-482):
+// distinct order ids, each entry's partition, district and customer
+// uniform; the order ids are drawn among the preloaded orders.
+Delivery_Query make_delivery(Rng &r, const Mix &m) {
+    Delivery_Query q{};
+    for (uint32_t j = 0; j < kDeliveries; ++j) {
+        bool dup;
+        do {
+            q.NO_O_ID[j] = (uint32_t)r.below(m.preload);
+            dup = false;
+            for (uint32_t k = 0; k < j; ++k) dup |= q.NO_O_ID[k] == q.NO_O_ID[j];
+        } while (dup && j < m.preload);
+    }
+    for (uint32_t j = 0; j < kDeliveries; ++j) {
+        q.NO_W_ID[j] = (uint32_t)r.below(m.W);
+        q.NO_D_ID[j] = (uint32_t)r.below(kDistricts);
+        q.NO_C_ID[j] = (uint32_t)r.below(kCustomers);
     }
     return q;
 }
@@ -84,7 +90,7 @@ Stocklevel_Query make_stocklevel(Rng &r, const Mix &m) {
     q.D_ID = (uint32_t)r.below(kDistricts);
     for (uint32_t k = 0; k < 10; ++k) {
         q.I_ID[k] = (uint32_t)r.nurand(8191, 0, kItems - 1);
-        q.O_OL_ID[k] = (uint32_t)r.below((uint64_t)m.cap * kMaxLines);
+        q.O_OL_ID[k] = (uint32_t)r.below((uint64_t)m.preload * kPreLines);  // a line of the district
     }
     return q;
 }
@@ -93,8 +99,9 @@ Stocklevel_Query make_stocklevel(Rng &r, const Mix &m) {
 
 using namespace oe;
 
-extern "C" int ltpmg_oe_schema(uint32_t W, uint32_t cap, ltpmg_table_desc out[LTPMG_OE_TABLES]) {
-    return oe::schema(W, cap, out);
+extern "C" int ltpmg_oe_schema(uint32_t W, uint32_t cap, uint32_t preload, ltpmg_table_desc out[LTPMG_OE_TABLES],
+                               uint32_t n_present[LTPMG_OE_TABLES]) {
+    return oe::schema(W, cap, preload, out, n_present);
 }
 
 extern "C" int ltpmg_oe_last_name_index(uint32_t W, uint64_t table_seed, uint32_t *first) {
@@ -107,18 +114,20 @@ extern "C" int ltpmg_oe_last_name_index(uint32_t W, uint64_t table_seed, uint32_
 }
 
 extern "C" uint64_t ltpmg_gen_oe_max_ops(const ltpmg_oe_spec *sp) {
-    // a Neworder of 15 remote lines is the longest: 14 + 15 * 11 ops
-    return sp ? (uint64_t)sp->n_txns * (14 + kMaxLines * 11) : 0;
+    // a Neworder of 15 remote lines: 9 + 15 * 7 ops; a Delivery: 10 * 13
+    const uint64_t per = sp && sp->delivery_pct ? kDeliveries * (3 + 2 * kPreLines) : 9 + kMaxLines * 7;
+    return sp ? (uint64_t)sp->n_txns * per : 0;
 }
 
 extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows, uint16_t *meta,
                             uint64_t *args, uint64_t max_ops, uint32_t *n_ops_out, uint32_t n_threads) {
-    if (!sp || !txn_off || !n_ops_out || sp->warehouses == 0 || sp->append_cap == 0 ||
-        sp->neworder_pct + sp->payment_pct > 100 || sp->lines_min < 1 || sp->lines_max > kMaxLines ||
-        sp->lines_min > sp->lines_max || sp->remote_bp > 10000 || sp->byname_pct > 100)
+    if (!sp || !txn_off || !n_ops_out || sp->warehouses == 0 ||
+        sp->neworder_pct + sp->payment_pct + sp->delivery_pct > 100 || sp->lines_min < 1 ||
+        sp->lines_max > kMaxLines || sp->lines_min > sp->lines_max || sp->remote_bp > 10000 ||
+        sp->byname_pct > 100 || sp->preload > LTPMG_OE_MAX_PRELOAD ||
+        (sp->preload == 0 && sp->neworder_pct + sp->payment_pct < 100))  // read-only and deliveries need orders
         return LTPMG_EINVAL;
     if (n_threads == 0) n_threads = std::max(1u, std::thread::hardware_concurrency());
-    const uint32_t W = sp->warehouses, cap = sp->append_cap;
     const uint32_t n_chunks = std::min<uint32_t>(n_threads, std::max<uint32_t>(1, sp->n_txns));
     struct Chunk {
         std::vector<uint32_t> len, rows;
@@ -129,25 +138,25 @@ extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows
     const ltpmg_oe_spec spec = *sp;
     auto gen = [&](uint32_t ci) {
         Chunk &ch = chunks[ci];
-        Sink sink{ch.rows, ch.meta, ch.args, spec.seed ^ 0x4F45ull, spec.table_seed};
+        Sink sink{ch.rows, ch.meta, ch.args, spec.seed ^ 0x4F45ull, spec.table_seed, spec.preload};
         const uint64_t t0 = (uint64_t)spec.n_txns * ci / n_chunks, t1 = (uint64_t)spec.n_txns * (ci + 1) / n_chunks;
-        const Mix mix{W, spec.lines_min, spec.lines_max, spec.remote_bp, cap, spec.byname_pct};
+        const Mix mix{spec.warehouses, spec.lines_min, spec.lines_max, spec.remote_bp, spec.preload, spec.byname_pct};
+        const uint32_t no = spec.neworder_pct, pay = no + spec.payment_pct, del = pay + spec.delivery_pct;
         for (uint64_t t = t0; t < t1; ++t) {
             Rng r(spec.seed, t);
             const size_t before = ch.rows.size();
-            // each txn owns one row of every append table (and 15 orderlines),
-            // whichever kind it is, so rows never depend on other txns
-            const Slots sl = slots_for(spec.append_next, cap, t);
             const uint32_t kind = (uint32_t)r.below(100);
-            if (kind < spec.neworder_pct) {
+            if (kind < no) {
                 Neworder_Query q = make_neworder(r, mix);
                 q.TID = (UINT32)t;
-                set_slots(q, sl);
                 decompose(q, sink);
-            } else if (kind < spec.neworder_pct + spec.payment_pct) {
+            } else if (kind < pay) {
                 Payment_Query q = make_payment(r, mix);
                 q.TID = (UINT32)t;
-                set_slots(q, sl);
+                decompose(q, sink);
+            } else if (kind < del) {
+                Delivery_Query q = make_delivery(r, mix);
+                q.TID = (UINT32)t;
                 decompose(q, sink);
             } else if (r.below(2) == 0) {
                 Orderstatus_Query q = make_orderstatus(r, mix);
@@ -183,9 +192,5 @@ extern "C" int ltpmg_gen_oe(ltpmg_oe_spec *sp, uint32_t *txn_off, uint32_t *rows
         }
     }
     *n_ops_out = (uint32_t)total;
-    sp->append_next[0] = (uint32_t)((sp->append_next[0] + (uint64_t)sp->n_txns) % cap);
-    sp->append_next[1] = (uint32_t)((sp->append_next[1] + (uint64_t)sp->n_txns) % cap);
-    sp->append_next[2] = (uint32_t)((sp->append_next[2] + (uint64_t)sp->n_txns) % cap);
-    sp->append_next[3] = (uint32_t)((sp->append_next[3] + (uint64_t)sp->n_txns * kMaxLines) % ((uint64_t)cap * kMaxLines));
     return LTPMG_OK;
 }

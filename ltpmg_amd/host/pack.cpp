@@ -6,9 +6,10 @@
 // B and D read 0-8 and update 9; C reads all ten; E scans [r, r+2) for 0-8
 // and inserts 9.  A read or scan of a row reads every column (one READ per
 // cell, consecutive result slots); an update writes column (TID + j) mod
-// n_cols with a payload seeded by (seed, TID, j); the YCSB_E insert writes
-// every column of its row (the engine's tables are fixed-size, so an insert
-// is an upsert of an existing row id).  Order-entry queries go through the
+// n_cols with a payload seeded by (seed, TID, j); the YCSB_E insert is an
+// INSERT into the YCSB table (an append table for that benchmark) whose row
+// payloads are seeded by (seed, TID, 9); the engine hands out the row, so the
+// query's ROW_ID[9] names none.  Order-entry queries go through the
 // adapter's decomposition (include/ltpmg_oe.h, oe_decompose.cpp).  The
 // reference's row operators are no-ops (gpulauncher.cu:8-165), so these
 // value rules are this build's own.
@@ -60,7 +61,7 @@ bool pack_ycsb(const std::any &a, Packer &p, int variant) {
                 p.read_row(r);
                 p.read_row(r + 1);
             } else {
-                for (uint32_t c = 0; c < p.cols; ++c) p.op(LTPMG_OP_UPDATE, c, r, p.payload(tid, j, c));
+                p.op(LTPMG_OP_ROW, LTPMG_ROW_INSERT, 0, p.payload(tid, j, 0));
             }
         } else if (update) {
             uint32_t c = (tid + j) % p.cols;
@@ -86,22 +87,27 @@ bool pack_oe(const std::any &a, oe::Sink &sink, KV_Batch &out) {
 
 }  // namespace
 
+uint32_t ycsb_e_capacity(const Param &p, uint32_t table_size) {
+    const uint64_t cap = (uint64_t)table_size + (uint64_t)p.ycsb_e_query_batch_size * (p.epoch_tp ? p.epoch_tp : 1);
+    return cap > 0xFFFFFFFFull ? 0u : (uint32_t)cap;
+}
+
 bool pack_queries(const std::vector<std::any> &batch, const Global_Table_Info &info0, uint64_t seed,
-                  KV_Batch &out, std::string *why) {
+                  uint32_t preload, KV_Batch &out, std::string *why) {
     out = KV_Batch{};
     if (batch.size() == 1 && batch[0].type() == typeid(KV_Batch)) {
         out = std::any_cast<const KV_Batch &>(batch[0]);
         return true;
     }
     Packer p{out, n_cols(info0), seed ^ 0x5943534Full};
-    oe::Sink sink{out.rows, out.meta, out.args, seed ^ 0x4F45ull, seed};
+    oe::Sink sink{out.rows, out.meta, out.args, seed ^ 0x4F45ull, seed, preload};
     out.txn_off.push_back(0);
     for (const std::any &a : batch) {
         if (pack_ycsb<YCSB_A_Query>(a, p, 'A') || pack_ycsb<YCSB_B_Query>(a, p, 'B') ||
             pack_ycsb<YCSB_C_Query>(a, p, 'C') || pack_ycsb<YCSB_D_Query>(a, p, 'D') ||
             pack_ycsb<YCSB_E_Query>(a, p, 'E') || pack_oe<Neworder_Query>(a, sink, out) ||
             pack_oe<Payment_Query>(a, sink, out) || pack_oe<Orderstatus_Query>(a, sink, out) ||
-            pack_oe<Stocklevel_Query>(a, sink, out))
+            pack_oe<Delivery_Query>(a, sink, out) || pack_oe<Stocklevel_Query>(a, sink, out))
             continue;
         if (why) *why = a.type().name();
         return false;

@@ -13,14 +13,21 @@
 #include <vector>
 
 #include "db_structure.h"
+#include "param.h"
 #include "txn_structure.h"
 
 namespace ltpmg_pack {
 
+// YCSB_E's table is an append table: rows [0, table_size) present and room
+// for one insert per query and epoch.  Returns its capacity (0 when it does
+// not fit 32-bit row ids).
+uint32_t ycsb_e_capacity(const Param &p, uint32_t table_size);
+
 // Packs the batch; info0 is table 0 of the loaded schema (the YCSB table's
-// column count), seed the run's seed.  Returns false, with the query type in
-// *why, for a query this engine does not decompose.
+// column count), seed the run's seed, preload the order-entry orders per
+// district the tables were created with.  Returns false, with the query type
+// in *why, for a query this engine does not decompose.
 bool pack_queries(const std::vector<std::any> &batch, const Global_Table_Info &info0, uint64_t seed,
-                  KV_Batch &out, std::string *why);
+                  uint32_t preload, KV_Batch &out, std::string *why);
 
 }  // namespace ltpmg_pack

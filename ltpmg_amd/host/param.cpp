@@ -224,6 +224,7 @@ RunOptions &options() {
         r.kv_ops = FLAGS_kv_ops;
         r.read_percent = FLAGS_read_percent;
         r.oe_append_cap = FLAGS_append_cap;
+        r.oe_preload = FLAGS_oe_preload;
         return r;
     }();
     return o;
@@ -238,6 +239,7 @@ void export_gpu_options() {
     setenv("LTPMG_CC_MODE", o.cc_mode == LTPMG_MODE_A ? "A" : "O", 1);
     setenv("LTPMG_SEED", std::to_string(o.seed).c_str(), 1);
     setenv("LTPMG_APPEND_CAP", std::to_string(o.oe_append_cap).c_str(), 1);
+    setenv("LTPMG_OE_PRELOAD", std::to_string(o.oe_preload).c_str(), 1);
 }
 
 }  // namespace ltpmg_host

@@ -43,28 +43,30 @@ void Query::generate_txn(std::shared_ptr<Param> param) {
         gen.initialize(param);
         gen.generate_txn(param, param->batch_size);
         transactions_batch.emplace_back(std::move(gen.get_txn().front()));
-    } else if (b == "TPCC_PART") {
+    } else if (b == "TPCC_PART" || b == "TPCC_ALL") {
         append(txn_generator_neworder_query, param, param->neworder_query_batch_size, transactions_batch);
         append(txn_generator_payment_query, param, param->payment_query_batch_size, transactions_batch);
+        if (b == "TPCC_ALL") {  // src/param.cpp:53-57 of the reference: 44/44/4/4/4
+            append(txn_generator_orderstatus_query, param, param->orderstatus_query_batch_size, transactions_batch);
+            append(txn_generator_delivery_query, param, param->delivery_query_batch_size, transactions_batch);
+            append(txn_generator_stocklevel_query, param, param->stocklevel_query_batch_size, transactions_batch);
+        }
         // interleave the kinds with a seeded Fisher-Yates shuffle (the
         // reference sorts by type-name pointer, src/query.cpp:115-116), then
-        // TID = position, and each TID owns its append rows
+        // TID = position
         ltpmg_host::Rng r(ltpmg_host::options().seed, 0x5348554646ull);
         for (size_t i = transactions_batch.size(); i > 1; --i)
             std::swap(transactions_batch[i - 1], transactions_batch[r.below(i)]);
-        const uint32_t start[4] = {0, 0, 0, 0};
         for (size_t t = 0; t < transactions_batch.size(); ++t) {
-            const oe::Slots sl = oe::slots_for(start, ltpmg_host::options().oe_append_cap, t);
-            if (auto *q = std::any_cast<Neworder_Query>(&transactions_batch[t])) {
-                q->TID = (UINT32)t;
-                oe::set_slots(*q, sl);
-            } else if (auto *p = std::any_cast<Payment_Query>(&transactions_batch[t])) {
-                p->TID = (UINT32)t;
-                oe::set_slots(*p, sl);
-            }
+            std::any &a = transactions_batch[t];
+            if (auto *q = std::any_cast<Neworder_Query>(&a)) q->TID = (UINT32)t;
+            else if (auto *p = std::any_cast<Payment_Query>(&a)) p->TID = (UINT32)t;
+            else if (auto *o = std::any_cast<Orderstatus_Query>(&a)) o->TID = (UINT32)t;
+            else if (auto *d = std::any_cast<Delivery_Query>(&a)) d->TID = (UINT32)t;
+            else if (auto *s = std::any_cast<Stocklevel_Query>(&a)) s->TID = (UINT32)t;
         }
     } else {
-        LOG(FATAL) << "benchmark " << b << ": the Delivery and TEST queries are not built in this engine";
+        LOG(FATAL) << "benchmark " << b << ": the TEST queries are not built in this engine";
     }
 }
 

@@ -40,7 +40,8 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info, co
     ltpmg_engine *e = gpu_engine(param);
     KV_Batch batch;
     std::string why;
-    if (!ltpmg_pack::pack_queries(queries, info[0], opt.seed, batch, &why)) LOG(FATAL) << "cannot pack " << why;
+    if (!ltpmg_pack::pack_queries(queries, info[0], opt.seed, opt.oe_preload, batch, &why))
+        LOG(FATAL) << "cannot pack " << why;
     const uint32_t n_txns = (uint32_t)(batch.txn_off.size() - 1), n_ops = (uint32_t)batch.rows.size();
     const uint32_t stride = ltpmg_result_stride(e);
     std::vector<uint8_t> commit(n_txns), results((size_t)n_ops * stride);
@@ -55,7 +56,10 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info, co
     fx.mode = opt.cc_mode;
     fx.n_tables = param->table_cnt;
     if (ltpmg_host::is_oe(*param)) {
-        ltpmg_oe_schema(param->warehouse_size, opt.oe_append_cap, fx.tables);
+        uint32_t present[LTPMG_OE_TABLES];
+        ltpmg_oe_schema(param->warehouse_size, opt.oe_append_cap, opt.oe_preload, fx.tables, present);
+        fx.append_mask = LTPMG_OE_APPEND_MASK;
+        for (uint32_t t = 0; t < LTPMG_OE_TABLES; ++t) fx.n_present[t] = present[t];
     } else {
         for (uint32_t t = 0; t < param->table_cnt; ++t) {
             const Global_Table_Info &ti = info[t];
@@ -65,11 +69,19 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info, co
             for (uint32_t c = 0; c < ti.double_size; ++c) d.col_bytes[d.n_cols++] = 8;
             for (uint32_t c = 0; c < ti.string_size; ++c) d.col_bytes[d.n_cols++] = 4 * ti.string_length;
         }
+        if (param->benchmark == "YCSB_E") {
+            fx.append_mask = 1u;
+            fx.n_present[0] = info[0].table_size;
+            fx.tables[0].n_rows = ltpmg_pack::ycsb_e_capacity(*param, info[0].table_size);
+        }
     }
     ltpmg_batch b{n_txns, n_ops, batch.txn_off.data(), batch.rows.data(), batch.meta.data(), batch.args.data()};
     if (ltpmg_fixture_write((p + ".batch").c_str(), &fx, &b) != LTPMG_OK) LOG(FATAL) << "cannot write " << p << ".batch";
     write_vec(p + ".commit.u8", commit);
     write_vec(p + ".results.u8", results);
+    std::vector<uint8_t> status(n_ops);
+    check(e, ltpmg_fetch_status(e, 0, status.data()), "ltpmg_fetch_status");
+    write_vec(p + ".status.u8", status);
     std::ofstream j(p + ".json");
     j << "{\"benchmark\": \"" << param->benchmark << "\", \"result_stride\": " << stride
       << ", \"committed\": " << st.committed << ", \"digests\": [";
@@ -78,7 +90,20 @@ void dump(const std::shared_ptr<Param> &param, const Global_Table_Info *info, co
         check(e, ltpmg_table_digest(e, t, &d), "ltpmg_table_digest");
         j << (t ? ", " : "") << d;
     }
-    j << "]}\n";
+    // append tables: the cursor after the batch and the presence bytes
+    j << "], \"rows\": {";
+    bool first = true;
+    for (uint32_t t = 0; t < param->table_cnt; ++t) {
+        if (!((fx.append_mask >> t) & 1u)) continue;
+        uint32_t rows = 0;
+        check(e, ltpmg_table_rows(e, t, &rows), "ltpmg_table_rows");
+        std::vector<uint8_t> pres(fx.tables[t].n_rows);
+        check(e, ltpmg_read_presence(e, t, 0, fx.tables[t].n_rows, pres.data()), "ltpmg_read_presence");
+        write_vec(p + ".present" + std::to_string(t) + ".u8", pres);
+        j << (first ? "" : ", ") << "\"" << t << "\": " << rows;
+        first = false;
+    }
+    j << "}}\n";
     LOG(INFO) << "dumped epoch 0 to " << p << ".*";
 }
 

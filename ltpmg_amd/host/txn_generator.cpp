@@ -6,9 +6,11 @@
 // from (seed, type, t), so the batch does not depend on the thread count.
 // KV_Batch: the whole batch in SoA form from ltpmg_gen_kv (ltpmg_workload.h)
 // or, for the OE benchmark, ltpmg_gen_oe (ltpmg_oe.h).  Neworder, Payment,
-// Orderstatus and Stocklevel queries come from the order-entry builders
-// (oe.h).  The Delivery and TEST generators are declared but not built.
+// Orderstatus, Delivery and Stocklevel queries come from the order-entry
+// builders (oe.h).  The TEST generators are declared but not built.
 #include "txn_generator.h"
+
+#include <algorithm>
 
 #include "ltpmg.h"
 #include "ltpmg_oe.h"
@@ -100,15 +102,15 @@ void Txn_Generator<YCSB_E_Query>::generate_txn(std::shared_ptr<Param> param, siz
 }
 
 // Order-entry queries (the reference's src/txn_generator.cpp:231-599 draws
-// them with rand(); here each query has its own seeded stream).  Append rows
-// (O_ID, N_O_ID, O_OL_ID, H_ID) are set by Query::generate_txn once TIDs are
-// known.
+// them with rand(); here each query has its own seeded stream).  TIDs are
+// set by Query::generate_txn once the kinds are interleaved; the rows that
+// INSERTs append are handed out by the engine.
 namespace {
 template <typename Q, typename Make>
 void gen_oe_queries(std::shared_ptr<Param> param, size_t size, std::vector<Q> &out, uint64_t tag,
                     Make make) {
     // 15 % of payments by last name, as src/txn_generator.cpp:346-351 draws them
-    const oe::Mix mix{param->warehouse_size, 5, 15, 100, ltpmg_host::options().oe_append_cap, 15};
+    const oe::Mix mix{param->warehouse_size, 5, 15, 100, std::max(1u, ltpmg_host::options().oe_preload), 15};
     out.resize(size);
     for (size_t i = 0; i < size; ++i) {
         ltpmg_host::Rng r(ltpmg_host::options().seed ^ (tag << 48), i);
@@ -134,6 +136,11 @@ void Txn_Generator<Orderstatus_Query>::generate_txn(std::shared_ptr<Param> param
     LOG(INFO) << "Orderstatus_Query transactions.size()=" << transactions.size();
 }
 template <>
+void Txn_Generator<Delivery_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
+    gen_oe_queries(param, size, transactions, 15, oe::make_delivery);
+    LOG(INFO) << "Delivery_Query transactions.size()=" << transactions.size();
+}
+template <>
 void Txn_Generator<Stocklevel_Query>::generate_txn(std::shared_ptr<Param> param, size_t size) {
     gen_oe_queries(param, size, transactions, 14, oe::make_stocklevel);
     LOG(INFO) << "Stocklevel_Query transactions.size()=" << transactions.size();
@@ -150,7 +157,7 @@ void Txn_Generator<KV_Batch>::generate_txn(std::shared_ptr<Param> param, size_t 
         sp.lines_min = 5;
         sp.lines_max = 15;
         sp.remote_bp = 100;
-        sp.append_cap = ltpmg_host::options().oe_append_cap;
+        sp.preload = ltpmg_host::options().oe_preload;
         sp.seed = ltpmg_host::options().seed ^ 0x4F45ull;
         sp.byname_pct = 15;  // src/txn_generator.cpp:346-351
         sp.table_seed = ltpmg_host::options().seed;  // the tables are filled from the run's seed

@@ -204,25 +204,41 @@ def digest_rows(table: int, col: int, rows: np.ndarray, cells: np.ndarray) -> in
         return int(h.sum(dtype=np.uint64))
 
 
-def touched_cells(batches, schema: dict) -> dict:
-    """{(table, col): sorted unique rows} that any of the batches names."""
+def touched_cells(batches, schema: dict, append: dict | None = None) -> dict:
+    """{(table, col): sorted unique rows} that any of the batches names, and
+    for each append table in append ({table: rows present at the start})
+    every column of the rows its INSERTs can take: [present, present + the
+    INSERT ops of the batches)."""
     out = {}
+    is_row = [(b.meta & 3) == 3 for b in batches]
     for t, (n, widths) in schema.items():
+        n_ins = 0
+        if append and t in append:
+            n_ins = sum(int(np.count_nonzero(r & (((b.meta >> 8) & 15) == t) & (((b.meta >> 2) & 63) == 0)))
+                        for b, r in zip(batches, is_row))
+        extra = np.arange(append[t], min(n, append[t] + n_ins), dtype=np.uint32) if n_ins else None
         for c in range(len(widths)):
-            parts = [b.rows[(((b.meta >> 8) & 15) == t) & (((b.meta >> 2) & 63) == c)] for b in batches]
+            parts = [b.rows[~r & (((b.meta >> 8) & 15) == t) & (((b.meta >> 2) & 63) == c)]
+                     for b, r in zip(batches, is_row)]
+            if extra is not None:
+                parts.append(extra)
             rows = np.unique(np.concatenate(parts)) if parts else np.zeros(0, np.uint32)
             if rows.size:
                 out[(t, c)] = rows.astype(np.uint32)
     return out
 
 
-def sparse_image(schema: dict, seed: int, batches) -> tuple[Image, dict]:
+def sparse_image(schema: dict, seed: int, batches, append: dict | None = None) -> tuple[Image, dict]:
     """An image of the tables the batches name that fills only the cells they
     touch (untouched pages of the zeroed columns stay unmapped), and those
-    cells as {(table, col): rows}."""
-    cells = touched_cells(batches, schema)
-    used = {t for t, _ in cells}
-    im = Image({t: schema[t] for t in sorted(used)})
+    cells as {(table, col): rows}.  Append tables (append = {table: rows
+    present at the start}) keep full presence arrays; the rows their INSERTs
+    can take count as touched."""
+    cells = touched_cells(batches, schema, append)
+    used = {t for t, _ in cells} | {t for b in batches for t in np.unique((b.meta >> 8) & 15).tolist()
+                                     if t in schema}
+    im = Image({t: schema[t] for t in sorted(used)},
+               append={t: k for t, k in (append or {}).items() if t in used})
     for (t, c), rows in cells.items():
         im.fill_cells(seed, t, c, rows)
     return im, cells

@@ -27,19 +27,26 @@ def run_driver(tmp_path: Path, *flags: str) -> dict:
     meta["batch"] = batch
     meta["commit"] = np.fromfile(f"{prefix}.commit.u8", dtype=np.uint8)
     meta["results"] = np.fromfile(f"{prefix}.results.u8", dtype=np.uint8).reshape(-1, meta["result_stride"])
+    meta["status"] = np.fromfile(f"{prefix}.status.u8", dtype=np.uint8)
+    meta["present"] = {t: np.fromfile(f"{prefix}.present{t}.u8", dtype=np.uint8) for t in meta["append"]}
     return meta
 
 
 def check_against_oracle(meta: dict):
-    im = orc.Image(meta["schema"]).fill(meta["seed"])
+    im = orc.Image(meta["schema"], append=meta["append"]).fill(meta["seed"])
     b = meta["batch"]
-    oc, ores = orc.replay(im, b, meta["mode"], meta["result_stride"])
+    oc, ores, found = orc.replay(im, b, meta["mode"], meta["result_stride"], with_found=True)
     assert (meta["commit"] == oc).all()
     m = orc.read_mask(b, oc)
     assert (meta["results"][m] == ores[m]).all()
+    assert (meta["status"] == found).all()
     for t in sorted(meta["schema"]):
         assert meta["digests"][t] == orc.digest_columns(t, im.cols[t])
+    for t in meta["append"]:
+        assert meta["rows"][str(t)] == im.cursor[t]
+        assert (meta["present"][t] == im.present[t]).all()
     assert meta["committed"] == int(oc.sum())
+    return oc, found
 
 
 @pytest.mark.gpu
@@ -58,18 +65,29 @@ def test_driver_ycsb(tmp_path, bench):
                       "--epoch_tp=1", "--data_distribution=zipf", "--zipf_config=0.9", "--cc_mode=A")
     b = meta["batch"]
     lens = np.diff(b.txn_off.astype(np.int64))
-    expect = {"YCSB_A": 55, "YCSB_B": 91, "YCSB_C": 100, "YCSB_E": 190}[bench]
+    expect = {"YCSB_A": 55, "YCSB_B": 91, "YCSB_C": 100, "YCSB_E": 181}[bench]
     assert (lens == expect).all()
-    check_against_oracle(meta)
+    _, found = check_against_oracle(meta)
+    if bench == "YCSB_E":  # one INSERT per committed query, appended after the preloaded rows
+        ins = (b.meta & 0xFF) == 3
+        assert meta["append"] == {0: meta["schema"][0][0] - 4096}
+        assert meta["rows"]["0"] == meta["append"][0] + int(found[ins].sum()) > meta["append"][0]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bench,mode", [("TPCC_PART", "A"), ("TPCC_PART", "O"), ("OE", "O")])
+@pytest.mark.parametrize("bench,mode", [("TPCC_PART", "A"), ("TPCC_PART", "O"), ("OE", "O"),
+                                        ("TPCC_ALL", "A"), ("TPCC_ALL", "O")])
 def test_driver_order_entry(tmp_path, bench, mode):
     meta = run_driver(tmp_path, f"--benchmark={bench}", "--table_size=2", "--append_cap=20000",
-                      "--batch_size=4096", "--epoch_tp=1", f"--cc_mode={mode}")
-    assert len(meta["schema"]) == 9
-    check_against_oracle(meta)
+                      "--oe_preload=200", "--batch_size=4096", "--epoch_tp=2", f"--cc_mode={mode}")
+    assert len(meta["schema"]) == 9 and set(meta["append"]) == {3, 4, 5, 6}
+    oc, found = check_against_oracle(meta)
+    b = meta["batch"]
+    dels = (b.meta & 0xFF) == 7
+    if bench == "TPCC_ALL":  # 4 % deliveries of 10 neworder rows each
+        assert dels.sum() > 1000 and found[dels].sum() > 0
+    else:
+        assert not dels.any()
 
 
 @pytest.mark.gpu

@@ -10,7 +10,7 @@ import pytest
 
 import oracle_ffi as orc
 from ltpmg_amd import MODE_A, MODE_O, OP_RMW_ADD, OP_UPDATE, Engine, LtpmgError, Stream, StreamPlan, gen_kv
-from ltpmg_amd.engine import OeStream, oe_schema
+from ltpmg_amd.engine import OeStream, create_oe_tables
 
 pytestmark = pytest.mark.gpu
 
@@ -110,20 +110,30 @@ def test_graph_replays_continue_from_the_tables():
 
 
 def test_order_entry_and_kv_in_one_stream():
-    W, cap = 2, 60_000
-    schema = dict(oe_schema(W, cap))
-    schema[KV] = (40_000, [100] * 5)
-    oe = OeStream(W, cap, seed=9)
+    # order-entry batches with INSERTs and Delivery DELETEs between
+    # key-value batches; a rewinding stream run twice gives the same rows
+    W, cap, pre = 2, 60_000, 50
+    oe = OeStream(W, neworder_pct=40, payment_pct=40, delivery_pct=10, preload=pre, seed=9)
     batches, modes = [], []
     kv = _kv_stream(6, 4, 40_000, 5, lo=300, hi=6000, table=KV)
     for i in range(4):
         batches += [oe.next(1500 + 700 * i), kv[i]]
         modes += [MODE_A if i % 2 else MODE_O, MODE_O]
     with Engine() as e:
-        im = _setup(e, schema, 25)
-        with Stream(e, batches, modes, keep_results=True) as st:
+        schema, append = create_oe_tables(e, W, cap, 25, preload=pre)
+        schema = dict(schema)
+        schema[KV] = (40_000, [100] * 5)
+        e.create_table(KV, 40_000, [100] * 5)
+        e.fill_table(KV, 25)
+        im = orc.Image(schema, append=append).fill(25)
+        with Stream(e, batches, modes, keep_results=True, rewind=True) as st:
             st.run(StreamPlan(merge_max_ops=1 << 22, fused_max_ops=1 << 22))
             _check(e, im, st, batches, modes)
+            for t in append:
+                assert e.table_rows(t) == im.cursor[t] and (e.read_presence(t) == im.present[t]).all()
+            cursors = {t: e.table_rows(t) for t in append}
+            st.run(StreamPlan.plain())  # rewinds to the rows at creation, appends the same count
+            assert {t: e.table_rows(t) for t in append} == cursors
 
 
 def test_tuner_picks_a_plan_that_runs_correctly():

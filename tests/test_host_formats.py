@@ -28,7 +28,34 @@ def test_fixture_round_trip(tmp_path, n_txns):
     got, info = read_fixture(p)
     for f in ("txn_off", "rows", "meta", "args"):
         assert (getattr(got, f) == getattr(b, f)).all()
-    assert info == {"seed": 0xABCDEF, "mode": MODE_A, "schema": schema}
+    assert info == {"seed": 0xABCDEF, "mode": MODE_A, "schema": schema, "append": {}}
+
+
+def test_fixture_records_append_tables(tmp_path):
+    b = gen_kv(300, 1000, 2, ops=(1, 5), seed=4)
+    schema = {0: (1000, [8, 8]), 1: (50, [4]), 3: (7000, [12])}
+    p = tmp_path / "a.batch"
+    write_fixture(p, b, seed=5, schema=schema, append={1: 0, 3: 6500})
+    _, info = read_fixture(p)
+    assert info["append"] == {1: 0, 3: 6500} and info["schema"] == schema
+    with pytest.raises(LtpmgError):  # more rows present than the table holds
+        write_fixture(tmp_path / "x.batch", b, seed=5, schema=schema, append={1: 51})
+
+
+def test_fixture_version_1_files_still_read(tmp_path):
+    # version 1 had no append mask (the word after n_tables) and no n_present
+    b = gen_kv(100, 1000, 2, ops=(1, 5), seed=6)
+    p = tmp_path / "v2.batch"
+    write_fixture(p, b, seed=9, schema={0: (1000, [8, 8])}, mode=MODE_A)
+    raw = bytearray(p.read_bytes())
+    assert int.from_bytes(raw[8:12], "little") == 2 and raw[32:36] == bytes(4)
+    v1 = raw[:32] + raw[36:]
+    v1[8:12] = (1).to_bytes(4, "little")
+    v1[12:16] = (int.from_bytes(raw[12:16], "little") - 4).to_bytes(4, "little")
+    q = tmp_path / "v1.batch"
+    q.write_bytes(bytes(v1))
+    got, info = read_fixture(q)
+    assert (got.args == b.args).all() and info["append"] == {} and info["mode"] == MODE_A
 
 
 def test_fixture_rejects_damage(tmp_path):
@@ -91,7 +118,7 @@ def test_bad_schemas_stop_the_driver(tmp_path):
 def test_packaged_order_entry_schema_matches_the_adapter():
     from ltpmg_amd.engine import oe_schema
     tpcc = json.loads((SCHEMAS / "TPCC.json").read_text())["tables"]
-    widths = oe_schema(1, 1000)
+    widths = oe_schema(1, 1000, 30)
     assert len(tpcc) == len(widths) == 9
     for t, table in enumerate(tpcc):
         assert len(table["attributes"]) == len(widths[t][1]), table["name"]
