@@ -13,6 +13,12 @@
  *                     mode is a serial replay in TID order, so a group of
  *                     batches b..c gives the same commit flags, read values
  *                     and final tables as running b, then b+1, ..., then c.
+ *                     Rows that INSERTs append appear at the end of the
+ *                     group, so a batch joins a group that INSERTs into an
+ *                     append table only if its other ops on that table name
+ *                     rows below the table's cursor at the start of the run.
+ *                     Lowest-TID-wins batches with row ops run the
+ *                     three-kernel pipeline plus the row-op kernels.
  *   lowest-TID-wins   each batch stays its own unit (its TIDs, its epoch).
  *                     A batch with at most fused_max_ops ops runs as one
  *                     persistent kernel (reserve | validate + read | write
@@ -103,8 +109,10 @@ int ltpmg_stream_fetch(ltpmg_engine *e, ltpmg_stream *s, uint32_t batch, uint8_t
 /* Committed txns per batch of the last run (n_batches entries). */
 int ltpmg_stream_committed(ltpmg_engine *e, ltpmg_stream *s, uint32_t *per_batch);
 
-/* Launch groups of a plan (the batch cut): writes up to cap first-batch
- * indices, returns the number of groups (or a negative status). */
+/* Launch groups of a plan (the batch cut) for a run that starts from the
+ * current append cursors (the cursors at creation with LTPMG_STREAM_REWIND):
+ * writes up to cap first-batch indices, returns the number of groups (or a
+ * negative status). */
 int ltpmg_stream_groups(ltpmg_engine *e, ltpmg_stream *s, const ltpmg_stream_plan *plan,
                         uint32_t *group_first, uint32_t cap);
 

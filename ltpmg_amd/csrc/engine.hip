@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -347,9 +348,13 @@ void free_row_bufs(ltpmg_engine *e, size_t ops, uint8_t *&found, uint16_t *&meta
 // What a batch does with rows: any op on an append table, DELETEs, and the
 // INSERTs per table (a pass over meta, split over host threads for large
 // batches; nothing to do while no append table exists).
-void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint32_t inserts[LTPMG_MAX_TABLES]) {
+// row_end (if given) gets, per append table, one past the highest row any
+// op other than an INSERT names (0 when none does).
+void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint32_t inserts[LTPMG_MAX_TABLES],
+                  uint32_t *row_end = nullptr) {
     rb = RowBufs{};
     std::fill(inserts, inserts + LTPMG_MAX_TABLES, 0u);
+    if (row_end) std::fill(row_end, row_end + LTPMG_MAX_TABLES, 0u);
     if (e->n_append == 0 || b->n_ops == 0) return;
     uint32_t append_mask = 0;
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
@@ -358,6 +363,7 @@ void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint
     const uint32_t n_threads = n < (1u << 20) ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     struct Acc {
         uint32_t ins[LTPMG_MAX_TABLES] = {};
+        uint32_t end[LTPMG_MAX_TABLES] = {};
         bool app = false, del = false;
     };
     std::vector<Acc> acc(n_threads);
@@ -369,10 +375,12 @@ void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint
             const uint32_t tb = (m >> 8) & 15u;
             if (!((append_mask >> tb) & 1u)) continue;
             a.app = true;
-            if ((m & 3u) == LTPMG_OP_ROW) {
-                if (((m >> 2) & 63u) == LTPMG_ROW_INSERT) ++a.ins[tb];
-                else a.del = true;
+            if ((m & 3u) == LTPMG_OP_ROW && ((m >> 2) & 63u) == LTPMG_ROW_INSERT) {
+                ++a.ins[tb];
+                continue;
             }
+            if ((m & 3u) == LTPMG_OP_ROW) a.del = true;
+            a.end[tb] = std::max(a.end[tb], b->rows[i] + 1u);
         }
     };
     if (n_threads == 1) {
@@ -385,7 +393,10 @@ void scan_row_ops(const ltpmg_engine *e, const ltpmg_batch *b, RowBufs &rb, uint
     for (const Acc &a : acc) {
         rb.has_append |= a.app;
         rb.has_delete |= a.del;
-        for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) inserts[t] += a.ins[t];
+        for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
+            inserts[t] += a.ins[t];
+            if (row_end) row_end[t] = std::max(row_end[t], a.end[t]);
+        }
     }
     for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
         if (inserts[t]) rb.insert_tables |= 1u << t;
@@ -1532,6 +1543,9 @@ struct ltpmg_stream {
     std::vector<RowBufs> rows;
     uint32_t stream_inserts[LTPMG_MAX_TABLES] = {};
     uint32_t rewind_rows[LTPMG_MAX_TABLES] = {};  // LTPMG_STREAM_REWIND: cursors at creation
+    // per batch and append table: one past the highest row its non-INSERT ops name
+    std::vector<std::array<uint32_t, LTPMG_MAX_TABLES>> row_end;
+    std::vector<uint64_t> graph_groups;  // the grouping the captured graph was made for
     uint8_t *d_found = nullptr;
     uint16_t *d_meta_eff = nullptr;
     uint32_t *d_ins_cnt = nullptr, *d_ins_base = nullptr;
@@ -1565,7 +1579,9 @@ uint32_t fused_grid_cap(ltpmg_engine *e, uint32_t tile, uint32_t wgs_per_cu, uin
     return (uint32_t)e->n_cu * std::min<uint32_t>(wgs_per_cu, (uint32_t)per_cu);
 }
 
-std::vector<Group> plan_groups(ltpmg_engine *e, const ltpmg_stream &s, const ltpmg_stream_plan &p) {
+// cursor: each append table's cursor when the run starts.
+std::vector<Group> plan_groups(ltpmg_engine *e, const ltpmg_stream &s, const ltpmg_stream_plan &p,
+                               const uint32_t cursor[LTPMG_MAX_TABLES]) {
     std::vector<Group> g;
     const uint32_t cap = fused_grid_cap(e, p.fused_tile_ops, p.fused_wgs_per_cu, s.res_stride);
     const bool each = (p.flags & LTPMG_PLAN_SYNC_EACH) != 0;
@@ -1575,9 +1591,15 @@ std::vector<Group> plan_groups(ltpmg_engine *e, const ltpmg_stream &s, const ltp
         if (s.modes[b] == LTPMG_MODE_O) {
             // A merged group runs as one ordered batch, which is the same as
             // its batches one by one -- except that rows an INSERT appends
-            // become visible only after the group, so a batch with INSERTs
-            // ends its group.
-            if (!each && !g.empty() && g.back().mode == LTPMG_MODE_O && !g.back().rb.has_insert &&
+            // become visible only after the group.  So a batch joins a group
+            // with INSERTs into table t only if its other ops on t name rows
+            // below t's cursor at the start of the run, which no INSERT of
+            // the run creates.
+            bool rows_ok = true;
+            if (!g.empty())
+                for (uint32_t m = g.back().rb.insert_tables; m; m &= m - 1)
+                    rows_ok &= s.row_end[b][__builtin_ctz(m)] <= cursor[__builtin_ctz(m)];
+            if (!each && !g.empty() && g.back().mode == LTPMG_MODE_O && rows_ok &&
                 (uint64_t)s.op_base[b + 1] - s.op_base[g.back().b0] <= p.merge_max_ops) {
                 Group &last = g.back();
                 last.b1 = b + 1;
@@ -1718,8 +1740,8 @@ int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan 
     if (max_slot_all(e) > s.res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the stream was created");
     for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
     e->tail_slot = -1;
+    uint32_t cur[LTPMG_MAX_TABLES];  // the append cursors at the start of the run
     {  // every INSERT of the stream must fit (the run is synchronous: exact cursors)
-        uint32_t cur[LTPMG_MAX_TABLES];
         HIP_TRY(e, hipMemcpy(cur, e->d_cursor, sizeof cur, hipMemcpyDeviceToHost));
         for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) {
             Table &tb = e->tables[t];
@@ -1737,12 +1759,14 @@ int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan 
             tb.cursor_bound = (uint64_t)cur[t] + s.stream_inserts[t];
         }
     }
-    std::vector<Group> groups = plan_groups(e, s, p);
+    std::vector<Group> groups = plan_groups(e, s, p, cur);
     int rc = stream_scratch(e, s, groups);
     if (rc != LTPMG_OK) return rc;
     const bool each = (p.flags & LTPMG_PLAN_SYNC_EACH) != 0;
     if (p.use_graph && !each && !marks) {
-        if (!s.graph || !same_plan(s.graph_plan, p)) {
+        std::vector<uint64_t> sig;  // the grouping can change with the cursors
+        for (const Group &g : groups) sig.push_back((uint64_t)g.b0 << 1 | (g.fused ? 1u : 0u));
+        if (!s.graph || !same_plan(s.graph_plan, p) || sig != s.graph_groups) {
             drop_graph(s);
             HIP_TRY(e, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
             for (const Group &g : groups) {
@@ -1763,6 +1787,7 @@ int stream_run_locked(ltpmg_engine *e, ltpmg_stream &s, const ltpmg_stream_plan 
                 return fail(e, LTPMG_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(st));
             }
             s.graph_plan = p;
+            s.graph_groups = sig;
         }
         HIP_TRY(e, hipEventRecord(s.ev0, s.stream));
         HIP_TRY(e, hipGraphLaunch(s.graph, s.stream));
@@ -1853,12 +1878,13 @@ int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *bat
     }
     s->res_stride = max_slot_all(e);
     s->rows.resize(n_batches);
+    s->row_end.resize(n_batches);
     for (uint32_t b = 0; b < n_batches; ++b) {
         const uint32_t o0 = s->op_base[b];
         ltpmg_batch one{s->txn_base[b + 1] - s->txn_base[b], s->op_base[b + 1] - o0, nullptr,
                         all->rows + o0, all->meta + o0, all->args + o0};
         uint32_t ins[LTPMG_MAX_TABLES];
-        scan_row_ops(e, &one, s->rows[b], ins);
+        scan_row_ops(e, &one, s->rows[b], ins, s->row_end[b].data());
         for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) s->stream_inserts[t] += ins[t];
     }
     if (flags & LTPMG_STREAM_REWIND) {  // the cursors every run starts from
@@ -2083,7 +2109,16 @@ int ltpmg_stream_groups(ltpmg_engine *e, ltpmg_stream *s, const ltpmg_stream_pla
     else if (s->tuned) p = s->tuned_plan;
     else ltpmg_stream_default_plan(&p);
     if (!plan_ok(p)) return fail(e, LTPMG_EINVAL, "bad stream plan");
-    std::vector<Group> gr = plan_groups(e, *s, p);
+    // the cursors the next run starts from
+    uint32_t cur[LTPMG_MAX_TABLES];
+    if (s->flags & LTPMG_STREAM_REWIND) {
+        std::copy(s->rewind_rows, s->rewind_rows + LTPMG_MAX_TABLES, cur);
+    } else {
+        for (Slot &sl : e->slots) HIP_TRY(e, hipStreamSynchronize(sl.stream));
+        HIP_TRY(e, hipStreamSynchronize(s->stream));
+        HIP_TRY(e, hipMemcpy(cur, e->d_cursor, sizeof cur, hipMemcpyDeviceToHost));
+    }
+    std::vector<Group> gr = plan_groups(e, *s, p, cur);
     for (size_t i = 0; i < gr.size() && i < cap && group_first; ++i) group_first[i] = gr[i].b0;
     return (int)gr.size();
 }

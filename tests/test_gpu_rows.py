@@ -212,3 +212,31 @@ def test_static_entry_refuses_engines_with_append_tables():
                                          C.c_void_p(res.data_ptr()), None)
         assert rc == -5  # LTPMG_ESTATE
         torch.cuda.synchronize()
+
+
+def test_ordered_groups_merge_past_inserts_only_below_the_start_cursor():
+    # batches that INSERT and then read / write / delete only rows below the
+    # cursor at the start of the run merge into one group; a batch that
+    # reads a row the group appended starts a new one; every batch equals
+    # the oracle's replay in stream order
+    rng = np.random.default_rng(31)
+    n_present, cap = 3000, 20_000
+    low = [random_row_batch(rng, 400, n_present, 2, table=0, insert_pct=20, delete_pct=10)
+           for _ in range(5)]
+    reader = Batch.from_txns([[(OP_READ, 0, 1, n_present + 3, 0)], [(OP_READ, 0, 0, 17, 0)]])
+    batches = low[:3] + [reader] + low[3:]
+    plan = StreamPlan(merge_max_ops=1 << 24, fused_max_ops=0, use_graph=False)
+    with Engine() as e:
+        im = _setup(e, {0: (cap, [8, 16])}, {0: n_present})
+        with Stream(e, batches, MODE_O, keep_results=True) as st:
+            assert st.groups(plan) == [0, 3]
+            st.run(plan)
+            rs = e.result_stride
+            for k, b in enumerate(batches):
+                oc, ores = orc.replay(im, b, MODE_O, rs)
+                c, r = st.fetch(k)
+                assert (c == oc).all()
+                mask = orc.read_mask(b, oc)
+                assert (r[mask] == ores[mask]).all(), f"batch {k}: result slots differ"
+            _check_tables(e, im)
+        assert im.present[0][n_present + 3] == 1  # the reader saw a row an earlier batch appended
