@@ -214,6 +214,29 @@ typedef struct {
 int ltpmg_stage_batch(ltpmg_engine *e, uint32_t slot, uint32_t n_txns, uint32_t n_ops,
                       ltpmg_staging *out);
 int ltpmg_submit_staged(ltpmg_engine *e, uint32_t slot, uint32_t flags);
+/* Submit from a pool without a host copy: after ltpmg_stage_batch, the
+ * caller writes only staging.txn_off; the batch's ops are the runs, each
+ * n_ops consecutive ops of pool (rows / meta / args; pool->txn_off is not
+ * read) copied to ops [dst_op, dst_op + n_ops) of the batch.  The runs must
+ * tile the batch's ops in order.  Runs of 4096 ops or more are copied
+ * straight from the pool by DMA, so a pool in pinned memory
+ * (ltpmg_pinned_alloc) is uploaded without passing through the CPU; shorter
+ * runs are gathered into the staging buffer first.  The ops are trusted as
+ * with LTPMG_SUBMIT_TRUSTED (check the pool once with ltpmg_check_batch);
+ * the offsets are checked.  Replaces the reference's per-batch pinned
+ * allocation and upload (cuda/src/gpuquery.cu:2098-2146) for the serving
+ * form. */
+typedef struct {
+    uint32_t src_op; /* first op of the run in the pool */
+    uint32_t dst_op; /* where it goes in the batch */
+    uint32_t n_ops;
+} ltpmg_op_run;
+int ltpmg_submit_runs(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *pool, const ltpmg_op_run *runs,
+                      uint32_t n_runs);
+/* Pinned (page-locked) host memory for pools and arrival rings; no
+ * counterpart in the reference (it pins a new buffer per batch). */
+int ltpmg_pinned_alloc(uint64_t bytes, void **out);
+int ltpmg_pinned_free(void *p);
 /* Batches take effect on the tables in the order of ltpmg_execute calls,
  * across slots: a batch's reads and writes of cells and lock words wait for
  * the previously executed batch.  In the ordered mode the key build, sort

@@ -12,7 +12,7 @@
  *     max_batch transactions wait or the oldest waited max_wait_us; up to
  *     n_inflight batches are on the device at once (their sort/scan stages
  *     overlap the previous batch's apply, include/ltpmg.h), and the next
- *     batch is packed while the device works;
+ *     batch is cut while the device works;
  *   plain form (min_batch > 0, n_inflight == 1): wait for min_batch
  *     transactions (or the end of the stream), one batch at a time.
  *
@@ -20,9 +20,16 @@
  * goes back to the head of the queue and so gets a low TID in the next
  * batch; TID 0 always commits, so every transaction eventually commits.
  * Latency of a transaction = completion of the batch it commits in minus
- * its scheduled arrival time.  Admission is bounded by HBM: a batch is only
- * cut into a free slot, and slot buffers are allocated under the engine's
- * budget (LTPMG_EBUDGET when a batch would not fit).
+ * its scheduled arrival time.
+ *
+ * The pool's op arrays are copied once into pinned memory before the clock
+ * starts; a cut batch is uploaded by DMA straight from there as runs of
+ * consecutive pool ops (ltpmg_submit_runs), so the host writes only the
+ * batch's offsets.  Admission is by HBM: slot buffers are allocated under
+ * the engine's budget before the clock starts, and with hbm_inflight_bytes
+ * set a batch is cut only as large as fits that many device bytes beside
+ * the batches already in flight (ops 14 B + a result slot + 16 B of sort
+ * keys per op, 5 B per transaction), otherwise it waits for one to finish.
  */
 #ifndef LTPMG_SERVE_H
 #define LTPMG_SERVE_H
@@ -44,7 +51,7 @@ typedef struct {
     uint32_t n_inflight;   /* batches on the device at once, <= engine slots */
     uint32_t mode;         /* LTPMG_MODE_O or LTPMG_MODE_A */
     uint64_t seed;         /* arrival process */
-    uint32_t pack_threads; /* host threads packing a batch (0 = 8) */
+    uint64_t hbm_inflight_bytes; /* admission budget of the batches in flight (0 = slots only) */
 } ltpmg_serve_cfg;
 
 typedef struct {
@@ -56,8 +63,10 @@ typedef struct {
     double committed_per_s;
     double p50_ms, p90_ms, p99_ms, p999_ms, max_ms; /* commit latency */
     double mean_batch_txns;
-    double host_pack_s;      /* host time cutting, packing and submitting */
+    double host_pack_s;      /* host time cutting and submitting batches (offsets, DMA issue) */
     double device_ms;        /* sum of the batches' device times */
+    uint64_t peak_inflight_bytes; /* most device bytes of batches in flight at once */
+    uint64_t budget_holds;   /* cuts made smaller, or put off, by hbm_inflight_bytes */
 } ltpmg_serve_stats;
 
 /* Optional record of what ran, for replay ("same alone and in the mix"). */

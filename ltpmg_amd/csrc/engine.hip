@@ -1140,6 +1140,33 @@ static int stage_locked(ltpmg_engine *e, Slot &s, uint32_t n_txns, uint32_t n_op
     return LTPMG_OK;
 }
 
+// Row-op buffers of the slot, once its batch touches an append table.
+static int slot_row_bufs(ltpmg_engine *e, Slot &s) {
+    if (!s.rb.has_append) return LTPMG_OK;
+    if (!s.has_row_bufs) {
+        int rc = alloc_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+        if (rc != LTPMG_OK) {
+            free_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
+            return rc;
+        }
+        s.has_row_bufs = true;
+    }
+    s.rb.found = s.d_found;
+    s.rb.meta_eff = s.d_meta_eff;
+    s.rb.ins_cnt = s.d_ins_cnt;
+    s.rb.ins_base = s.d_ins_base;
+    s.rb.cursor = e->d_cursor;
+    s.rb.inserted = s.d_counters + 1;
+    return LTPMG_OK;
+}
+
+static void slot_uploaded(Slot &s, uint32_t n_txns, uint32_t n_ops) {
+    s.n_txns = n_txns;
+    s.n_ops = n_ops;
+    s.has_batch = true;
+    s.executed = false;
+}
+
 static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
     const uint32_t n_txns = s.staged_txns, n_ops = s.staged_ops;
     ltpmg_staging st = stage_ptrs(s, n_txns, n_ops);
@@ -1149,26 +1176,9 @@ static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
         ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};
         int rc = trusted ? validate_structure(e, &b, 1ull << 29) : validate_batch(e, &b);
         if (rc != LTPMG_OK) return rc;
-    }
-    {
-        ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};
         scan_row_ops(e, &b, s.rb, s.inserts);
-    }
-    if (s.rb.has_append) {
-        if (!s.has_row_bufs) {
-            int rc = alloc_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
-            if (rc != LTPMG_OK) {
-                free_row_bufs(e, s.cap_ops, s.d_found, s.d_meta_eff, s.d_ins_cnt, s.d_ins_base);
-                return rc;
-            }
-            s.has_row_bufs = true;
-        }
-        s.rb.found = s.d_found;
-        s.rb.meta_eff = s.d_meta_eff;
-        s.rb.ins_cnt = s.d_ins_cnt;
-        s.rb.ins_base = s.d_ins_base;
-        s.rb.cursor = e->d_cursor;
-        s.rb.inserted = s.d_counters + 1;
+        rc = slot_row_bufs(e, s);
+        if (rc != LTPMG_OK) return rc;
     }
     HIP_TRY(e, hipEventRecord(s.up0, s.stream));
     HIP_TRY(e, hipMemcpyAsync(s.d_txn_off, st.txn_off, ((size_t)n_txns + 1) * 4, hipMemcpyHostToDevice, s.stream));
@@ -1178,10 +1188,85 @@ static int upload_locked(ltpmg_engine *e, Slot &s, bool trusted) {
         HIP_TRY(e, hipMemcpyAsync(s.d_meta, st.meta, (size_t)n_ops * 2, hipMemcpyHostToDevice, s.stream));
     }
     HIP_TRY(e, hipEventRecord(s.up1, s.stream));
-    s.n_txns = n_txns;
-    s.n_ops = n_ops;
-    s.has_batch = true;
-    s.executed = false;
+    slot_uploaded(s, n_txns, n_ops);
+    return LTPMG_OK;
+}
+
+// Runs of at least this many ops go to the device straight from the pool;
+// shorter ones are gathered into the pinned staging buffer first, and each
+// stretch of them goes up as one copy per array.
+constexpr uint32_t kDirectRunOps = 4096;
+
+static int upload_runs_locked(ltpmg_engine *e, Slot &s, const ltpmg_batch *pool, const ltpmg_op_run *runs,
+                              uint32_t n_runs) {
+    const uint32_t n_txns = s.staged_txns, n_ops = s.staged_ops;
+    ltpmg_staging st = stage_ptrs(s, n_txns, n_ops);
+    // the runs tile the batch's ops in order and stay inside the pool
+    uint64_t at = 0;
+    for (uint32_t r = 0; r < n_runs; ++r) {
+        if (runs[r].dst_op != at || runs[r].n_ops == 0 || (uint64_t)runs[r].src_op + runs[r].n_ops > pool->n_ops)
+            return fail(e, LTPMG_EINVAL, "run %u: ops [%u, +%u) of the pool to op %u of the batch (expected op %llu)",
+                        r, runs[r].src_op, runs[r].n_ops, runs[r].dst_op, (unsigned long long)at);
+        at += runs[r].n_ops;
+    }
+    if (at != n_ops) return fail(e, LTPMG_EINVAL, "the runs hold %llu ops, %u staged", (unsigned long long)at, n_ops);
+    {
+        ltpmg_batch b{n_txns, n_ops, st.txn_off, st.rows, st.meta, st.args};  // only the offsets are read
+        int rc = validate_structure(e, &b, 1ull << 29);
+        if (rc != LTPMG_OK) return rc;
+        s.rb = RowBufs{};
+        std::fill(s.inserts, s.inserts + LTPMG_MAX_TABLES, 0u);
+        for (uint32_t r = 0; r < n_runs && e->n_append; ++r) {
+            const ltpmg_op_run &run = runs[r];
+            ltpmg_batch v{0, run.n_ops, nullptr, pool->rows + run.src_op, pool->meta + run.src_op, nullptr};
+            RowBufs one;
+            uint32_t ins[LTPMG_MAX_TABLES];
+            scan_row_ops(e, &v, one, ins);
+            s.rb.has_append |= one.has_append;
+            s.rb.has_delete |= one.has_delete;
+            s.rb.insert_tables |= one.insert_tables;
+            for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t) s.inserts[t] += ins[t];
+        }
+        s.rb.has_insert = s.rb.insert_tables != 0;
+        rc = slot_row_bufs(e, s);
+        if (rc != LTPMG_OK) return rc;
+    }
+    HIP_TRY(e, hipEventRecord(s.up0, s.stream));
+    HIP_TRY(e, hipMemcpyAsync(s.d_txn_off, st.txn_off, ((size_t)n_txns + 1) * 4, hipMemcpyHostToDevice, s.stream));
+    auto up = [&](const uint32_t *rows, const uint16_t *meta, const uint64_t *args, uint32_t dst, uint32_t n) -> int {
+        HIP_TRY(e, hipMemcpyAsync(s.d_args + dst, args, (size_t)n * 8, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(e, hipMemcpyAsync(s.d_rows + dst, rows, (size_t)n * 4, hipMemcpyHostToDevice, s.stream));
+        HIP_TRY(e, hipMemcpyAsync(s.d_meta + dst, meta, (size_t)n * 2, hipMemcpyHostToDevice, s.stream));
+        return LTPMG_OK;
+    };
+    uint32_t packed0 = 0, packed1 = 0;  // the pending stretch of short runs in staging
+    auto flush = [&]() -> int {
+        int rc = LTPMG_OK;
+        if (packed1 > packed0)
+            rc = up(st.rows + packed0, st.meta + packed0, st.args + packed0, packed0, packed1 - packed0);
+        packed0 = packed1;
+        return rc;
+    };
+    for (uint32_t r = 0; r < n_runs; ++r) {
+        const ltpmg_op_run &run = runs[r];
+        int rc = LTPMG_OK;
+        if (run.n_ops >= kDirectRunOps) {
+            rc = flush();
+            if (rc == LTPMG_OK)
+                rc = up(pool->rows + run.src_op, pool->meta + run.src_op, pool->args + run.src_op, run.dst_op, run.n_ops);
+            packed0 = packed1 = run.dst_op + run.n_ops;
+        } else {
+            std::memcpy(st.rows + run.dst_op, pool->rows + run.src_op, (size_t)run.n_ops * 4);
+            std::memcpy(st.meta + run.dst_op, pool->meta + run.src_op, (size_t)run.n_ops * 2);
+            std::memcpy(st.args + run.dst_op, pool->args + run.src_op, (size_t)run.n_ops * 8);
+            packed1 = run.dst_op + run.n_ops;
+        }
+        if (rc != LTPMG_OK) return rc;
+    }
+    int rc = flush();
+    if (rc != LTPMG_OK) return rc;
+    HIP_TRY(e, hipEventRecord(s.up1, s.stream));
+    slot_uploaded(s, n_txns, n_ops);
     return LTPMG_OK;
 }
 
@@ -1234,6 +1319,35 @@ int ltpmg_submit_staged(ltpmg_engine *e, uint32_t slot, uint32_t flags) {
     int rc = upload_locked(e, *s, (flags & LTPMG_SUBMIT_TRUSTED) != 0);
     s->staged_txns = UINT32_MAX;
     return rc;
+}
+
+int ltpmg_submit_runs(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *pool, const ltpmg_op_run *runs,
+                      uint32_t n_runs) {
+    if (!e || !pool || (n_runs && !runs) || (pool->n_ops && (!pool->rows || !pool->meta || !pool->args)))
+        return LTPMG_EINVAL;
+    std::lock_guard<std::mutex> g(e->mu);
+    HIP_TRY(e, hipSetDevice(e->device));
+    Slot *s = get_slot(e, slot);
+    if (!s) return fail(e, LTPMG_EINVAL, "slot %u out of range", slot);
+    if (s->staged_txns == UINT32_MAX) return fail(e, LTPMG_ESTATE, "nothing staged in slot %u", slot);
+    int rc = upload_runs_locked(e, *s, pool, runs, n_runs);
+    s->staged_txns = UINT32_MAX;
+    return rc;
+}
+
+int ltpmg_pinned_alloc(uint64_t bytes, void **out) {
+    if (!out) return LTPMG_EINVAL;
+    *out = nullptr;
+    if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return LTPMG_ENOMEM;
+    }
+    return LTPMG_OK;
+}
+
+int ltpmg_pinned_free(void *p) {
+    if (p && hipHostFree(p) != hipSuccess) return LTPMG_EHIP;
+    return LTPMG_OK;
 }
 
 int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {

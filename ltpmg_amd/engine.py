@@ -100,6 +100,9 @@ ENGINE_API = {
     "ltpmg_stage_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.POINTER(_Staging)]),
     "ltpmg_submit_staged": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "ltpmg_submit_runs": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Batch), C.c_void_p, C.c_uint32]),
+    "ltpmg_pinned_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "ltpmg_pinned_free": (C.c_int, [C.c_void_p]),
     "ltpmg_execute": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "ltpmg_query": (C.c_int, [C.c_void_p, C.c_uint32]),
     "ltpmg_wait": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -149,7 +152,7 @@ STREAM_API = {
 class _ServeCfg(C.Structure):
     _fields_ = [("arrival_rate", C.c_double), ("n_arrivals", C.c_uint32), ("max_batch", C.c_uint32),
                 ("min_batch", C.c_uint32), ("max_wait_us", C.c_double), ("n_inflight", C.c_uint32),
-                ("mode", C.c_uint32), ("seed", C.c_uint64), ("pack_threads", C.c_uint32)]
+                ("mode", C.c_uint32), ("seed", C.c_uint64), ("hbm_inflight_bytes", C.c_uint64)]
 
 
 class _ServeStats(C.Structure):
@@ -157,7 +160,8 @@ class _ServeStats(C.Structure):
                 ("batches", C.c_uint64), ("wall_s", C.c_double), ("committed_per_s", C.c_double),
                 ("p50_ms", C.c_double), ("p90_ms", C.c_double), ("p99_ms", C.c_double),
                 ("p999_ms", C.c_double), ("max_ms", C.c_double), ("mean_batch_txns", C.c_double),
-                ("host_pack_s", C.c_double), ("device_ms", C.c_double)]
+                ("host_pack_s", C.c_double), ("device_ms", C.c_double),
+                ("peak_inflight_bytes", C.c_uint64), ("budget_holds", C.c_uint64)]
 
 
 class _ServeRecord(C.Structure):
@@ -544,6 +548,31 @@ class Engine:
         self._check("ltpmg_submit_batch", self._lib.ltpmg_submit_batch(self._h, slot, C.byref(b.c_struct())))
         self._batches[slot] = b
 
+    def submit_runs(self, pool: Batch, ids: np.ndarray, slot: int = 0) -> Batch:
+        """Submits pool transactions `ids` in that order through
+        ltpmg_submit_runs (consecutive pool txns form one run of ops) and
+        returns the batch that went up."""
+        ids = np.asarray(ids, np.int64)
+        b = take_txns(pool, ids)
+        st = _Staging()
+        self._check("ltpmg_stage_batch", self._lib.ltpmg_stage_batch(self._h, slot, b.n_txns, b.n_ops, C.byref(st)))
+        C.memmove(st.txn_off, b.txn_off.ctypes.data, b.txn_off.nbytes)
+        brk = np.flatnonzero(np.diff(ids) != 1) + 1
+        starts = np.concatenate([[0], brk])
+        ends = np.concatenate([brk, [len(ids)]])
+        off = pool.txn_off.astype(np.int64)
+        runs = np.zeros((len(starts), 3), np.uint32)
+        runs[:, 0] = off[ids[starts]]
+        runs[:, 1] = b.txn_off[starts]
+        runs[:, 2] = off[ids[ends - 1] + 1] - off[ids[starts]]
+        keep = runs[:, 2] > 0
+        runs = np.ascontiguousarray(runs[keep])
+        pc = pool.normalized()
+        self._check("ltpmg_submit_runs", self._lib.ltpmg_submit_runs(self._h, slot, C.byref(pc.c_struct()),
+                                                                     _ptr(runs), len(runs)))
+        self._batches[slot] = b
+        return b
+
     def submit_staged(self, batch: Batch, slot: int = 0, trusted: bool = False) -> None:
         """Writes the batch straight into the slot's pinned staging, then uploads."""
         b = batch.normalized()
@@ -632,14 +661,15 @@ class Engine:
 
     def serve(self, pool: Batch, *, arrival_rate: float, n_arrivals: int, max_batch: int,
               min_batch: int = 0, max_wait_us: float = 200.0, n_inflight: int = 2,
-              mode: int = MODE_O, seed: int = 1, pack_threads: int = 8, record: bool = False,
+              mode: int = MODE_O, seed: int = 1, hbm_inflight_bytes: int = 0, record: bool = False,
               record_results: bool = False, record_cap: int | None = None) -> dict:
         """Open-loop serving run (include/ltpmg_serve.h).  Returns the stats,
         plus the batch record when asked for (record_cap entries; retried
-        txns of the lowest-TID-wins mode take one entry per attempt)."""
+        txns of the lowest-TID-wins mode take one entry per attempt).
+        hbm_inflight_bytes bounds the device bytes of the batches in flight."""
         p = pool.normalized()
         cfg = _ServeCfg(arrival_rate, n_arrivals, max_batch, min_batch, max_wait_us, n_inflight,
-                        mode, seed & 0xFFFFFFFFFFFFFFFF, pack_threads)
+                        mode, seed & 0xFFFFFFFFFFFFFFFF, hbm_inflight_bytes)
         st = _ServeStats()
         rec = None
         keep = {}

@@ -2,11 +2,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <functional>
-#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -29,64 +26,31 @@ struct Flight {
     size_t first;  // index into `entries`
     uint32_t n;
     uint32_t n_ops;
+    uint64_t bytes;  // device bytes the batch holds (admission)
 };
 
-// Packing workers that live for the whole run: a batch is split into one
-// range per worker, and run() returns when every range is copied.
-class PackPool {
-  public:
-    explicit PackPool(unsigned n) {
-        for (unsigned w = 1; w < n; ++w) th_.emplace_back([this, w] { loop(w); });
-        n_ = n;
+// A pinned copy of the pool's op arrays, made before the clock starts: cut
+// batches go up from it by DMA (ltpmg_submit_runs) without a host copy.
+struct PinnedPool {
+    ltpmg_batch b{};
+    void *rows = nullptr, *meta = nullptr, *args = nullptr;
+    ~PinnedPool() {
+        ltpmg_pinned_free(rows);
+        ltpmg_pinned_free(meta);
+        ltpmg_pinned_free(args);
     }
-    ~PackPool() {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
+    int init(const ltpmg_batch &pool) {
+        const size_t n = pool.n_ops;
+        if (ltpmg_pinned_alloc(n * 4, &rows) != LTPMG_OK || ltpmg_pinned_alloc(n * 2, &meta) != LTPMG_OK ||
+            ltpmg_pinned_alloc(n * 8, &args) != LTPMG_OK)
+            return LTPMG_ENOMEM;
+        std::memcpy(rows, pool.rows, n * 4);
+        std::memcpy(meta, pool.meta, n * 2);
+        std::memcpy(args, pool.args, n * 8);
+        b = ltpmg_batch{pool.n_txns, pool.n_ops, pool.txn_off, static_cast<const uint32_t *>(rows),
+                        static_cast<const uint16_t *>(meta), static_cast<const uint64_t *>(args)};
+        return LTPMG_OK;
     }
-    // fn(w, n) for every worker w in [0, n); the caller runs w = 0
-    void run(const std::function<void(unsigned, unsigned)> &fn) {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            job_ = &fn;
-            pending_ = n_ - 1;
-            ++gen_;
-        }
-        cv_.notify_all();
-        fn(0, n_);
-        std::unique_lock<std::mutex> l(mu_);
-        done_cv_.wait(l, [this] { return pending_ == 0; });
-        job_ = nullptr;
-    }
-    unsigned size() const { return n_; }
-
-  private:
-    void loop(unsigned w) {
-        uint64_t seen = 0;
-        while (true) {
-            const std::function<void(unsigned, unsigned)> *fn;
-            {
-                std::unique_lock<std::mutex> l(mu_);
-                cv_.wait(l, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-                fn = job_;
-            }
-            (*fn)(w, n_);
-            std::lock_guard<std::mutex> g(mu_);
-            if (--pending_ == 0) done_cv_.notify_one();
-        }
-    }
-    std::vector<std::thread> th_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
-    const std::function<void(unsigned, unsigned)> *job_ = nullptr;
-    uint64_t gen_ = 0;
-    unsigned pending_ = 0, n_ = 1;
-    bool stop_ = false;
 };
 
 double pct(std::vector<double> &v, double q) {
@@ -139,8 +103,18 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
         }
     }
 
+    PinnedPool pinned;
+    rc = pinned.init(*pool);
+    if (rc != LTPMG_OK) return rc;
+    // device bytes of a batch: its ops and result slots, the two key / value
+    // buffers of the ordered sort, its offsets and commit flags
+    const uint64_t op_bytes = 14 + (uint64_t)stride + 16, txn_bytes = 5;
+    auto batch_bytes = [&](uint64_t txns, uint64_t ops) { return ops * op_bytes + (txns + 1) * txn_bytes; };
+    uint64_t inflight_bytes = 0, peak_bytes = 0, holds = 0;
+
     std::deque<uint32_t> queue;
     std::vector<uint32_t> entries;  // arrival ids of all cut batches, in order
+    std::vector<ltpmg_op_run> runs;
     std::vector<uint8_t> commit_buf, res_buf;
     std::deque<Flight> inflight;
     std::vector<uint32_t> free_slots;
@@ -150,8 +124,6 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
     uint32_t next = 0, done = 0;
     double pack_s = 0.0, device_ms = 0.0;
     uint64_t batches = 0, retries = 0, batch_txn_sum = 0;
-    const unsigned pack_threads = cfg->pack_threads ? cfg->pack_threads : 8;
-    PackPool pool_threads(pack_threads);
     if (rec) {
         rec->n_batches = rec->n_entries = rec->overflow = 0;
         if (rec->batch_off && rec->cap_batches) rec->batch_off[0] = 0;
@@ -202,6 +174,7 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
             }
             retries += retry.size();
             for (size_t k = retry.size(); k-- > 0;) queue.push_front(retry[k]);
+            inflight_bytes -= f.bytes;
             free_slots.push_back(f.slot);
             inflight.pop_front();
             progressed = true;
@@ -218,56 +191,59 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
             }
             if (cut) {
                 const auto p0 = Clock::now();
-                const uint32_t k = (uint32_t)std::min<size_t>(queue.size(), cfg->max_batch);
+                // admission: the batch's device bytes must fit the in-flight
+                // budget beside the batches already on the device; a batch
+                // that does not fit is cut smaller, or waits for one to finish
+                uint32_t k = (uint32_t)std::min<size_t>(queue.size(), cfg->max_batch);
+                uint64_t n_ops = 0, j = 0;
+                for (; j < k; ++j) {
+                    const uint32_t pt = pool_txn(queue[j]);
+                    const uint64_t len = pool->txn_off[pt + 1] - pool->txn_off[pt];
+                    if (cfg->hbm_inflight_bytes &&
+                        inflight_bytes + batch_bytes(j + 1, n_ops + len) > cfg->hbm_inflight_bytes)
+                        break;
+                    n_ops += len;
+                }
+                if (j < k) ++holds;
+                k = (uint32_t)j;
+                if (k == 0) {
+                    if (inflight.empty()) return LTPMG_EBUDGET;  // one transaction exceeds the budget
+                    std::this_thread::yield();
+                    continue;
+                }
                 const size_t first = entries.size();
-                uint64_t n_ops = 0;
-                for (uint32_t j = 0; j < k; ++j) {
-                    uint32_t id = queue.front();
+                for (uint32_t q = 0; q < k; ++q) {
+                    entries.push_back(queue.front());
                     queue.pop_front();
-                    entries.push_back(id);
-                    uint32_t pt = pool_txn(id);
-                    n_ops += pool->txn_off[pt + 1] - pool->txn_off[pt];
                 }
                 const uint32_t slot = free_slots.back();
                 ltpmg_staging st;
                 rc = ltpmg_stage_batch(e, slot, k, (uint32_t)n_ops, &st);
                 if (rc != LTPMG_OK) return rc;
-                // offsets first (serial prefix), then op copies in parallel
+                // offsets, and the ops as runs of the pinned pool: arrivals in
+                // order map to consecutive pool txns, whose ops are contiguous
+                // (a retried txn or the pool's wrap ends a run)
+                runs.clear();
                 st.txn_off[0] = 0;
-                for (uint32_t j = 0; j < k; ++j) {
-                    uint32_t pt = pool_txn(entries[first + j]);
-                    st.txn_off[j + 1] = st.txn_off[j] + (pool->txn_off[pt + 1] - pool->txn_off[pt]);
-                }
-                // arrivals in order map to consecutive pool txns, whose ops
-                // are contiguous in the pool: each such run is one copy per
-                // array (a retried txn or the pool's wrap ends a run)
-                auto copy = [&](uint32_t a, uint32_t b) {
-                    for (uint32_t j = a; j < b;) {
-                        const uint32_t pt = pool_txn(entries[first + j]);
-                        uint32_t end = j + 1, pt_end = pt + 1;
-                        while (end < b && pt_end < pool->n_txns && pool_txn(entries[first + end]) == pt_end)
-                            ++end, ++pt_end;
-                        const uint32_t src = pool->txn_off[pt], len = pool->txn_off[pt_end] - src;
-                        const uint32_t dst = st.txn_off[j];
-                        std::memcpy(st.rows + dst, pool->rows + src, (size_t)len * 4);
-                        std::memcpy(st.meta + dst, pool->meta + src, (size_t)len * 2);
-                        std::memcpy(st.args + dst, pool->args + src, (size_t)len * 8);
-                        j = end;
+                for (uint32_t q = 0; q < k;) {
+                    const uint32_t pt = pool_txn(entries[first + q]);
+                    uint32_t end = q + 1, pt_end = pt + 1;
+                    st.txn_off[q + 1] = st.txn_off[q] + (pool->txn_off[pt + 1] - pool->txn_off[pt]);
+                    while (end < k && pt_end < pool->n_txns && pool_txn(entries[first + end]) == pt_end) {
+                        st.txn_off[end + 1] = st.txn_off[end] + (pool->txn_off[pt_end + 1] - pool->txn_off[pt_end]);
+                        ++end, ++pt_end;
                     }
-                };
-                // the workers pay off once a batch carries a few MB of ops
-                if (n_ops < (1u << 17) || pool_threads.size() == 1) {
-                    copy(0, k);
-                } else {
-                    pool_threads.run([&](unsigned w, unsigned nw) {
-                        copy((uint32_t)((uint64_t)k * w / nw), (uint32_t)((uint64_t)k * (w + 1) / nw));
-                    });
+                    runs.push_back(ltpmg_op_run{pool->txn_off[pt], st.txn_off[q], pool->txn_off[pt_end] - pool->txn_off[pt]});
+                    q = end;
                 }
-                rc = ltpmg_submit_staged(e, slot, LTPMG_SUBMIT_TRUSTED);  // the pool was checked
+                rc = ltpmg_submit_runs(e, slot, &pinned.b, runs.data(), (uint32_t)runs.size());  // the pool was checked
                 if (rc == LTPMG_OK) rc = ltpmg_execute(e, slot, cfg->mode);
                 if (rc != LTPMG_OK) return rc;
                 free_slots.pop_back();
-                inflight.push_back(Flight{slot, first, k, (uint32_t)n_ops});
+                const uint64_t bytes = batch_bytes(k, n_ops);
+                inflight_bytes += bytes;
+                peak_bytes = std::max(peak_bytes, inflight_bytes);
+                inflight.push_back(Flight{slot, first, k, (uint32_t)n_ops, bytes});
                 ++batches;
                 batch_txn_sum += k;
                 if (rec && rec->batch_off && rec->batch_txns && !rec->overflow) {
@@ -300,5 +276,7 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
     out->mean_batch_txns = batches ? (double)batch_txn_sum / batches : 0.0;
     out->host_pack_s = pack_s;
     out->device_ms = device_ms;
+    out->peak_inflight_bytes = peak_bytes;
+    out->budget_holds = holds;
     return LTPMG_OK;
 }

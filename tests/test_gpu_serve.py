@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_ffi as orc
-from ltpmg_amd import MODE_A, MODE_O, OP_RMW_ADD, OP_UPDATE, Engine, gen_kv
+from ltpmg_amd import MODE_A, MODE_O, OP_RMW_ADD, OP_UPDATE, Batch, Engine, LtpmgError, gen_kv
 from ltpmg_amd.engine import take_txns
 
 pytestmark = pytest.mark.gpu
@@ -98,3 +98,48 @@ def test_many_inflight_and_reused_pool():
         assert out["committed"] == 12_000 and out["batch_off"][-1] == 12_000
         im, _ = _check_record(pool, out, MODE_O)
         assert e.digest(0) == im.digest()
+
+
+@pytest.mark.parametrize("mode", [MODE_A, MODE_O])
+def test_submit_runs_equals_submit(mode):
+    # a batch uploaded as runs of a pinned pool (long runs by DMA straight
+    # from the pool, short ones gathered in staging) runs exactly like the
+    # same batch submitted whole
+    import ctypes as C
+    from ltpmg_amd.engine import take_txns
+    rng = np.random.default_rng(5)
+    pool = gen_kv(60_000, 50_000, 4, ops=(1, 16), zipf_theta=0.8, read_pct=50, write_kind=OP_UPDATE, seed=77)
+    pinned = {}
+    with Engine(n_slots=2) as e:
+        for f, dt in (("rows", np.uint32), ("meta", np.uint16), ("args", np.uint64)):
+            a = getattr(pool, f)
+            p = C.c_void_p()
+            assert e._lib.ltpmg_pinned_alloc(a.nbytes, C.byref(p)) == 0
+            view = np.ctypeslib.as_array((C.c_uint8 * a.nbytes).from_address(p.value)).view(dt)
+            view[:] = a
+            pinned[f] = (p, view)
+        pp = Batch(pool.txn_off, pinned["rows"][1], pinned["meta"][1], pinned["args"][1])
+        # one long run, scattered single txns (retries), the pool's wrap
+        ids = np.concatenate([rng.choice(60_000, 40, replace=False), np.arange(10_000, 30_000),
+                              np.arange(59_000, 60_000), np.arange(0, 700)])
+        e.create_table(0, 50_000, [8, 16, 4, 12])
+        e.fill_table(0, 3)
+        im = orc.Image({0: (50_000, [8, 16, 4, 12])}).fill(3)
+        b = e.submit_runs(pp, ids, slot=1)
+        assert (b.rows == take_txns(pool, ids).rows).all()
+        e.execute(mode, slot=1)
+        commit, res = e.fetch(slot=1)
+        oc, ores = orc.replay(im, b, mode, e.result_stride)
+        assert (commit == oc).all()
+        m = orc.read_mask(b, oc)
+        assert (res[m] == ores[m]).all()
+        assert e.digest(0) == orc.digest_columns(0, im.cols[0])
+        with pytest.raises(LtpmgError):  # runs that do not tile the batch
+            st_runs = np.array([[0, 1, 5]], np.uint32)
+            from ltpmg_amd.engine import _Staging
+            st = _Staging()
+            e._check("ltpmg_stage_batch", e._lib.ltpmg_stage_batch(e._h, 0, 1, 5, C.byref(st)))
+            e._check("ltpmg_submit_runs", e._lib.ltpmg_submit_runs(e._h, 0, C.byref(pp.normalized().c_struct()),
+                                                                    st_runs.ctypes.data, 1))
+        for p, _ in pinned.values():
+            assert e._lib.ltpmg_pinned_free(p) == 0
