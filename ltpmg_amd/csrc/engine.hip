@@ -666,10 +666,10 @@ int launch_mode_a(ltpmg_engine *e, hipStream_t st, const OpView &ov, uint32_t *t
     rec.end();
     if (rb) {
         rec.begin("row_deletes");
-        launch_row_deletes(st, dt, ov, *rb, commit);
+        launch_row_deletes(st, dt, ov, *rb);
         rec.end();
         rec.begin("inserts");
-        launch_inserts(st, dt, ov, *rb, commit, results, rs);
+        launch_inserts(st, dt, ov, *rb, results, rs);
         rec.end();
     }
     return LTPMG_OK;
@@ -764,10 +764,10 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBuf
     rec.end();
     if (rb) {
         rec.begin("row_deletes");
-        launch_row_deletes(st, dt, ov_in, *rb, commit);
+        launch_row_deletes(st, dt, ov_in, *rb);
         rec.end();
         rec.begin("inserts");
-        launch_inserts(st, dt, ov_in, *rb, commit, results, rs);
+        launch_inserts(st, dt, ov_in, *rb, results, rs);
         rec.end();
     }
     return LTPMG_OK;

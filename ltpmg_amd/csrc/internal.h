@@ -266,8 +266,9 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 void launch_stream_copy(hipStream_t s, const void *src, void *dst, uint64_t n16);
 void launch_cc_reserve(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                        uint32_t n_tiles, const uint32_t *epoch_ctr);
-// found (nullable): per op, 1 if its row is present at the batch start
-// (validate writes it, write skips cell writes of absent rows)
+// found (nullable): per op, 1 if its row is present at the batch start and
+// its txn commits (validate writes it, write skips cell writes of absent
+// rows)
 void launch_cc_validate_read(hipStream_t s, DevTables dt, OpView ov, const uint32_t *tile_t0,
                              uint32_t n_tiles, const uint32_t *epoch_ctr, uint8_t *commit,
                              uint8_t *results, uint32_t res_stride, uint32_t *counters, uint8_t *found);
@@ -296,9 +297,10 @@ hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile
 struct RowBufs {
     bool has_append = false, has_delete = false, has_insert = false;
     uint32_t insert_tables = 0;  // bit t: the batch INSERTs into table t
-    uint8_t *found = nullptr;    // [n_ops]: 1 = the op's row was present (an INSERT: appended)
+    uint8_t *found = nullptr;    // [n_ops]: 1 = the op takes effect: its txn commits and its row
+                                 // is present (an INSERT: it gets a row)
     uint16_t *meta_eff = nullptr;  // [n_ops]: ordered mode's meta, ops on absent rows marked invalid
-    uint32_t *ins_cnt = nullptr;   // [ins_tiles(n_ops) * LTPMG_MAX_TABLES] per tile, then its offsets
+    uint32_t *ins_cnt = nullptr;   // [LTPMG_MAX_TABLES * ins_tiles(n_ops)] per table and tile, then offsets
     uint32_t *ins_base = nullptr;  // [LTPMG_MAX_TABLES] cursor of each table at the batch start
     uint32_t *cursor = nullptr;    // [LTPMG_MAX_TABLES] the engine's append cursors
     uint32_t *inserted = nullptr;  // [1] rows the batch appended
@@ -307,15 +309,15 @@ constexpr uint32_t kInsTile = 2048;  // ops per tile of the insert kernels
 inline uint32_t ins_tiles(uint32_t n_ops) { return (n_ops + kInsTile - 1) / kInsTile; }
 // ordered mode, before the sort: the lowest op that deletes each row (lock
 // words, tagged with a new epoch), then found / meta_eff for every op (a READ
-// of an absent row gets a zeroed result slot).  After the sort and apply:
-// row_deletes clears the presence of rows whose DELETE found them (commit,
-// if given, gates each op by its txn's flag); inserts appends the committed
-// INSERTs.
+// of an absent row gets a zeroed result slot; every txn commits).  After the
+// cell writes of either mode (found then holds presence AND commit):
+// row_deletes clears the presence of rows whose DELETE found them; inserts
+// appends the INSERTs found marks.
 void launch_row_presence_o(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, uint32_t *epoch_ctr,
                            uint8_t *results, uint32_t res_stride);
-void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit);
-void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit,
-                    uint8_t *results, uint32_t res_stride);
+void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb);
+void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, uint8_t *results,
+                    uint32_t res_stride);
 
 struct SortBufs {
     uint32_t *keys[2];

@@ -11,7 +11,9 @@
 //            order over the committed INSERTs of each table: count per tile,
 //            scan over tiles (which also moves the cursor), then each INSERT
 //            writes its row.  An aborted txn takes no row, so the table has no
-//            holes.
+//            holes.  Which INSERTs commit is in found[] (the lowest-TID-wins
+//            validate kernel folds the commit flags into it; in the ordered
+//            mode every txn commits), so no kernel here maps ops to txns.
 //   DELETE   clears the row's presence byte, after the batch's cell writes.
 //
 // An op sees a row as present if its presence byte is set at the batch start
@@ -28,17 +30,6 @@
 namespace ltpmg {
 
 namespace {
-
-// The txn that owns op o: the largest t < n_txns with txn_off[t] <= o.
-__device__ inline uint32_t txn_of(const OpView &ov, uint32_t o) {
-    uint32_t lo = 0, hi = ov.n_txns;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ov.txn_off[mid] <= o) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __device__ inline bool op_ok(const TabDesc &td, uint16_t m, uint32_t row) {
     return op_check(m, row, td.live ? td.n_rows : 0u, td.n_cols, td.present != nullptr);
@@ -82,80 +73,91 @@ __global__ __launch_bounds__(kBlock) void k_row_presence(DevTables dt, OpView ov
     }
 }
 
-// Clears the presence of every row whose DELETE found it (in a committed txn).
-__global__ __launch_bounds__(kBlock) void k_row_deletes(DevTables dt, OpView ov, const uint8_t *found,
-                                                         const uint8_t *commit) {
+// Clears the presence of every row whose DELETE took effect (found: the row
+// was present and the txn commits).
+__global__ __launch_bounds__(kBlock) void k_row_deletes(DevTables dt, OpView ov, const uint8_t *found) {
     for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < ov.n_ops; o += gridDim.x * blockDim.x) {
         const uint16_t m = ov.meta[o];
         if (!is_delete(m) || !found[o]) continue;
         const TabDesc &td = dt.tabs[op_tab(m)];
         const uint32_t row = ov.rows[o];
-        if (!op_ok(td, m, row) || !commit[txn_of(ov, o)]) continue;
+        if (!op_ok(td, m, row)) continue;
         td.present[row] = 0;
     }
 }
 
-// The table of op o if it is a valid INSERT of a committed txn, else 0xFF.
-__device__ inline uint32_t committed_insert(const DevTables &dt, const OpView &ov, const uint8_t *commit,
-                                            uint32_t o) {
+// The table of op o if it is a valid INSERT that takes effect, else 0xFF.
+__device__ inline uint32_t taken_insert(const DevTables &dt, const OpView &ov, const uint8_t *found, uint32_t o) {
     if (o >= ov.n_ops) return 0xFFu;
     const uint16_t m = ov.meta[o];
-    if (!is_insert(m) || !op_ok(dt.tabs[op_tab(m)], m, ov.rows[o]) || !commit[txn_of(ov, o)]) return 0xFFu;
+    if (!is_insert(m) || !found[o] || !op_ok(dt.tabs[op_tab(m)], m, ov.rows[o])) return 0xFFu;
     return op_tab(m);
 }
 
-// Committed INSERTs per table in each kInsTile-op tile.
-__global__ __launch_bounds__(kBlock) void k_ins_count(DevTables dt, OpView ov, const uint8_t *commit,
-                                                       uint32_t *cnt) {
+// INSERTs per table in each kInsTile-op tile: cnt[table * n_tiles + tile].
+__global__ __launch_bounds__(kBlock) void k_ins_count(DevTables dt, OpView ov, const uint8_t *found, uint32_t *cnt,
+                                                       uint32_t n_tiles) {
     __shared__ uint32_t c[LTPMG_MAX_TABLES];
     if (threadIdx.x < LTPMG_MAX_TABLES) c[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t o0 = blockIdx.x * kInsTile;
     for (uint32_t i = threadIdx.x; i < kInsTile; i += blockDim.x) {
-        const uint32_t tb = committed_insert(dt, ov, commit, o0 + i);
+        const uint32_t tb = taken_insert(dt, ov, found, o0 + i);
         if (tb != 0xFFu) atomicAdd(&c[tb], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < LTPMG_MAX_TABLES) cnt[(uint64_t)blockIdx.x * LTPMG_MAX_TABLES + threadIdx.x] = c[threadIdx.x];
+    if (threadIdx.x < LTPMG_MAX_TABLES) cnt[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
 }
 
-// One block of LTPMG_MAX_TABLES waves, wave w for table w: the counts become
-// each tile's exclusive offset, base[w] gets the cursor at the batch start,
-// and the cursor moves past the rows handed out (the host has checked that
-// they fit the table).
-__global__ __launch_bounds__(64 * LTPMG_MAX_TABLES) void k_ins_scan(DevTables dt, uint32_t *cnt, uint32_t n_tiles,
-                                                                    uint32_t *cursor, uint32_t *base,
-                                                                    uint32_t *inserted) {
-    __shared__ uint32_t total[LTPMG_MAX_TABLES];
-    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
-    uint32_t run = 0;
-    for (uint32_t k0 = 0; k0 < n_tiles; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        const uint32_t v = k < n_tiles ? cnt[(uint64_t)k * LTPMG_MAX_TABLES + w] : 0u;
-        uint32_t inc = v;
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(inc, off);
-            if (lane >= off) inc += y;
-        }
-        if (k < n_tiles) cnt[(uint64_t)k * LTPMG_MAX_TABLES + w] = run + inc - v;
-        run += __shfl(inc, 63);
+constexpr uint32_t kScanThreads = 1024;
+
+// One workgroup per table (only the tables in table_mask do work): each
+// thread sums a contiguous chunk of the table's tile counts, one block scan
+// gives every chunk its start, and the thread rewrites its chunk's counts as
+// exclusive offsets.  base[t] gets the cursor at the batch start, and the
+// cursor moves past the rows handed out (the host has checked that they fit
+// the table).
+__global__ __launch_bounds__(kScanThreads) void k_ins_scan(DevTables dt, uint32_t *cnt, uint32_t n_tiles,
+                                                           uint32_t table_mask, uint32_t *cursor, uint32_t *base,
+                                                           uint32_t *inserted) {
+    __shared__ uint32_t ws[kScanThreads / 64];
+    const uint32_t t = blockIdx.x;
+    if (!((table_mask >> t) & 1u)) return;  // uniform per workgroup
+    uint32_t *c = cnt + (uint64_t)t * n_tiles;
+    const uint32_t chunk = (n_tiles + kScanThreads - 1) / kScanThreads;
+    const uint32_t k0 = threadIdx.x * chunk, k1 = k0 + chunk < n_tiles ? k0 + chunk : n_tiles;
+    uint32_t sum = 0;
+    for (uint32_t k = k0; k < k1; ++k) sum += c[k];
+    // block exclusive sum of the chunk sums
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = sum;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
     }
-    if (lane == 0) {
-        const uint32_t b = cursor[w], cap = dt.tabs[w].present ? dt.tabs[w].n_rows : b;
-        base[w] = b;
-        cursor[w] = (uint64_t)b + run < cap ? b + run : cap;
-        total[w] = run;
-    }
+    if (lane == 63) ws[w] = inc;
     __syncthreads();
+    uint32_t pre = 0, total = 0;
+    for (uint32_t i = 0; i < kScanThreads / 64; ++i) {
+        if (i < w) pre += ws[i];
+        total += ws[i];
+    }
+    uint32_t run = pre + inc - sum;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t v = c[k];
+        c[k] = run;
+        run += v;
+    }
     if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (uint32_t i = 0; i < LTPMG_MAX_TABLES; ++i) t += total[i];
-        *inserted = t;
+        const uint32_t b = cursor[t], cap = dt.tabs[t].present ? dt.tabs[t].n_rows : b;
+        base[t] = b;
+        cursor[t] = (uint64_t)b + total < cap ? b + total : cap;
+        atomicAdd(inserted, total);
     }
 }
 
-// Exclusive sum over the block's threads.
-__device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws) {
+// Exclusive sum over the block's threads; *total gets the sum of all.
+__device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws, uint32_t *total) {
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     uint32_t inc = x;
     for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -165,58 +167,77 @@ __device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws) {
     __syncthreads();  // earlier readers of ws are done
     if (lane == 63) ws[w] = inc;
     __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t i = 0; i < w; ++i) pre += ws[i];
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t i = 0; i < kBlock / 64; ++i) {
+        if (i < w) pre += ws[i];
+        tot += ws[i];
+    }
+    *total = tot;
     return pre + inc - x;
 }
 
 constexpr uint32_t kInsPerThread = kInsTile / kBlock;
 
-// Every committed INSERT writes its row: row = base + tile offset + rank in
-// the tile (op order; a thread takes kInsPerThread consecutive ops, so
-// thread order then item order is op order).  All columns get the payload of
+// Every INSERT that takes effect writes its row: row = base + tile offset +
+// rank in the tile (op order; a thread takes kInsPerThread consecutive ops,
+// so thread order then item order is op order).  Per table, the tile's
+// INSERTs are first listed in LDS in rank order; then the whole workgroup
+// writes the rows column by column, consecutive threads on consecutive rows,
+// so each column's stores are contiguous.  Every column gets the payload of
 // rowseed(arg, column), the presence byte is set, the result slot gets the
 // row id.
-__global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, const uint8_t *commit,
-                                                       uint32_t table_mask, const uint32_t *cnt, const uint32_t *base,
+__global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, uint32_t table_mask,
+                                                       const uint32_t *cnt, uint32_t n_tiles, const uint32_t *base,
                                                        uint8_t *results, uint32_t res_stride, uint8_t *found) {
     __shared__ uint32_t ws[kBlock / 64];
+    __shared__ uint32_t list[kInsTile];  // op index of the tile's k-th INSERT into the table
     const uint32_t o0 = blockIdx.x * kInsTile + threadIdx.x * kInsPerThread;
     uint32_t tb[kInsPerThread];
 #pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = committed_insert(dt, ov, commit, o0 + j);
+    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = taken_insert(dt, ov, found, o0 + j);
     for (uint32_t mask = table_mask; mask; mask &= mask - 1) {
         const uint32_t t = __builtin_ctz(mask);  // uniform
         uint32_t mine = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kInsPerThread; ++j) mine += tb[j] == t;
-        uint32_t rank = block_excl(mine, ws);
-        if (!mine) continue;
+        uint32_t n;
+        uint32_t rank = block_excl(mine, ws, &n);  // its barriers also order the previous table's list reads
+        if (n == 0) continue;  // uniform
+#pragma unroll
+        for (uint32_t j = 0; j < kInsPerThread; ++j)
+            if (tb[j] == t) list[rank++] = o0 + j;
+        __syncthreads();
         const TabDesc td = dt.tabs[t];
-        const uint32_t first = base[t] + cnt[(uint64_t)blockIdx.x * LTPMG_MAX_TABLES + t];
-        for (uint32_t j = 0; j < kInsPerThread; ++j) {
-            if (tb[j] != t) continue;
-            const uint32_t o = o0 + j, row = first + rank++;
-            uint8_t *res = results + (uint64_t)o * res_stride;
-            if (row >= td.n_rows) {  // never with the host's capacity check
+        const uint32_t first = base[t] + cnt[(uint64_t)t * n_tiles + blockIdx.x];
+        // rows past the capacity never occur with the host's capacity check
+        const uint32_t fit = first >= td.n_rows ? 0u : (td.n_rows - first < n ? td.n_rows - first : n);
+        for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+            const uint32_t o = list[k];
+            if (k >= fit) {
                 found[o] = 0;
                 continue;
             }
-            const uint64_t arg = ov.args[o];
-            for (uint32_t c = 0; c < td.n_cols; ++c) {
-                const ColDesc cd = dt.cols[t * LTPMG_MAX_COLS + c];
-                uint8_t *cell = cd.base + (uint64_t)row * cd.slot;
-                const uint64_t seed = rowseed(arg, c);
+            td.present[first + k] = 1;
+            uint8_t *res = results + (uint64_t)o * res_stride;
+            if ((res_stride & 15u) == 0) {
+                st16(res, u32x4{first + k, 0u, 0u, 0u});
+                for (uint32_t b = 16; b < res_stride; b += 16) st16(res + b, u32x4{0u, 0u, 0u, 0u});
+            } else {
+                st4(res, first + k);
+                for (uint32_t b = 4; b < res_stride; b += 4) st4(res + b, 0u);
+            }
+        }
+        for (uint32_t c = 0; c < td.n_cols; ++c) {
+            const ColDesc cd = dt.cols[t * LTPMG_MAX_COLS + c];
+            for (uint32_t k = threadIdx.x; k < fit; k += kBlock) {
+                uint8_t *cell = cd.base + (uint64_t)(first + k) * cd.slot;
+                const uint64_t seed = rowseed(ov.args[list[k]], c);
                 if (cd.slot >= 16) {
                     for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
                 } else {
                     for (uint32_t b = 0; b < cd.slot; b += 4) st4(cell + b, b < cd.width ? stream_word(seed, b >> 2) : 0u);
                 }
             }
-            td.present[row] = 1;
-            found[o] = 1;
-            st4(res, row);
-            for (uint32_t b = 4; b < res_stride; b += 4) st4(res + b, 0u);
         }
     }
 }
@@ -238,19 +259,20 @@ void launch_row_presence_o(hipStream_t s, const DevTables &dt, OpView ov, RowBuf
                        rb.meta_eff, results, res_stride);
 }
 
-void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit) {
+void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb) {
     if (ov.n_ops == 0 || !rb.has_delete) return;
-    hipLaunchKernelGGL(k_row_deletes, dim3(grid_for(ov.n_ops)), dim3(kBlock), 0, s, dt, ov, rb.found, commit);
+    hipLaunchKernelGGL(k_row_deletes, dim3(grid_for(ov.n_ops)), dim3(kBlock), 0, s, dt, ov, rb.found);
 }
 
-void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, const uint8_t *commit,
-                    uint8_t *results, uint32_t res_stride) {
+void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, uint8_t *results,
+                    uint32_t res_stride) {
     if (ov.n_ops == 0 || !rb.has_insert) return;
     const uint32_t tiles = ins_tiles(ov.n_ops);
-    hipLaunchKernelGGL(k_ins_count, dim3(tiles), dim3(kBlock), 0, s, dt, ov, commit, rb.ins_cnt);
-    hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(64 * LTPMG_MAX_TABLES), 0, s, dt, rb.ins_cnt, tiles, rb.cursor,
-                       rb.ins_base, rb.inserted);
-    hipLaunchKernelGGL(k_ins_write, dim3(tiles), dim3(kBlock), 0, s, dt, ov, commit, rb.insert_tables, rb.ins_cnt,
+    hipLaunchKernelGGL(k_ins_count, dim3(tiles), dim3(kBlock), 0, s, dt, ov, rb.found, rb.ins_cnt, tiles);
+    launch_zero_u32(s, rb.inserted, 1);
+    hipLaunchKernelGGL(k_ins_scan, dim3(LTPMG_MAX_TABLES), dim3(kScanThreads), 0, s, dt, rb.ins_cnt, tiles,
+                       rb.insert_tables, rb.cursor, rb.ins_base, rb.inserted);
+    hipLaunchKernelGGL(k_ins_write, dim3(tiles), dim3(kBlock), 0, s, dt, ov, rb.insert_tables, rb.ins_cnt, tiles,
                        rb.ins_base, results, res_stride, rb.found);
 }
 

@@ -319,12 +319,18 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
     if (__lane_id() == 0 && mine) atomicAdd(&n_commit, mine);
     // committed READs of this tile -> LDS list (loop bound uniform per wave);
-    // a READ of an absent row gets zeros (found[] of op i was written by this
-    // same thread above: both loops give op i to thread i % blockDim.x)
+    // a READ of an absent row gets zeros.  found[] of op i was written by this
+    // same thread above (both loops give op i to thread i % blockDim.x); an
+    // op of an aborted txn takes no effect, so found[] becomes presence AND
+    // commit, all the row-op kernels after this one need
     for (uint32_t base = 0; base < t.nops; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool take = false;
-        if (i < t.nops) take = op_kind(ov.meta[t.op0 + i]) == LTPMG_OP_READ && ok[owner[i]];
+        if (i < t.nops) {
+            const bool c = ok[owner[i]] != 0;
+            take = op_kind(ov.meta[t.op0 + i]) == LTPMG_OP_READ && c;
+            if (found && !c) found[t.op0 + i] = 0;
+        }
         if (take && found && !found[t.op0 + i]) {
             for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)(t.op0 + i) * res_stride + b, 0u);
             take = false;
