@@ -179,7 +179,9 @@ def serve_compare(e, pool, args) -> dict:
       plain:  batches of exactly --plain-batch txns, one at a time;
       plain at equal p99: plain batches of each size in --plain-sweep, and
               the one with the most committed txn/s whose p99 is at most the
-              served p99 (the smallest p99 when none is)."""
+              served p99 (the smallest p99 when none is);
+      capacity at p99: per served p99, the highest offered rate each form
+              carries within it (capacity_at_p99)."""
     from ltpmg_amd import MODE_O
     budget = int(args.serve_hbm_mb * (1 << 20))
     out = {"arrivals": args.serve_arrivals, "plain_batch": args.plain_batch,
@@ -223,6 +225,31 @@ def serve_compare(e, pool, args) -> dict:
             f"{best['committed_per_s']:.3g} txn/s (p99 {best['p99_ms']:.2f} ms); served "
             f"{row['served_over_plain_at_equal_p99']:.2f}x")
         out["rates"].append(row)
+    out["capacity_at_p99"] = capacity_at_p99(out["rates"])
+    for c in out["capacity_at_p99"]:
+        log(f"serve capacity at p99 <= {c['p99_ms']:.2f} ms: served {c['served_rate']:.3g}/s, "
+            f"plain {c['plain_rate']:.3g}/s (batch {c['plain_batch']})")
+    return out
+
+
+def capacity_at_p99(rates: list) -> list:
+    """For each served p99 P measured above: the highest offered rate of the
+    sweep that each form carries (>= 95 % of the offered txn/s, the rest being the drain after the last arrival) at a p99 of
+    at most P -- plain at its best batch size for that rate.  0 when no rate
+    qualifies."""
+    out = []
+    for ref in rates:
+        p99 = ref["served"]["p99_ms"]
+        served = [r["arrival_rate"] for r in rates
+                  if r["served"]["p99_ms"] <= p99 and r["served"]["committed_per_s"] >= 0.95 * r["arrival_rate"]]
+        plain, batch = 0.0, None
+        for r in rates:
+            ok = [x for x in r["plain_sweep"]
+                  if x["p99_ms"] <= p99 and x["committed_per_s"] >= 0.95 * r["arrival_rate"]]
+            if ok and r["arrival_rate"] > plain:
+                plain, batch = r["arrival_rate"], min(ok, key=lambda x: x["p99_ms"])["batch"]
+        out.append({"p99_ms": p99, "served_rate": max(served, default=0.0), "plain_rate": plain,
+                    "plain_batch": batch})
     return out
 
 
@@ -506,7 +533,8 @@ def main() -> int:
     ap.add_argument("--serve-arrivals", type=int, default=1 << 22)
     ap.add_argument("--plain-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
-    ap.add_argument("--serve-wait-us", type=float, default=2000.0)
+    ap.add_argument("--serve-wait-us", type=float, default=0.0,
+                    help="served form: cut when the oldest waited this long (0: as soon as a slot is free)")
     ap.add_argument("--serve-hbm-mb", type=float, default=2048.0,
                     help="served form: device bytes the batches in flight may hold")
     ap.add_argument("--plain-sweep", type=int, nargs="*",
@@ -520,7 +548,7 @@ def main() -> int:
     ap.add_argument("--c5-seconds", type=float, default=0.5, help="arrival window per rate")
     ap.add_argument("--c5-plain-batches", type=int, nargs="*", default=[1 << 15, 1 << 17])
     ap.add_argument("--c5-served-max", type=int, default=1 << 17)
-    ap.add_argument("--c5-wait-us", type=float, default=1000.0)
+    ap.add_argument("--c5-wait-us", type=float, default=0.0)
     ap.add_argument("--no-c3", action="store_true", help="skip the order-entry plain batch")
     ap.add_argument("--c3-txns", type=int, default=1 << 20)
     ap.add_argument("--c3-steps", type=int, default=5)
