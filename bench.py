@@ -12,6 +12,8 @@ txn commits, each cell sees its ops in TID order); the lowest-TID-wins mode
 is reported beside it.
 
 Beside the headline line's fields it reports, from the same run:
+  small_batch          execute-to-completion latency of 16..512-txn ordered
+                       batches: multi-kernel pipeline vs the one-launch kernel
   serving              served vs plain batches on the same table (ordered
                        mode, open-loop Poisson arrivals of the batch's txns)
   batching             a stream of 200 uneven batches (1k-1M txns, 1-64 ops,
@@ -190,8 +192,11 @@ def serve_compare(e, pool, args) -> dict:
     keys = ("committed_per_s", "p50_ms", "p99_ms", "p999_ms", "mean_batch_txns", "batches",
             "host_pack_s", "device_ms", "wall_s")
 
+    def arrivals(rate):  # about --serve-seconds of arrivals, at most --serve-arrivals
+        return int(min(args.serve_arrivals, max(1 << 16, rate * args.serve_seconds)))
+
     def plain(rate, b):
-        return e.serve(pool, arrival_rate=rate, n_arrivals=args.serve_arrivals, max_batch=b, min_batch=b,
+        return e.serve(pool, arrival_rate=rate, n_arrivals=arrivals(rate), max_batch=b, min_batch=b,
                        n_inflight=1, mode=MODE_O, seed=7)
 
     # one short untimed run first: the first DMA from the freshly pinned
@@ -199,8 +204,8 @@ def serve_compare(e, pool, args) -> dict:
     e.serve(pool, arrival_rate=0, n_arrivals=1 << 18, max_batch=args.serve_max_batch, n_inflight=2,
             mode=MODE_O, seed=3)
     for rate in args.serve_rates:
-        row = {"arrival_rate": rate}
-        r = e.serve(pool, arrival_rate=rate, n_arrivals=args.serve_arrivals, max_batch=args.serve_max_batch,
+        row = {"arrival_rate": rate, "arrivals": arrivals(rate)}
+        r = e.serve(pool, arrival_rate=rate, n_arrivals=arrivals(rate), max_batch=args.serve_max_batch,
                     max_wait_us=args.serve_wait_us, n_inflight=2, mode=MODE_O, seed=7,
                     hbm_inflight_bytes=budget)
         row["served"] = {k: r[k] for k in keys + ("peak_inflight_bytes", "budget_holds")}
@@ -230,6 +235,41 @@ def serve_compare(e, pool, args) -> dict:
         log(f"serve capacity at p99 <= {c['p99_ms']:.2f} ms: served {c['served_rate']:.3g}/s, "
             f"plain {c['plain_rate']:.3g}/s (batch {c['plain_batch']})")
     return out
+
+
+def small_batch_latency(e, rows: int, n_cols: int, width: int, reps: int = 200) -> dict:
+    """Execute-to-completion latency of small ordered batches of the C2
+    shape (16 ops per txn, zipf 0.99, 50/50 READ/UPDATE) on the C2 table,
+    through the multi-kernel pipeline and through the one-launch kernel
+    (batches of at most LTPMG_SMALL_MAX_OPS ops): host time of the execute
+    call, wall time from execute to completion, device time."""
+    from ltpmg_amd import MODE_O, OP_UPDATE, gen_kv
+    out = []
+    for txns in (16, 64, 256, 512):
+        b = gen_kv(txns, rows, n_cols, ops=16, zipf_theta=0.99, read_pct=50, write_kind=OP_UPDATE,
+                   seed=BATCH_SEED + 7)
+        e.submit(b)
+        row = {"txns": txns, "ops": b.n_ops}
+        for name, pipe in (("pipeline", True), ("one_launch", False)):
+            for _ in range(10):
+                e.execute(MODE_O, pipeline=pipe)
+                e.wait()
+            host, wall, dev = [], [], []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                e.execute(MODE_O, pipeline=pipe)
+                t1 = time.perf_counter()
+                e.wait()
+                wall.append(time.perf_counter() - t0)
+                host.append(t1 - t0)
+                dev.append(e.stats()["exec_ms"])
+            row[name] = {"host_us": 1e6 * pct(host, 50), "p50_us": 1e6 * pct(wall, 50),
+                         "p99_us": 1e6 * pct(wall, 99), "device_us": 1e3 * pct(dev, 50)}
+        log(f"small batch {txns} txns ({b.n_ops} ops): pipeline {row['pipeline']['p50_us']:.0f} us "
+            f"(device {row['pipeline']['device_us']:.0f}), one launch {row['one_launch']['p50_us']:.0f} us "
+            f"(device {row['one_launch']['device_us']:.0f}), execute+wait p50")
+        out.append(row)
+    return {"reps": reps, "sizes": out}
 
 
 def capacity_at_p99(rates: list) -> list:
@@ -528,9 +568,13 @@ def main() -> int:
     ap.add_argument("--theta", type=float, default=0.99)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[2e7, 5e7, 1e8, 2e8],
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[1e6, 5e6, 2e7, 5e7, 1e8, 2e8],
                     help="arrival rates (txn/s) for the served-vs-plain comparison; none to skip")
     ap.add_argument("--serve-arrivals", type=int, default=1 << 22)
+    ap.add_argument("--serve-seconds", type=float, default=0.25,
+                    help="arrival window per rate (capped at --serve-arrivals arrivals)")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small-batch latency comparison")
     ap.add_argument("--plain-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-max-batch", type=int, default=1 << 18)
     ap.add_argument("--serve-wait-us", type=float, default=0.0,
@@ -640,6 +684,10 @@ def main() -> int:
                 "gbs": (a * l / (ms * 1e-3) / 1e9) if a and ms > 0 else None,
                 "traffic_per_launch": traffic.get(k)}
 
+    small = None
+    if not args.no_small:
+        small = small_batch_latency(e, args.rows, n_cols, width)
+
     serving = None
     if args.serve_rates:
         serving = serve_compare(e, batch, args)
@@ -691,6 +739,7 @@ def main() -> int:
         "upload_ms": upload_ms, "upload_ms_first": uploads[0][1], "upload_wall_s": upload_wall,
         "upload_bytes": int(batch.n_ops * 14 + (batch.n_txns + 1) * 4),
         "roofline": roofline, "kernels": per_kernel, "cpu_baseline": cpu, "serving": serving,
+        "small_batch": small,
         "order_entry_plain": order_entry, "batching": batching, "serving_large_schema": serving_large,
         "batch_counts": counts,
     }

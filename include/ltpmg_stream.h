@@ -25,15 +25,21 @@
  *                     behind two grid barriers, fused_tile_ops ops per
  *                     tile, fused_wgs_per_cu workgroups per CU); a larger
  *                     one runs the three-kernel pipeline.
+ *   small groups      an ordered launch group of at most small_max_ops
+ *                     ops (LTPMG_SMALL_MAX_OPS at most) that touches no
+ *                     append table runs as one launch of one workgroup --
+ *                     sort, scan and cell moves fused in LDS -- instead of
+ *                     the multi-kernel ordered pipeline.
  *   use_graph         the plan's whole launch sequence is captured once as
  *                     a hipGraph and replayed by each run.
  *
- * merge_max_ops and fused_max_ops are the batch-cut parameters, and
- * fused_tile_ops / fused_wgs_per_cu the tile parameters, that
- * ltpmg_stream_tune picks by timing the stream itself.  With every batch
- * cut alone, no graph and a host wait after each batch
+ * merge_max_ops, fused_max_ops and small_max_ops are the batch-cut
+ * parameters, and fused_tile_ops / fused_wgs_per_cu the tile parameters,
+ * that ltpmg_stream_tune picks by timing the stream itself.  With every
+ * batch cut alone, no graph and a host wait after each batch
  * (LTPMG_PLAN_SYNC_EACH), a run is the plain per-batch path with its inputs
- * already in HBM.
+ * already in HBM (ltpmg_execute's path when small_max_ops is
+ * LTPMG_SMALL_MAX_OPS).
  *
  * Every result equals that of submitting the batches one by one to
  * ltpmg_submit_batch + ltpmg_execute in stream order, whatever the plan.
@@ -71,6 +77,8 @@ typedef struct {
     uint32_t fused_wgs_per_cu; /* persistent grid = CUs x this, clamped to occupancy */
     uint32_t use_graph;        /* capture the launch sequence once and replay it */
     uint32_t flags;            /* LTPMG_PLAN_* */
+    uint32_t small_max_ops;    /* ordered mode: one-launch kernel for groups with <= this many ops
+                                  (at most LTPMG_SMALL_MAX_OPS count) and no row ops */
 } ltpmg_stream_plan;
 
 /* Uploads a stream of n_batches batches.  all holds them back to back:

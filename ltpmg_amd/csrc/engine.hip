@@ -698,14 +698,24 @@ int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
 // presence bytes before its sort, so it waits for the previous batch first
 // (then its sort does not overlap that batch), and sorts by a private copy of
 // the meta words in which ops on absent rows are invalid.
+// small_max: a batch of 1 .. min(small_max, kSmallMaxOps) ops without row
+// ops runs as the one-launch kernel instead (every stage in one workgroup).
 template <class Rec, class Wait>
 int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBufs &ob,
                   uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables,
-                  RowBufs *rb = nullptr) {
+                  RowBufs *rb = nullptr, uint32_t small_max = 0) {
     uint32_t key_bits = 0;
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
     DevTables dt = dev_tables(e);
+    if (!rb && ov_in.n_ops >= 1 && ov_in.n_ops <= std::min(small_max, kSmallMaxOps)) {
+        rc = wait_tables();
+        if (rc != LTPMG_OK) return rc;
+        rec.begin("mo_small");
+        launch_mo_small(st, dt, ov_in, key_bits, results, rs, commit);
+        rec.end();
+        return LTPMG_OK;
+    }
     OpView ov = ov_in;
     if (rb) {
         rc = wait_tables();
@@ -1359,7 +1369,8 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
     Slot &s = *sp;
     if (!s.has_batch) return fail(e, LTPMG_ESTATE, "no batch submitted to slot %u", slot);
     const bool profile = (e->flags & LTPMG_CFG_PROFILE) || (mode & LTPMG_EXEC_PROFILE);
-    mode &= ~LTPMG_EXEC_PROFILE;
+    const uint32_t small_max = (mode & LTPMG_EXEC_PIPELINE) ? 0u : kSmallMaxOps;
+    mode &= ~(LTPMG_EXEC_PROFILE | LTPMG_EXEC_PIPELINE);
     if (mode != LTPMG_MODE_A && mode != LTPMG_MODE_O) return fail(e, LTPMG_EINVAL, "mode %u", mode);
     const uint32_t rs = max_slot_all(e);
     if (rs > s.cap_res_stride) return fail(e, LTPMG_ESTATE, "tables changed since the batch was submitted");
@@ -1382,7 +1393,8 @@ int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode) {
     int rc = mode == LTPMG_MODE_A
                  ? launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit, s.d_results, rs,
                                  rec, wait_tail, rb)
-                 : launch_mode_o(e, s.stream, ov, s.ob, s.d_commit, s.d_results, rs, rec, wait_tail, rb);
+                 : launch_mode_o(e, s.stream, ov, s.ob, s.d_commit, s.d_results, rs, rec, wait_tail, rb,
+                                 small_max);
     if (rc != LTPMG_OK) return rc;
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(s.done, s.stream));
@@ -1812,7 +1824,7 @@ int issue_group(ltpmg_engine *e, ltpmg_stream &s, const Group &g, const ltpmg_st
     rb.inserted = s.d_counters + 1;
     RowBufs *rbp = g.rb.has_append ? &rb : nullptr;
     if (g.mode == LTPMG_MODE_O)
-        return launch_mode_o(e, s.stream, ov, s.ob, s.d_commit + t0, results, rs, rec, now, rbp);
+        return launch_mode_o(e, s.stream, ov, s.ob, s.d_commit + t0, results, rs, rec, now, rbp, p.small_max_ops);
     if (!g.fused)
         return launch_mode_a(e, s.stream, ov, s.d_tile_t0, s.d_counters, s.d_commit + t0, results, rs, rec, now, rbp);
     GridBar bar{e->d_sync + 1, e->d_sync + 2, e->d_sync + 3, 1u << 24};
@@ -1948,7 +1960,7 @@ extern "C" {
 
 void ltpmg_stream_default_plan(ltpmg_stream_plan *p) {
     if (!p) return;
-    *p = ltpmg_stream_plan{1u << 24, 1u << 18, 2048, 2, 1, 0};
+    *p = ltpmg_stream_plan{1u << 24, 1u << 18, 2048, 2, 1, 0, kSmallMaxOps};
 }
 
 int ltpmg_stream_create(ltpmg_engine *e, uint32_t n_batches, const uint32_t *batch_txns, const uint8_t *modes,
@@ -2152,9 +2164,48 @@ int ltpmg_stream_tune(ltpmg_engine *e, ltpmg_stream *s, ltpmg_stream_plan *best,
         }
     }
     if (any_o) {
+        // the one-launch kernel for small ordered batches: per-batch times of
+        // the pipeline and of the one-launch kernel, then the size cut that
+        // minimises their sum
+        std::vector<uint32_t> o_small;
+        for (uint32_t b = 0; b < s->n_batches; ++b) {
+            const uint32_t ops = s->op_base[b + 1] - s->op_base[b];
+            if (s->modes[b] == LTPMG_MODE_O && ops <= kSmallMaxOps && !s->rows[b].has_append) o_small.push_back(ops);
+        }
+        p.small_max_ops = 0;
+        if (!o_small.empty()) {
+            ltpmg_stream_plan q = p;
+            q.merge_max_ops = 0;
+            q.use_graph = 0;
+            std::vector<float> pipe, one;
+            q.small_max_ops = 0;
+            rc = stream_run_locked(e, *s, q, nullptr, &pipe);
+            if (rc != LTPMG_OK) return rc;
+            q.small_max_ops = kSmallMaxOps;
+            rc = stream_run_locked(e, *s, q, nullptr, &one);
+            if (rc != LTPMG_OK) return rc;
+            trials += 2;
+            std::sort(o_small.begin(), o_small.end());
+            auto eligible = [&](uint32_t b) {
+                return s->modes[b] == LTPMG_MODE_O && !s->rows[b].has_append &&
+                       s->op_base[b + 1] - s->op_base[b] <= kSmallMaxOps;
+            };
+            double best_ms = 0;
+            for (uint32_t b = 0; b < s->n_batches; ++b)
+                if (eligible(b)) best_ms += pipe[b];
+            for (uint32_t cut : o_small) {
+                double sum = 0;
+                for (uint32_t b = 0; b < s->n_batches; ++b)
+                    if (eligible(b)) sum += (s->op_base[b + 1] - s->op_base[b] <= cut) ? one[b] : pipe[b];
+                if (sum < best_ms) {
+                    best_ms = sum;
+                    p.small_max_ops = cut;
+                }
+            }
+        }
         double best_ms = -1;
         uint32_t best_merge = 0;
-        for (uint32_t merge : {0u, 1u << 20, 1u << 22, 1u << 24, 1u << 26, (1u << 28)}) {
+        for (uint32_t merge : {0u, kSmallMaxOps, 1u << 20, 1u << 22, 1u << 24, 1u << 26, (1u << 28)}) {
             ltpmg_stream_plan q = p;
             q.merge_max_ops = merge;
             double ms = 0;

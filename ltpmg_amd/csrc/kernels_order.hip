@@ -61,8 +61,9 @@ __device__ inline St elem_state(uint32_t val, const uint64_t *args) {
     return {kNoSet, 0u};
 }
 
-// Exclusive segmented scan over the block's 256 thread aggregates; ws is
-// kBlock / 64 entries of LDS, free again when the scan returns.
+// Exclusive segmented scan over the block's NT thread aggregates; ws is
+// NT / 64 entries of LDS, free again when the scan returns.
+template <uint32_t NT = kBlock>
 __device__ inline Seg block_seg_exclusive(Seg x, Seg *total, Seg *ws) {
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     Seg inc = x;
@@ -79,7 +80,7 @@ __device__ inline Seg block_seg_exclusive(Seg x, Seg *total, Seg *ws) {
     Seg pre{0u, kNoSet, 0u};
     for (uint32_t i = 0; i < w; ++i) pre = seg_op(pre, ws[i]);
     Seg tot{0u, kNoSet, 0u};
-    for (uint32_t i = 0; i < kBlock / 64; ++i) tot = seg_op(tot, ws[i]);
+    for (uint32_t i = 0; i < NT / 64; ++i) tot = seg_op(tot, ws[i]);
     __syncthreads();
     *total = tot;
     return seg_op(pre, ex);
@@ -121,7 +122,8 @@ __device__ inline Seg tile_carry(const uint32_t *tflag, const uint32_t *tset, co
     return acc;
 }
 
-// Exclusive sum over the block's threads; ws as for block_seg_exclusive.
+// Exclusive sum over the block's NT threads; ws as for block_seg_exclusive.
+template <uint32_t NT = kBlock>
 __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total, uint32_t *ws) {
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     uint32_t inc = x;
@@ -132,7 +134,7 @@ __device__ inline uint32_t block_exclusive_sum(uint32_t x, uint32_t *total, uint
     if (lane == 63) ws[w] = inc;
     __syncthreads();
     uint32_t pre = 0, tot = 0;
-    for (uint32_t i = 0; i < kBlock / 64; ++i) {
+    for (uint32_t i = 0; i < NT / 64; ++i) {
         if (i < w) pre += ws[i];
         tot += ws[i];
     }
@@ -999,6 +1001,214 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
     }
 }
 
+// ---------------- ordered batches in one workgroup ----------------
+//
+// A batch of up to kSmallMaxOps ops runs as one launch of one workgroup of
+// kSmallBlock threads with every intermediate in LDS, so a small batch pays
+// one launch instead of the pipeline's twenty (about 3 us of host time
+// each; a 4,096-op batch took 134 us from execute to completion that way):
+//
+//   sort   LSD radix sort of the (cell key, op | kind) pairs, 8-bit
+//          digits; wave w ranks its chunks of 64 consecutive elements in
+//          order with one ballot per digit bit (wave64 multisplit) and
+//          per-wave digit counters, so the sort is stable and the ops of a
+//          cell stay in TID order
+//   scan   the cell-state monoid over the sorted list, each thread folding
+//          its run of consecutive elements, one block-wide segmented scan
+//          between them
+//   moves  the READs' moves, then (after a barrier) the final writes, from
+//          one list in LDS, by the pipeline's lane groups (move_cells).
+//          The whole batch is in the workgroup, so every segment starts
+//          and ends in it: no carry between tiles, no cross-tile writes.
+//
+// The sorted pairs (64 KB) and the move list (16 B per entry, at most one
+// per element: 128 KB) share one LDS region; with the column list the
+// kernel takes about 133 KB of the CU's 160 KB.
+constexpr uint32_t kSmallWaves = kSmallBlock / 64;
+constexpr uint32_t kSmallItems = kSmallMaxOps / kSmallBlock;  // elements per thread
+constexpr uint32_t kSmallBits = 8;
+constexpr uint32_t kSmallD = 1u << kSmallBits;
+static_assert(kSmallItems * kSmallBlock == kSmallMaxOps, "whole chunks per thread");
+
+struct SmallSortLds {
+    uint32_t sk[kSmallMaxOps], sv[kSmallMaxOps];
+    uint32_t cnt[kSmallWaves][kSmallD];  // per wave and digit: ranked so far, then the wave's offset
+    uint32_t dig_off[kSmallD];           // start of each digit's run
+};
+union SmallLds {
+    SmallSortLds s;
+    uint4 list[kSmallMaxOps];  // READs {op, set, d, key} from the front, final writes {key, set, d, 0} from the back
+};
+
+template <int G, bool VEC16, bool ONE, bool LCOLS>
+__global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView ov, uint32_t key_bits,
+                                                          uint8_t *results, uint32_t res_stride,
+                                                          uint8_t *commit) {
+    __shared__ SmallLds L;
+    __shared__ ColDesc lcols[kColList];
+    __shared__ OpSrc::Cache cache;
+    __shared__ uint32_t sum_ws[kSmallWaves];
+    __shared__ Seg seg_ws[kSmallWaves];
+    const uint32_t n = ov.n_ops, lane = __lane_id(), w = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < ov.n_txns; i += kSmallBlock) commit[i] = 1;  // every txn commits
+    if (LCOLS)
+        for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kSmallBlock) lcols[i] = dt.clist[i];
+    const OpSrc src{dt.tabs, ov.rows, ov.meta};
+    src.prepare(cache);
+    __syncthreads();
+    // chunks of 64 elements per wave; element wbase + c * 64 + lane is the
+    // lane's c-th, in op order before the sort
+    const uint32_t nc = (n + kSmallBlock - 1) / kSmallBlock;
+    const uint32_t wbase = w * 64 * nc;
+    uint32_t k[kSmallItems], v[kSmallItems], r[kSmallItems];
+#pragma unroll
+    for (uint32_t c = 0; c < kSmallItems; ++c) {
+        const uint32_t i = wbase + c * 64 + lane;
+        k[c] = 0xFFFFFFFFu;
+        v[c] = kInvalid;
+        if (c < nc && i < n) src.kv(i, k[c], v[c], cache);
+    }
+    const uint32_t passes = (key_bits + kSmallBits - 1) / kSmallBits;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = p * kSmallBits;
+        for (uint32_t i = threadIdx.x; i < kSmallWaves * kSmallD; i += kSmallBlock) (&L.s.cnt[0][0])[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t c = 0; c < kSmallItems; ++c) {
+            if (c >= nc) break;  // uniform
+            const bool valid = wbase + c * 64 + lane < n;
+            const uint32_t d = (k[c] >> shift) & (kSmallD - 1u);
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (uint32_t b = 0; b < kSmallBits; ++b) {
+                const uint64_t bb = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
+            uint32_t prior = 0;
+            if (valid && lane == leader) {
+                prior = L.s.cnt[w][d];
+                L.s.cnt[w][d] = prior + (uint32_t)__popcll(peers);
+            }
+            prior = __shfl(prior, (int)leader);
+            r[c] = prior + lanemask_lt_popc(peers);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        // thread d < kSmallD owns digit d: each wave's offset inside the
+        // digit's run, then the runs' starts (an exclusive scan in thread
+        // order, which is digit order)
+        uint32_t run = 0;
+        if (threadIdx.x < kSmallD)
+            for (uint32_t q = 0; q < kSmallWaves; ++q) {
+                const uint32_t t = L.s.cnt[q][threadIdx.x];
+                L.s.cnt[q][threadIdx.x] = run;
+                run += t;
+            }
+        uint32_t all;
+        const uint32_t ex = block_exclusive_sum<kSmallBlock>(run, &all, sum_ws);
+        if (threadIdx.x < kSmallD) L.s.dig_off[threadIdx.x] = ex;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t c = 0; c < kSmallItems; ++c) {
+            if (c >= nc) break;
+            if (wbase + c * 64 + lane < n) {
+                const uint32_t d = (k[c] >> shift) & (kSmallD - 1u);
+                const uint32_t pos = L.s.dig_off[d] + L.s.cnt[w][d] + r[c];
+                L.s.sk[pos] = k[c];
+                L.s.sv[pos] = v[c];
+            }
+        }
+        __syncthreads();
+        if (p + 1 < passes) {
+#pragma unroll
+            for (uint32_t c = 0; c < kSmallItems; ++c) {
+                const uint32_t i = wbase + c * 64 + lane;
+                if (c < nc && i < n) {
+                    k[c] = L.s.sk[i];
+                    v[c] = L.s.sv[i];
+                }
+            }
+        }
+    }
+    if (passes == 0) {  // one cell: the op order is the sorted order
+#pragma unroll
+        for (uint32_t c = 0; c < kSmallItems; ++c) {
+            const uint32_t i = wbase + c * 64 + lane;
+            if (c < nc && i < n) {
+                L.s.sk[i] = k[c];
+                L.s.sv[i] = v[c];
+            }
+        }
+        __syncthreads();
+    }
+    // the sorted list, thread t taking positions [t * nc, t * nc + nc);
+    // positions past n read as all-ones keys (never a real key)
+    const uint32_t first = threadIdx.x * nc;
+#pragma unroll
+    for (uint32_t j = 0; j < kSmallItems; ++j) {
+        const bool in = j < nc && first + j < n;
+        k[j] = in ? L.s.sk[first + j] : 0xFFFFFFFFu;
+        v[j] = in ? L.s.sv[first + j] : kInvalid;
+    }
+    const uint32_t prev = first > 0 && first - 1 < n ? L.s.sk[first - 1] : 0xFFFFFFFFu;
+    const uint32_t next = first + nc < n ? L.s.sk[first + nc] : 0xFFFFFFFFu;
+    Seg agg{0u, kNoSet, 0u};
+#pragma unroll
+    for (uint32_t j = 0; j < kSmallItems; ++j) {
+        if (j >= nc || first + j >= n) break;
+        const St s = elem_state(v[j], ov.args);
+        const bool h = k[j] != (j ? k[j - 1] : prev);
+        agg = seg_op(agg, Seg{h ? 1u : 0u, s.set, s.d});
+    }
+    Seg tot;
+    // (its barriers also order every read of sk / sv above before the list
+    // overwrites them)
+    const Seg ex0 = block_seg_exclusive<kSmallBlock>(agg, &tot, seg_ws);
+    auto walk = [&](auto &&visit) {
+        St run{ex0.set, ex0.d};
+#pragma unroll
+        for (uint32_t j = 0; j < kSmallItems; ++j) {
+            if (j >= nc || first + j >= n) break;
+            const St s = elem_state(v[j], ov.args);
+            const bool h = k[j] != (j ? k[j - 1] : prev);
+            const St ex = h ? St{kNoSet, 0u} : run;
+            run = h ? s : combine(run, s);
+            const bool rd = (v[j] >> 30) == LTPMG_OP_READ && !(v[j] & kInvalid);
+            const bool last = k[j] != (j + 1 < nc ? k[j + 1] : next);
+            visit(j, ex, run, rd, last && run.set != kNoSet);
+        }
+    };
+    uint32_t n_rd_t = 0, n_wr_t = 0;
+    walk([&](uint32_t, St, St, bool rd, bool wr) {
+        n_rd_t += rd;
+        n_wr_t += wr;
+    });
+    // reads + write entries <= n <= kSmallMaxOps < 2^16: one word of the scan
+    uint32_t both;
+    const uint32_t pos = block_exclusive_sum<kSmallBlock>(n_rd_t | (n_wr_t << 16), &both, sum_ws);
+    const uint32_t n_rd = both & 0xFFFFu, n_wr = both >> 16;
+    uint32_t rp = pos & 0xFFFFu, wp = pos >> 16;
+    walk([&](uint32_t j, St ex, St run, bool rd, bool wr) {
+        if (rd) L.list[rp++] = make_uint4(v[j] & kOpMask, ex.set, ex.d, k[j]);
+        if (wr) L.list[kSmallMaxOps - 1 - wp++] = make_uint4(k[j], run.set, run.d, 0u);
+    });
+    __syncthreads();
+    move_cells<G, VEC16, ONE, LCOLS>(n_rd, [&](uint32_t q, uint32_t &key_out, St &st, uint32_t &op) {
+        const uint4 e = L.list[q];
+        key_out = e.w;
+        st = St{e.y, e.z};
+        op = e.x;
+    }, dt, dt.dec, ov.args, results, res_stride, threadIdx.x / G, kSmallBlock / G, lcols);
+    __syncthreads();  // every READ of the batch is done
+    move_cells<G, VEC16, ONE, LCOLS>(n_wr, [&](uint32_t q, uint32_t &key_out, St &st, uint32_t &op) {
+        const uint4 e = L.list[kSmallMaxOps - 1 - q];
+        key_out = e.x;
+        st = St{e.y, e.z};
+        op = kCellOp;
+    }, dt, dt.dec, ov.args, results, res_stride, threadIdx.x / G, kSmallBlock / G, lcols);
+}
+
 }  // namespace
 
 size_t sort_partials_needed() { return (size_t)kMaxSortChunks << kMaxDigitBits; }
@@ -1111,26 +1321,29 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint
     hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, keys, vals);
 }
 
-#define LTPMG_MO_LAUNCH(KERNEL, G_, LC, GRID, ...)                                                 \
+// G lanes per cell move (a 16-byte chunk each) from the result stride; ONE:
+// one chunk per lane covers a slot
+#define LTPMG_MO_LAUNCH_B(KERNEL, G_, LC, GRID, BLOCK, ...)                                         \
     do {                                                                                           \
         if (res_stride == G_ * 16)                                                                 \
-            hipLaunchKernelGGL((KERNEL<G_, true, true, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__);  \
+            hipLaunchKernelGGL((KERNEL<G_, true, true, LC>), GRID, BLOCK, 0, s, __VA_ARGS__);         \
         else                                                                                       \
-            hipLaunchKernelGGL((KERNEL<G_, true, false, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+            hipLaunchKernelGGL((KERNEL<G_, true, false, LC>), GRID, BLOCK, 0, s, __VA_ARGS__);        \
     } while (0)
-#define LTPMG_MO_DISPATCH(KERNEL, LC, GRID, ...)                                                \
+#define LTPMG_MO_DISPATCH_B(KERNEL, LC, GRID, BLOCK, ...)                                       \
     do {                                                                                        \
         if (res_stride >= 128)                                                                  \
-            LTPMG_MO_LAUNCH(KERNEL, 8, LC, GRID, __VA_ARGS__);                                  \
+            LTPMG_MO_LAUNCH_B(KERNEL, 8, LC, GRID, BLOCK, __VA_ARGS__);                         \
         else if (res_stride >= 64)                                                              \
-            LTPMG_MO_LAUNCH(KERNEL, 4, LC, GRID, __VA_ARGS__);                                  \
+            LTPMG_MO_LAUNCH_B(KERNEL, 4, LC, GRID, BLOCK, __VA_ARGS__);                         \
         else if (res_stride >= 32)                                                              \
-            LTPMG_MO_LAUNCH(KERNEL, 2, LC, GRID, __VA_ARGS__);                                  \
+            LTPMG_MO_LAUNCH_B(KERNEL, 2, LC, GRID, BLOCK, __VA_ARGS__);                         \
         else if (res_stride >= 16)                                                              \
-            LTPMG_MO_LAUNCH(KERNEL, 1, LC, GRID, __VA_ARGS__);                                  \
+            LTPMG_MO_LAUNCH_B(KERNEL, 1, LC, GRID, BLOCK, __VA_ARGS__);                         \
         else                                                                                    \
-            hipLaunchKernelGGL((KERNEL<1, false, false, LC>), GRID, dim3(kBlock), 0, s, __VA_ARGS__); \
+            hipLaunchKernelGGL((KERNEL<1, false, false, LC>), GRID, BLOCK, 0, s, __VA_ARGS__);  \
     } while (0)
+#define LTPMG_MO_DISPATCH(KERNEL, LC, GRID, ...) LTPMG_MO_DISPATCH_B(KERNEL, LC, GRID, dim3(kBlock), __VA_ARGS__)
 
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted) {
     const uint32_t n = ov.n_ops;
@@ -1193,6 +1406,17 @@ void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, Order
     const uint32_t blocks = (n_tiles * 8 + kBlock - 1) / kBlock;
     LTPMG_MO_DISPATCH(k_mo_final_writes, false, dim3(blocks), ob.wlist, ob.wtotal, dt, ov.args, commit,
                       ov.n_txns);
+}
+
+void launch_mo_small(hipStream_t s, const DevTables &dt, OpView ov, uint32_t key_bits, uint8_t *results,
+                     uint32_t res_stride, uint8_t *commit) {
+    if (ov.n_ops == 0 || ov.n_ops > kSmallMaxOps || key_bits > 32) return;
+    if (dt.n_clist >= dt.dec.n_cols_all)
+        LTPMG_MO_DISPATCH_B(k_mo_small, true, dim3(1), dim3(kSmallBlock), dt, ov, key_bits, results, res_stride,
+                            commit);
+    else
+        LTPMG_MO_DISPATCH_B(k_mo_small, false, dim3(1), dim3(kSmallBlock), dt, ov, key_bits, results, res_stride,
+                            commit);
 }
 
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,

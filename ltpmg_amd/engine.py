@@ -22,6 +22,8 @@ OP_READ, OP_UPDATE, OP_RMW_ADD, OP_ROW = 0, 1, 2, 3
 ROW_INSERT, ROW_DELETE = 0, 1  # column field of an OP_ROW op
 MODE_A, MODE_O = 0, 1
 EXEC_PROFILE = 0x100
+EXEC_PIPELINE = 0x200  # ordered batches always take the multi-kernel pipeline
+SMALL_MAX_OPS = 8192   # ordered batches up to this size (no row ops) run as one launch
 CFG_PROFILE = 1
 
 STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EBUDGET", -4: "EHIP", -5: "ESTATE",
@@ -124,7 +126,7 @@ STATIC_MAX_OPS = 1 << 18  # LTPMG_STATIC_MAX_OPS
 class _StreamPlan(C.Structure):
     _fields_ = [("merge_max_ops", C.c_uint32), ("fused_max_ops", C.c_uint32),
                 ("fused_tile_ops", C.c_uint32), ("fused_wgs_per_cu", C.c_uint32),
-                ("use_graph", C.c_uint32), ("flags", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("flags", C.c_uint32), ("small_max_ops", C.c_uint32)]
 
 
 STREAM_KEEP_RESULTS = 0x1
@@ -588,8 +590,10 @@ class Engine:
                     self._lib.ltpmg_submit_staged(self._h, slot, 1 if trusted else 0))
         self._batches[slot] = b
 
-    def execute(self, mode: int, slot: int = 0, profile: bool = False) -> None:
-        m = mode | (EXEC_PROFILE if profile else 0)
+    def execute(self, mode: int, slot: int = 0, profile: bool = False, pipeline: bool = False) -> None:
+        """pipeline: an ordered batch takes the multi-kernel pipeline even when
+        it is small enough for the one-launch kernel."""
+        m = mode | (EXEC_PROFILE if profile else 0) | (EXEC_PIPELINE if pipeline else 0)
         self._check("ltpmg_execute", self._lib.ltpmg_execute(self._h, slot, m))
 
     def idle(self, slot: int = 0) -> bool:
@@ -632,9 +636,9 @@ class Engine:
                 "kernel_launches": launches, "hbm_allocated": int(s.hbm_allocated),
                 "inserted": int(s.inserted)}
 
-    def run(self, batch: Batch, mode: int, slot: int = 0, results: bool = True):
+    def run(self, batch: Batch, mode: int, slot: int = 0, results: bool = True, pipeline: bool = False):
         self.submit(batch, slot)
-        self.execute(mode, slot)
+        self.execute(mode, slot, pipeline=pipeline)
         return self.fetch(slot, results)
 
     def selftest_sort(self, keys: np.ndarray, vals: np.ndarray, key_bits: int):
@@ -707,21 +711,24 @@ class StreamPlan:
     fused_wgs_per_cu: int = 2
     use_graph: bool = True
     sync_each: bool = False
+    small_max_ops: int = SMALL_MAX_OPS
 
     def c_struct(self) -> _StreamPlan:
         return _StreamPlan(self.merge_max_ops, self.fused_max_ops, self.fused_tile_ops,
                            self.fused_wgs_per_cu, int(self.use_graph),
-                           PLAN_SYNC_EACH if self.sync_each else 0)
+                           PLAN_SYNC_EACH if self.sync_each else 0, self.small_max_ops)
 
     @staticmethod
     def from_c(p: _StreamPlan) -> "StreamPlan":
         return StreamPlan(p.merge_max_ops, p.fused_max_ops, p.fused_tile_ops, p.fused_wgs_per_cu,
-                          bool(p.use_graph), bool(p.flags & PLAN_SYNC_EACH))
+                          bool(p.use_graph), bool(p.flags & PLAN_SYNC_EACH), p.small_max_ops)
 
     @staticmethod
     def plain() -> "StreamPlan":
-        """Every batch alone, three-kernel / ordered pipeline, host wait after each."""
-        return StreamPlan(0, 0, 2048, 1, False, True)
+        """Every batch alone as ltpmg_execute runs it (three-kernel pipeline in
+        lowest-TID-wins; ordered pipeline, or the one-launch kernel for small
+        ordered batches), host wait after each."""
+        return StreamPlan(0, 0, 2048, 1, False, True, SMALL_MAX_OPS)
 
     @staticmethod
     def default() -> "StreamPlan":

@@ -26,9 +26,9 @@ def _setup(e: Engine, schema: dict, seed: int) -> orc.Image:
     return orc.Image(schema).fill(seed)
 
 
-def _compare(e: Engine, im: orc.Image, batch: Batch, mode: int, check_image=True):
+def _compare(e: Engine, im: orc.Image, batch: Batch, mode: int, check_image=True, pipeline=False):
     rs = e.result_stride
-    commit, res = e.run(batch, mode)
+    commit, res = e.run(batch, mode, pipeline=pipeline)
     oc, ores = orc.replay(im, batch, mode, rs)
     assert commit.shape == oc.shape
     bad = np.nonzero(commit != oc)[0]
@@ -219,10 +219,12 @@ def test_result_strides(mode, widths):
 
 
 @pytest.mark.parametrize("n_ops", [1, 2047, 2048, 2049, 4096 + 255])
-@pytest.mark.parametrize("mode", BOTH)
-def test_tile_boundaries(n_ops, mode):
+@pytest.mark.parametrize("mode,pipeline", [(MODE_A, False), (MODE_O, True), (MODE_O, False)],
+                         ids=["lowest_tid_wins", "ordered_pipeline", "ordered_one_launch"])
+def test_tile_boundaries(n_ops, mode, pipeline):
     # one-op txns; when there is room, a 255-op txn starts at op 1900 and
-    # straddles the first 2048-op tile boundary
+    # straddles the first 2048-op tile boundary (of the pipeline's tiles;
+    # batches this small otherwise take the one-launch ordered path)
     rng = np.random.default_rng(n_ops)
 
     def single():
@@ -236,7 +238,7 @@ def test_tile_boundaries(n_ops, mode):
         txns = [single() for _ in range(n_ops)]
     with Engine() as e:
         im = _setup(e, {0: (5000, [16])}, 3)
-        _compare(e, im, Batch.from_txns(txns), mode)
+        _compare(e, im, Batch.from_txns(txns), mode, pipeline=pipeline)
 
 
 def test_max_length_txns():
