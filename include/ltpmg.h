@@ -96,9 +96,9 @@ extern "C" {
 /* or-ed into the mode: an ordered batch always runs the multi-kernel
  * pipeline, never the one-launch path for small batches (below) */
 #define LTPMG_EXEC_PIPELINE 0x200u
-/* Ordered batches of at most this many ops that touch no append table run
- * as one launch of one workgroup (sort, scan and cell moves in LDS) instead
- * of the multi-kernel pipeline, with the same results. */
+/* Ordered batches of at most this many ops run as one launch of one
+ * workgroup (sort, scan, cell moves and row ops in LDS) instead of the
+ * multi-kernel pipeline, with the same results. */
 #define LTPMG_SMALL_MAX_OPS 8192u
 
 #define LTPMG_CFG_PROFILE 1u /* record per-kernel HIP events on every execute */
@@ -248,8 +248,8 @@ int ltpmg_pinned_free(void *p);
  * across slots: a batch's reads and writes of cells and lock words wait for
  * the previously executed batch.  In the ordered mode the key build, sort
  * and scan of a batch only touch its slot and overlap the previous batch;
- * an ordered batch of at most LTPMG_SMALL_MAX_OPS ops without row ops runs
- * as one launch (unless LTPMG_EXEC_PIPELINE is or-ed into the mode).
+ * an ordered batch of at most LTPMG_SMALL_MAX_OPS ops runs as one launch
+ * (unless LTPMG_EXEC_PIPELINE is or-ed into the mode).
  * Replaces the reference's one-batch-at-a-time epoch loop
  * (src/scheduler.cpp:44-59) for the serving form. */
 int ltpmg_execute(ltpmg_engine *e, uint32_t slot, uint32_t mode);

@@ -26,10 +26,10 @@
  *                     tile, fused_wgs_per_cu workgroups per CU); a larger
  *                     one runs the three-kernel pipeline.
  *   small groups      an ordered launch group of at most small_max_ops
- *                     ops (LTPMG_SMALL_MAX_OPS at most) that touches no
- *                     append table runs as one launch of one workgroup --
- *                     sort, scan and cell moves fused in LDS -- instead of
- *                     the multi-kernel ordered pipeline.
+ *                     ops (LTPMG_SMALL_MAX_OPS at most) runs as one launch
+ *                     of one workgroup -- sort, scan, cell moves and row
+ *                     ops fused in LDS -- instead of the multi-kernel
+ *                     ordered pipeline.
  *   use_graph         the plan's whole launch sequence is captured once as
  *                     a hipGraph and replayed by each run.
  *
@@ -78,7 +78,7 @@ typedef struct {
     uint32_t use_graph;        /* capture the launch sequence once and replay it */
     uint32_t flags;            /* LTPMG_PLAN_* */
     uint32_t small_max_ops;    /* ordered mode: one-launch kernel for groups with <= this many ops
-                                  (at most LTPMG_SMALL_MAX_OPS count) and no row ops */
+                                  (at most LTPMG_SMALL_MAX_OPS count) */
 } ltpmg_stream_plan;
 
 /* Uploads a stream of n_batches batches.  all holds them back to back:

@@ -698,8 +698,9 @@ int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
 // presence bytes before its sort, so it waits for the previous batch first
 // (then its sort does not overlap that batch), and sorts by a private copy of
 // the meta words in which ops on absent rows are invalid.
-// small_max: a batch of 1 .. min(small_max, kSmallMaxOps) ops without row
-// ops runs as the one-launch kernel instead (every stage in one workgroup).
+// small_max: a batch of 1 .. min(small_max, kSmallMaxOps) ops runs as the
+// one-launch kernel instead (every stage, row ops included, in one
+// workgroup).
 template <class Rec, class Wait>
 int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBufs &ob,
                   uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables,
@@ -708,11 +709,11 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBuf
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
     DevTables dt = dev_tables(e);
-    if (!rb && ov_in.n_ops >= 1 && ov_in.n_ops <= std::min(small_max, kSmallMaxOps)) {
+    if (ov_in.n_ops >= 1 && ov_in.n_ops <= std::min(small_max, kSmallMaxOps)) {
         rc = wait_tables();
         if (rc != LTPMG_OK) return rc;
         rec.begin("mo_small");
-        launch_mo_small(st, dt, ov_in, key_bits, results, rs, commit);
+        launch_mo_small(st, dt, ov_in, key_bits, results, rs, commit, rb);
         rec.end();
         return LTPMG_OK;
     }
@@ -2170,7 +2171,7 @@ int ltpmg_stream_tune(ltpmg_engine *e, ltpmg_stream *s, ltpmg_stream_plan *best,
         std::vector<uint32_t> o_small;
         for (uint32_t b = 0; b < s->n_batches; ++b) {
             const uint32_t ops = s->op_base[b + 1] - s->op_base[b];
-            if (s->modes[b] == LTPMG_MODE_O && ops <= kSmallMaxOps && !s->rows[b].has_append) o_small.push_back(ops);
+            if (s->modes[b] == LTPMG_MODE_O && ops <= kSmallMaxOps) o_small.push_back(ops);
         }
         p.small_max_ops = 0;
         if (!o_small.empty()) {
@@ -2187,8 +2188,7 @@ int ltpmg_stream_tune(ltpmg_engine *e, ltpmg_stream *s, ltpmg_stream_plan *best,
             trials += 2;
             std::sort(o_small.begin(), o_small.end());
             auto eligible = [&](uint32_t b) {
-                return s->modes[b] == LTPMG_MODE_O && !s->rows[b].has_append &&
-                       s->op_base[b + 1] - s->op_base[b] <= kSmallMaxOps;
+                return s->modes[b] == LTPMG_MODE_O && s->op_base[b + 1] - s->op_base[b] <= kSmallMaxOps;
             };
             double best_ms = 0;
             for (uint32_t b = 0; b < s->n_batches; ++b)

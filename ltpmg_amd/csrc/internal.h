@@ -364,13 +364,16 @@ void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderB
 void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob,
                             uint32_t res_stride, uint8_t *commit);  // also sets every commit flag
 uint32_t seg_tiles(uint32_t n);
-// An ordered batch of 1 .. kSmallMaxOps ops without row ops in one launch of
-// one workgroup of kSmallBlock threads (sort, scan and moves in LDS); also
-// sets every commit flag.  The caller checks the size.
+// An ordered batch of 1 .. kSmallMaxOps ops in one launch of one workgroup
+// of kSmallBlock threads (sort, scan and moves in LDS); also sets every
+// commit flag.  rb (nullable): the batch's row-op buffers when it touches an
+// append table -- the kernel then also finds each op's presence (found),
+// clears the rows DELETEs remove and appends the INSERTs' rows, as the row-op
+// kernels do.  The caller checks the size and, for INSERTs, the capacity.
 constexpr uint32_t kSmallBlock = 1024;
 constexpr uint32_t kSmallMaxOps = LTPMG_SMALL_MAX_OPS;
 void launch_mo_small(hipStream_t s, const DevTables &dt, OpView ov, uint32_t key_bits, uint8_t *results,
-                     uint32_t res_stride, uint8_t *commit);
+                     uint32_t res_stride, uint8_t *commit, const RowBufs *rb);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
                              uint32_t *out, uint32_t n, uint32_t *tile_scratch);

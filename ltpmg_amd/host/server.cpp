@@ -149,9 +149,14 @@ extern "C" int ltpmg_serve_run(ltpmg_engine *e, const ltpmg_serve_cfg *cfg, cons
             commit_buf.resize(f.n);
             const bool want_res = rec && rec->results;
             if (want_res) res_buf.resize((size_t)f.n_ops * stride);
-            rc = ltpmg_fetch_results(e, f.slot, commit_buf.data(), want_res ? res_buf.data() : nullptr,
-                                     want_res ? res_buf.size() : 0);
-            if (rc != LTPMG_OK) return rc;
+            if (cfg->mode == LTPMG_MODE_O && !want_res) {
+                // the ordered mode commits every transaction: no flags to copy back
+                std::fill(commit_buf.begin(), commit_buf.end(), (uint8_t)1);
+            } else {
+                rc = ltpmg_fetch_results(e, f.slot, commit_buf.data(), want_res ? res_buf.data() : nullptr,
+                                         want_res ? res_buf.size() : 0);
+                if (rc != LTPMG_OK) return rc;
+            }
             ltpmg_batch_stats st;
             if (ltpmg_stats(e, f.slot, &st) == LTPMG_OK) device_ms += st.exec_ms;
             std::vector<uint32_t> retry;

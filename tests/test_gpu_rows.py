@@ -43,9 +43,9 @@ def _check_tables(e: Engine, im: orc.Image, what: str = ""):
             assert bad.size == 0, f"{what}table {t}: presence differs at rows {bad[:8]}"
 
 
-def _run_compare(e: Engine, im: orc.Image, batch: Batch, mode: int, what: str = ""):
+def _run_compare(e: Engine, im: orc.Image, batch: Batch, mode: int, what: str = "", pipeline: bool = False):
     rs = e.result_stride
-    commit, res = e.run(batch, mode)
+    commit, res = e.run(batch, mode, pipeline=pipeline)
     status = e.fetch_status()
     oc, ores, ofound = orc.replay(im, batch, mode, rs, with_found=True)
     bad = np.nonzero(commit != oc)[0]
@@ -64,16 +64,23 @@ def _run_compare(e: Engine, im: orc.Image, batch: Batch, mode: int, what: str = 
     return oc, ofound
 
 
-@pytest.mark.parametrize("mode", BOTH)
+# batches of up to LTPMG_SMALL_MAX_OPS ops take the one-launch ordered
+# kernel unless the multi-kernel pipeline is asked for
+PATHS = [(MODE_A, False), (MODE_O, True), (MODE_O, False)]
+PATH_IDS = ["lowest_tid_wins", "ordered_pipeline", "ordered_one_launch"]
+
+
+@pytest.mark.parametrize("mode,pipeline", PATHS, ids=PATH_IDS)
 @pytest.mark.parametrize("seed", range(3))
-def test_row_ops_random_batches(mode, seed):
+def test_row_ops_random_batches(mode, pipeline, seed):
     rng = np.random.default_rng(seed)
     schema = {0: (12_000, [4, 8, 16, 100, 32])}
     with Engine() as e:
         im = _setup(e, schema, {0: 3000})
         for k in range(3):
             b = random_row_batch(rng, 2000, 4000, 5, max_len=6, insert_pct=15, delete_pct=10)
-            oc, found = _run_compare(e, im, b, mode, f"batch {k}: ")
+            assert b.n_ops <= 8192
+            oc, found = _run_compare(e, im, b, mode, f"batch {k}: ", pipeline)
             assert found.sum() > 0 and (mode == MODE_O or 0 < oc.sum() < len(oc))
         _check_tables(e, im)
 
@@ -91,8 +98,8 @@ def test_row_ops_over_many_tiles(mode):
         _check_tables(e, im)
 
 
-@pytest.mark.parametrize("mode", BOTH)
-def test_row_ops_with_ordinary_and_two_append_tables(mode):
+@pytest.mark.parametrize("mode,pipeline", PATHS, ids=PATH_IDS)
+def test_row_ops_with_ordinary_and_two_append_tables(mode, pipeline):
     rng = np.random.default_rng(11)
     schema = {0: (5000, [4, 12]), 3: (4000, [8, 100]), 5: (3000, [16])}
     parts = [random_batch(rng, 600, 5000, 2, table=0),
@@ -109,7 +116,7 @@ def test_row_ops_with_ordinary_and_two_append_tables(mode):
     b = Batch.from_txns([txns[i] for i in order])
     with Engine() as e:
         im = _setup(e, schema, {3: 2000, 5: 1500})
-        _run_compare(e, im, b, mode)
+        _run_compare(e, im, b, mode, pipeline=pipeline)
         _check_tables(e, im)
 
 

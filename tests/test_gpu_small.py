@@ -151,19 +151,87 @@ def test_tables_past_the_column_list():
     _both_paths(schema, 0x4C54504D4700000B, b)
 
 
-def test_larger_and_row_op_batches_take_the_pipeline():
+def test_larger_batches_take_the_pipeline():
     rng = np.random.default_rng(11)
     big = _single_op_txns(rng, SMALL_MAX_OPS + 1, 3000, 3)
     _, _, _, kernels = _run({0: (3000, [100, 4, 16])}, 1, big, pipeline=False)
     assert "mo_small" not in kernels and "mo_apply_reads" in kernels
+
+
+def _run_rows(schema: dict, append: dict, batches, pipeline: bool):
+    """Runs the batches in order on fresh append tables; checks each against
+    the oracle (commit flags, per-op status, result slots, inserted count)
+    and returns what every batch left: statuses, results, cursors, presence
+    and columns."""
     with Engine() as e:
-        e.create_append_table(0, 400, [16, 8], 200)
-        e.fill_table(0, 4)
-        b = random_row_batch(rng, 50, 200, 2)
-        e.submit(b)
-        e.execute(MODE_O, profile=True)
-        e.wait()
-        assert "mo_small" not in e.stats()["kernels"]
+        for t, (n, widths) in sorted(schema.items()):
+            if t in append:
+                e.create_append_table(t, n, widths, append[t])
+            else:
+                e.create_table(t, n, widths)
+            e.fill_table(t, 0x5EED0006)
+        im = orc.Image(schema, append=append).fill(0x5EED0006)
+        out = []
+        for b in batches:
+            e.submit(b)
+            e.execute(MODE_O, profile=True, pipeline=pipeline)
+            commit, res = e.fetch()
+            status = e.fetch_status()
+            st = e.stats()
+            assert ("mo_small" in st["kernels"]) == (not pipeline)
+            oc, ores, ofound = orc.replay(im, b, MODE_O, e.result_stride, with_found=True)
+            assert (commit == oc).all()
+            bad = np.nonzero(status != ofound)[0]
+            assert bad.size == 0, f"status differs at ops {bad[:10]}"
+            m = orc.read_mask(b, oc)
+            if not (res[m] == ores[m]).all():
+                idx = np.nonzero(m)[0]
+                wrong = idx[np.any(res[m] != ores[m], axis=1)]
+                pytest.fail(f"{wrong.size} result slots differ, first ops {wrong[:8]}")
+            is_row = (b.meta & 3) == 3
+            assert st["inserted"] == int((ofound & is_row & (((b.meta >> 2) & 63) == 0)).sum())
+            out.append((status, np.where(m[:, None], res, 0)))
+        tables = {}
+        for t, (n, widths) in im.schema.items():
+            for c in range(len(widths)):
+                got = e.read_column(t, c)
+                assert (got == im.cols[t][c]).all(), f"table {t} col {c} differs from the oracle"
+                tables[(t, c)] = got
+            if t in im.present:
+                assert e.table_rows(t) == im.cursor[t]
+                assert (e.read_presence(t) == im.present[t]).all()
+                tables[(t, "present")] = e.read_presence(t)
+        return out, tables
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_row_op_batches_match_oracle_and_pipeline(seed):
+    # INSERTs, DELETEs (several of one row in a batch), cell ops on an append
+    # table and an ordinary one, batch after batch, both paths
+    rng = np.random.default_rng(100 + seed)
+    schema = {0: (3000, [4, 12]), 2: (6000, [8, 100, 16]), 4: (2500, [16])}
+    append = {2: 2500, 4: 1200}
+    batches = []
+    for k in range(4):
+        parts = [random_batch(rng, 150, 3000, 2, table=0),
+                 random_row_batch(rng, 500, 2500 + 300 * k, 3, table=2, insert_pct=20, delete_pct=15),
+                 random_row_batch(rng, 300, 1200, 1, table=4, delete_pct=35)]
+        txns = []
+        for p in parts:
+            for t in range(p.n_txns):
+                a, z = int(p.txn_off[t]), int(p.txn_off[t + 1])
+                txns.append([(int(p.meta[i] & 3), int((p.meta[i] >> 8) & 15), int((p.meta[i] >> 2) & 63),
+                              int(p.rows[i]), int(p.args[i])) for i in range(a, z)])
+        b = Batch.from_txns([txns[i] for i in rng.permutation(len(txns))])
+        assert b.n_ops <= SMALL_MAX_OPS
+        batches.append(b)
+    log(f"{len(batches)} row-op batches, {[b.n_ops for b in batches]} ops")
+    one = _run_rows(schema, append, batches, pipeline=False)
+    pipe = _run_rows(schema, append, batches, pipeline=True)
+    for (s1, r1), (s2, r2) in zip(one[0], pipe[0]):
+        assert (s1 == s2).all() and (r1 == r2).all()
+    for k in one[1]:
+        assert (one[1][k] == pipe[1][k]).all(), f"{k} differs between the paths"
 
 
 def test_stream_of_small_batches():
