@@ -200,8 +200,12 @@ uint32_t ltpmg_result_stride(const ltpmg_engine *e);
  * on the slot's stream (replaces GPUlauncher::txn_kernel_launcher,
  * cuda/src/gpulauncher.cu:755-1116); wait blocks until the slot is idle;
  * fetch copies the commit flags (n_txns bytes) and the read results
- * (n_ops * ltpmg_result_stride() bytes, slot i holds op i's READ) back.  The
- * reference has no results path (Global_Txn_Result is never written). */
+ * (n_ops * ltpmg_result_stride() bytes, slot i holds op i's READ: its
+ * column's bytes, then zeros; an INSERT's: its row id, then zeros) back.  The
+ * kernels write a slot only as far as the column's own slot size and the
+ * fetch writes the zeros past it on the host, so the padding of wide result
+ * strides never crosses HBM.  The reference has no results path
+ * (Global_Txn_Result is never written). */
 int ltpmg_submit_batch(ltpmg_engine *e, uint32_t slot, const ltpmg_batch *b);
 /* The checks submit applies, without submitting (for a pool of requests). */
 int ltpmg_check_batch(ltpmg_engine *e, const ltpmg_batch *b);
@@ -288,8 +292,9 @@ int ltpmg_selftest_copy(ltpmg_engine *e, uint64_t n_bytes, uint32_t iters, doubl
  * the CC + execute kernels of ltpmg_execute on a batch that is already in
  * device memory (dev->txn_off / rows / meta / args are device pointers,
  * n_txns / n_ops host values) and writes dev_commit (n_txns bytes) and
- * dev_results (n_ops * ltpmg_result_stride() bytes), both device buffers
- * the caller owns, on hip_stream (a hipStream_t; NULL = the null stream).
+ * dev_results (n_ops * ltpmg_result_stride() bytes, each READ's slot written
+ * to its end, zeros past the column), both device buffers the caller owns,
+ * on hip_stream (a hipStream_t; NULL = the null stream).
  * It allocates nothing: its sort, scan and tile scratch is one static
  * device array per GPU of about 5 MiB, held until the process ends, which
  * bounds a batch to LTPMG_STATIC_MAX_OPS ops (larger ones fail with

@@ -700,11 +700,13 @@ int order_key_bits(ltpmg_engine *e, uint32_t *key_bits) {
 // the meta words in which ops on absent rows are invalid.
 // small_max: a batch of 1 .. min(small_max, kSmallMaxOps) ops runs as the
 // one-launch kernel instead (every stage, row ops included, in one
-// workgroup).
+// workgroup).  pad: READ result slots are written to their end; otherwise
+// only each READ's column slot, and a fetch zeroes the rest on the host
+// (zero_result_padding).
 template <class Rec, class Wait>
 int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBufs &ob,
                   uint8_t *commit, uint8_t *results, uint32_t rs, Rec &rec, Wait wait_tables,
-                  RowBufs *rb = nullptr, uint32_t small_max = 0) {
+                  RowBufs *rb = nullptr, uint32_t small_max = 0, bool pad = false) {
     uint32_t key_bits = 0;
     int rc = order_key_bits(e, &key_bits);
     if (rc != LTPMG_OK) return rc;
@@ -768,7 +770,7 @@ int launch_mode_o(ltpmg_engine *e, hipStream_t st, const OpView &ov_in, OrderBuf
     launch_mo_warm_seeds(st, ov, ob);
     rec.end();
     rec.begin("mo_apply_reads");
-    launch_mo_apply_reads(st, dt, ov, ob, sorted, results, rs);
+    launch_mo_apply_reads(st, dt, ov, ob, sorted, results, rs, pad);
     rec.end();
     rec.begin("mo_final_writes");
     launch_mo_final_writes(st, dt, ov, ob, rs, commit);
@@ -788,6 +790,44 @@ struct NoRec {
     void begin(const char *) {}
     void end() {}
 };
+
+// The ordered kernels write a READ's result slot only as far as its column's
+// slot (an INSERT's, its first 16 bytes), so that the zeros past it never
+// cross HBM; the fetch zeroes the rest of each slot here, on the host, from
+// the batch's meta words, so callers get the same bytes as before.  (The
+// lowest-TID-wins kernels write whole slots; zeroing again changes nothing.)
+int zero_result_padding(ltpmg_engine *e, const uint16_t *d_meta, uint64_t n_ops, uint8_t *results,
+                        uint32_t stride) {
+    if (n_ops == 0) return LTPMG_OK;
+    std::vector<uint16_t> meta(n_ops);
+    HIP_TRY(e, hipMemcpy(meta.data(), d_meta, n_ops * 2, hipMemcpyDeviceToHost));
+    uint32_t keep[LTPMG_MAX_TABLES][LTPMG_MAX_COLS + 1];  // defined bytes per (table, column)
+    for (uint32_t t = 0; t < LTPMG_MAX_TABLES; ++t)
+        for (uint32_t c = 0; c <= LTPMG_MAX_COLS; ++c)
+            keep[t][c] = e->tables[t].live && c < e->tables[t].n_cols ? std::min(e->tables[t].slot[c], stride) : stride;
+    auto run = [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) {
+            const uint16_t m = meta[i];
+            const uint32_t kind = m & 3u, col = (m >> 2) & 63u, tab = (m >> 8) & 15u;
+            uint32_t from = stride;
+            if (m >> 12) continue;  // not a valid op: no defined result
+            if (kind == LTPMG_OP_READ) from = keep[tab][col];
+            else if (kind == LTPMG_OP_ROW && col == LTPMG_ROW_INSERT) from = std::min(16u, stride);
+            if (from < stride) std::memset(results + i * stride + from, 0, stride - from);
+        }
+    };
+    const uint64_t kChunk = 1u << 20;
+    if (n_ops <= kChunk) {
+        run(0, n_ops);
+    } else {
+        std::vector<std::thread> th;
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        for (unsigned k = 0; k < nt; ++k)
+            th.emplace_back(run, n_ops * k / nt, n_ops * (k + 1) / nt);
+        for (auto &t : th) t.join();
+    }
+    return LTPMG_OK;
+}
 
 }  // namespace
 
@@ -1441,6 +1481,8 @@ int ltpmg_fetch_results(ltpmg_engine *e, uint32_t slot, uint8_t *commit, void *r
             return fail(e, LTPMG_EINVAL, "results buffer holds %llu bytes, %llu needed",
                         (unsigned long long)results_bytes, (unsigned long long)need);
         if (need) HIP_TRY(e, hipMemcpy(results, s->d_results, need, hipMemcpyDeviceToHost));
+        int rc = zero_result_padding(e, s->d_meta, s->n_ops, static_cast<uint8_t *>(results), max_slot_all(e));
+        if (rc != LTPMG_OK) return rc;
     }
     return LTPMG_OK;
 }
@@ -1630,7 +1672,8 @@ extern "C" int ltpmg_execute_static(ltpmg_engine *e, uint32_t mode, const ltpmg_
     auto now = []() { return LTPMG_OK; };
     uint8_t *res = static_cast<uint8_t *>(dev_results);
     int rc = mode == LTPMG_MODE_A ? launch_mode_a(e, st, ov, sb.tile_t0, sb.counters, dev_commit, res, rs, rec, now)
-                                  : launch_mode_o(e, st, ov, sb.ob, dev_commit, res, rs, rec, now);
+                                  : launch_mode_o(e, st, ov, sb.ob, dev_commit, res, rs, rec, now, nullptr, 0u,
+                                                  true);
     if (rc != LTPMG_OK) return rc;
     HIP_TRY(e, hipGetLastError());
     return LTPMG_OK;
@@ -2245,6 +2288,8 @@ int ltpmg_stream_fetch(ltpmg_engine *e, ltpmg_stream *s, uint32_t batch, uint8_t
             return fail(e, LTPMG_EINVAL, "results buffer holds %llu bytes, %llu needed",
                         (unsigned long long)results_bytes, (unsigned long long)need);
         if (need) HIP_TRY(e, hipMemcpy(results, s->d_results + op0 * s->res_stride, need, hipMemcpyDeviceToHost));
+        int rc = zero_result_padding(e, s->d_meta + op0, no, static_cast<uint8_t *>(results), s->res_stride);
+        if (rc != LTPMG_OK) return rc;
     }
     return LTPMG_OK;
 }

@@ -359,8 +359,10 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys,
 // ordered-mode stages, run in this order after launch_mo_keys + launch_radix_sort
 void launch_mo_seg_scan(hipStream_t s, OpView ov, OrderBufs &ob, int sorted);
 void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob);
+// pad: each READ's result slot is written to its end (zeros past the cell);
+// else only the cell's slot bytes, and the host zeroes the rest on fetch
 void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint8_t *results, uint32_t res_stride);
+                           uint8_t *results, uint32_t res_stride, bool pad);
 void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob,
                             uint32_t res_stride, uint8_t *commit);  // also sets every commit flag
 uint32_t seg_tiles(uint32_t n);

@@ -184,8 +184,8 @@ constexpr uint32_t kInsPerThread = kInsTile / kBlock;
 // INSERTs are first listed in LDS in rank order; then the whole workgroup
 // writes the rows column by column, consecutive threads on consecutive rows,
 // so each column's stores are contiguous.  Every column gets the payload of
-// rowseed(arg, column), the presence byte is set, the result slot gets the
-// row id.
+// rowseed(arg, column), the presence byte is set, the result slot's first
+// 16 bytes get the row id and zeros.
 __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, uint32_t table_mask,
                                                        const uint32_t *cnt, uint32_t n_tiles, const uint32_t *base,
                                                        uint8_t *results, uint32_t res_stride, uint8_t *found) {
@@ -218,13 +218,14 @@ __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, u
                 continue;
             }
             td.present[first + k] = 1;
+            // the row id and zeros to 16 bytes (the host zeroes the rest of
+            // the result slot when it fetches the results)
             uint8_t *res = results + (uint64_t)o * res_stride;
             if ((res_stride & 15u) == 0) {
                 st16(res, u32x4{first + k, 0u, 0u, 0u});
-                for (uint32_t b = 16; b < res_stride; b += 16) st16(res + b, u32x4{0u, 0u, 0u, 0u});
             } else {
                 st4(res, first + k);
-                for (uint32_t b = 4; b < res_stride; b += 4) st4(res + b, 0u);
+                for (uint32_t b = 4; b < res_stride && b < 16; b += 4) st4(res + b, 0u);
             }
         }
         for (uint32_t c = 0; c < td.n_cols; ++c) {

@@ -279,9 +279,11 @@ __global__ __launch_bounds__(kBlock) void k_selftest_apply(SelftestSrc src, cons
 // issues every independent source load (the payload seed of the UPDATE an
 // entry sees, or the snapshot cell) before any store, so each lane keeps R
 // HBM requests in flight instead of one.  fetch(k, key, st, op) gives entry
-// k's cell key, state and destination: op's result slot (res_span bytes at
-// results + op * res_span, zero past the cell) or, for op = kCellOp, the
-// cell itself.  ONE: G * 16 bytes are the result span, so each lane moves
+// k's cell key, state and destination: op's result slot (at results + op *
+// res_span; the cell's slot bytes, and with res_fill = res_span zeros on to
+// the end of the slot -- with res_fill = 0 the rest of the result slot is
+// left alone, and the host zeroes it when it fetches the results) or, for
+// op = kCellOp, the cell itself.  ONE: G * 16 bytes are the result span, so each lane moves
 // at most one 16-byte chunk (lane * 16) of an entry.  LCOLS: every live
 // column's descriptor is in lcols, the column list in LDS (else they come
 // from dt.cols).  Per entry a round
@@ -292,7 +294,7 @@ constexpr uint32_t kCellOp = 0xFFFFFFFFu;
 template <int G, bool VEC16, bool ONE, bool LCOLS, int R = kMoveDepthO, class Fetch>
 __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, const KeyDecode &dec,
                                   const uint64_t *args, uint8_t *results, uint32_t res_span,
-                                  uint32_t g, uint32_t groups, const ColDesc *lcols) {
+                                  uint32_t res_fill, uint32_t g, uint32_t groups, const ColDesc *lcols) {
     const uint32_t lane = threadIdx.x % G, b0 = lane * 16;
     // stream64(s, j) = mix64(s + (j + 1) * golden): the lane's two words
     const uint64_t gold0 = (uint64_t)((b0 >> 3) + 1) * kGolden, gold1 = gold0 + kGolden;
@@ -340,7 +342,7 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, 
             const uint32_t slot = sw[r] & 0xFFFFu, width = sw[r] >> 16;
             const bool to_result = op[r] != kCellOp;
             uint8_t *dst = to_result ? results + (uint64_t)op[r] * res_span : const_cast<uint8_t *>(cell[r]);
-            const uint32_t span = to_result ? res_span : slot;
+            const uint32_t span = to_result && res_fill > slot ? res_fill : slot;
             const bool has_set = st[r].set < kRmwOnly;
             if (ONE && slot >= 16) {
                 u32x4 x{0u, 0u, 0u, 0u};
@@ -357,7 +359,7 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, 
                 // one streaming store for result slots and cells alike:
                 // two store forms cost two instructions per move (0.566
                 // against 0.530 ms for the apply kernel)
-                if (to_result || b0 < span) st16_nt(dst + b0, x);
+                if (b0 < span) st16_nt(dst + b0, x);
             } else if (VEC16 && slot >= 16) {
                 for (uint32_t b = b0; b < span; b += G * 16) {
                     u32x4 x{0u, 0u, 0u, 0u};
@@ -395,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
                                                            const uint32_t *tset,
                                                            const uint32_t *tdelta, DevTables dt,
                                                            uint8_t *results, uint32_t res_stride,
-                                                           uint4 *wlist, uint32_t *wtotal) {
+                                                           uint32_t res_fill, uint4 *wlist, uint32_t *wtotal) {
     // READs from the front {op, set, d, key}; in-tile final writes from the
     // back {key, set, d, 0}.  Such a write entry needs a write op of this
     // tile in its segment, so reads + write entries <= tile elements and the
@@ -469,14 +471,14 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         key_out = e.w;
         st = St{e.y, e.z};
         op = e.x;
-    }, dt, dec, src.args, results, res_stride, threadIdx.x / G, kBlock / G, lcols);
+    }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / G, kBlock / G, lcols);
     __syncthreads();  // every read of this tile's cells is done
     move_cells<G, VEC16, ONE, LCOLS>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
         uint4 e = list[kSegTile - 1 - k];
         key_out = e.x;
         st = St{e.y, e.z};
         op = kCellOp;  // the cell itself
-    }, dt, dec, src.args, results, res_stride, threadIdx.x / G, kBlock / G, lcols);
+    }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / G, kBlock / G, lcols);
 }
 
 // The final writes of segments that span tiles, after every tile's reads:
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_final_writes(const uint4 *wlist, 
         key = e.x;
         st = St{e.y, e.z};
         op = kCellOp;
-    }, dt, dt.dec, args, nullptr, 0u, g, groups, nullptr);
+    }, dt, dt.dec, args, nullptr, 0u, 0u, g, groups, nullptr);
 }
 
 // ---------------- radix sort ----------------
@@ -1335,14 +1337,14 @@ __global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView o
         key_out = e.w;
         st = St{e.y, e.z};
         op = e.x;
-    }, dt, dt.dec, ov.args, results, res_stride, threadIdx.x / G, kSmallBlock / G, lcols);
+    }, dt, dt.dec, ov.args, results, res_stride, 0u, threadIdx.x / G, kSmallBlock / G, lcols);
     __syncthreads();  // every READ of the batch is done
     move_cells<G, VEC16, ONE, LCOLS, RS>(n_wr, [&](uint32_t q, uint32_t &key_out, St &st, uint32_t &op) {
         const uint4 e = L.list[kSmallMaxOps - 1 - q];
         key_out = e.x;
         st = St{e.y, e.z};
         op = kCellOp;
-    }, dt, dt.dec, ov.args, results, res_stride, threadIdx.x / G, kSmallBlock / G, lcols);
+    }, dt, dt.dec, ov.args, results, res_stride, 0u, threadIdx.x / G, kSmallBlock / G, lcols);
     if (!ra.on) return;  // uniform
     // After the cell writes: a DELETE that took effect clears its row's
     // presence; the INSERTs that take effect get rows at the end of each
@@ -1419,8 +1421,9 @@ __global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView o
                 continue;
             }
             td.present[base + q] = 1;
+            // the row id and zeros to 16 bytes; the host zeroes the rest
             uint8_t *res = results + (uint64_t)i * res_stride;
-            for (uint32_t b = 0; b < res_stride; b += 4) st4(res + b, b == 0 ? base + q : 0u);
+            for (uint32_t b = 0; b < res_stride && b < 16; b += 4) st4(res + b, b == 0 ? base + q : 0u);
         }
         for (uint32_t col = 0; col < td.n_cols; ++col) {  // uniform
             const ColDesc cd = tcols[col];
@@ -1614,7 +1617,8 @@ void launch_mo_warm_seeds(hipStream_t s, OpView ov, OrderBufs &ob) {
 }
 
 void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob, int sorted,
-                           uint8_t *results, uint32_t res_stride) {
+                           uint8_t *results, uint32_t res_stride, bool pad) {
+    const uint32_t res_fill = pad ? res_stride : 0u;
     const uint32_t n = ov.n_ops;
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
@@ -1622,10 +1626,10 @@ void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderB
     // the column list in LDS when it holds every live column
     if (dt.n_clist >= dt.dec.n_cols_all)
         LTPMG_MO_DISPATCH(k_mo_apply_reads, true, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta,
-                          dt, results, res_stride, ob.wlist, ob.wtotal);
+                          dt, results, res_stride, res_fill, ob.wlist, ob.wtotal);
     else
         LTPMG_MO_DISPATCH(k_mo_apply_reads, false, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta,
-                          dt, results, res_stride, ob.wlist, ob.wtotal);
+                          dt, results, res_stride, res_fill, ob.wlist, ob.wtotal);
 }
 
 void launch_mo_final_writes(hipStream_t s, const DevTables &dt, OpView ov, OrderBufs &ob,

@@ -11,7 +11,7 @@ import pytest
 
 import oracle_ffi as orc
 from batches import OP_READ, OP_RMW_ADD, OP_UPDATE, random_batch, random_row_batch
-from ltpmg_amd import MODE_O, Batch, Engine, Stream, StreamPlan, gen_kv
+from ltpmg_amd import MODE_A, MODE_O, Batch, Engine, Stream, StreamPlan, gen_kv
 from ltpmg_amd.engine import SMALL_MAX_OPS
 
 pytestmark = pytest.mark.gpu
@@ -271,3 +271,28 @@ def test_stream_of_small_batches():
                     assert (r[mask] == ores[mask]).all(), f"batch {b}: read results differ"
                 for t in schema:
                     assert e.digest(t) == orc.digest_columns(t, im.cols[t]), f"table {t} differs"
+
+
+@pytest.mark.parametrize("mode,pipeline", [(MODE_O, False), (MODE_O, True), (MODE_A, False)],
+                         ids=["ordered_one_launch", "ordered_pipeline", "lowest_tid_wins"])
+def test_result_slots_are_zero_past_the_column(mode, pipeline):
+    # The ordered kernels write a READ's result slot only up to its column's
+    # slot and the fetch zeroes the rest on the host: a 4-byte READ after a
+    # 256-byte READ in the same result slot must come back with zeros past
+    # its 4 bytes, in both paths (and in lowest-TID-wins, which writes whole
+    # slots).
+    schema = {0: (50, [256, 4, 24])}
+    wide = Batch.from_txns([[(OP_READ, 0, 0, r, 0)] for r in range(20)])
+    narrow = Batch.from_txns([[(OP_READ, 0, 1 + r % 2, r, 0)] for r in range(20)])
+    with Engine() as e:
+        im = _setup(e, schema, 9)
+        stride = e.result_stride
+        assert stride == 256
+        for b in (wide, narrow, wide, narrow):
+            commit, res = e.run(b, mode, pipeline=pipeline)
+            oc, ores = orc.replay(im, b, mode, stride)
+            assert commit.all()
+            assert (res == ores).all(), "result slots differ from the oracle's (padding included)"
+        for r in range(20):
+            w = 4 if r % 2 == 0 else 24
+            assert not res[r, w:].any(), f"op {r}: bytes past its {w}-byte column are not zero"
