@@ -50,26 +50,68 @@ __global__ __launch_bounds__(kBlock) void k_row_first_delete(DevTables dt, OpVie
     }
 }
 
+// The tables' fields these kernels need, in LDS (a TabDesc gathered from
+// global memory per op, by lanes of different tables, made the row-op
+// kernels latency-bound).
+struct RowTab {
+    const uint8_t *present;
+    const unsigned long long *lock;
+    uint32_t n_rows;  // 0 for a table that is not live
+    uint32_t n_cols;
+};
+__device__ inline void load_row_tabs(const DevTables &dt, RowTab *rt) {
+    if (threadIdx.x < LTPMG_MAX_TABLES) {
+        const TabDesc &td = dt.tabs[threadIdx.x];
+        rt[threadIdx.x] = RowTab{td.present, td.lock, td.live ? td.n_rows : 0u, td.n_cols};
+    }
+}
+constexpr uint32_t kPresPerThread = 8;  // ops per thread per round, their loads issued together
+
 // Ordered mode: found[o] and meta_eff[o] for every op; a READ of an absent
-// row gets a zeroed result slot (the apply kernel skips invalid ops).
+// row gets a zeroed result slot (the apply kernel skips invalid ops).  Each
+// thread takes kPresPerThread ops a round (consecutive threads on
+// consecutive ops), issues every meta and row load, then every lock-word and
+// presence load, before it uses any.
 __global__ __launch_bounds__(kBlock) void k_row_presence(DevTables dt, OpView ov, const uint32_t *epoch_ctr,
                                                           uint8_t *found, uint16_t *meta_eff, uint8_t *results,
                                                           uint32_t res_stride) {
+    __shared__ RowTab rt[LTPMG_MAX_TABLES];
+    load_row_tabs(dt, rt);
+    __syncthreads();
     const uint32_t epoch = *epoch_ctr;
-    for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < ov.n_ops; o += gridDim.x * blockDim.x) {
-        const uint16_t m = ov.meta[o];
-        const uint32_t row = ov.rows[o];
-        const TabDesc &td = dt.tabs[op_tab(m)];
-        bool f = op_ok(td, m, row);
-        if (f && td.present && !is_insert(m)) {
-            const unsigned long long lk = td.lock[row];
-            const bool deleted_before = (uint32_t)(lk >> 32) == epoch && (uint32_t)~(uint32_t)lk < o;
-            f = td.present[row] && !deleted_before;
+    constexpr uint32_t P = kPresPerThread;
+    for (uint32_t o0 = blockIdx.x * blockDim.x * P; o0 < ov.n_ops; o0 += gridDim.x * blockDim.x * P) {
+        uint32_t m[P], row[P], pres[P];
+        unsigned long long lk[P];
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+            const uint32_t o = o0 + j * blockDim.x + threadIdx.x;
+            m[j] = o < ov.n_ops ? ov.meta[o] : 0xF000u;  // reserved bits: not a valid op
+            row[j] = o < ov.n_ops ? ov.rows[o] : 0u;
         }
-        found[o] = f ? 1 : 0;
-        meta_eff[o] = f ? m : (uint16_t)(m | 0x8000u);  // a reserved bit: an identity in the sort
-        if (!f && op_kind(m) == LTPMG_OP_READ)
-            for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)o * res_stride + b, 0u);
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+            const uint16_t mj = (uint16_t)m[j];
+            const RowTab &t = rt[op_tab(mj)];
+            const bool ok = op_check(mj, row[j], t.n_rows, t.n_cols, t.present != nullptr);
+            const bool look = ok && t.present && !is_insert(mj);
+            pres[j] = look ? t.present[row[j]] : 1u;
+            lk[j] = look ? t.lock[row[j]] : 0ull;
+            m[j] |= (ok ? 0u : 0x10000u) | (look ? 0x20000u : 0u);  // bit 16: op_check failed; 17: looked up
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < P; ++j) {
+            const uint32_t o = o0 + j * blockDim.x + threadIdx.x;
+            if (o >= ov.n_ops) continue;
+            const uint16_t mj = (uint16_t)m[j];
+            const bool deleted_before =
+                (m[j] & 0x20000u) && (uint32_t)(lk[j] >> 32) == epoch && (uint32_t)~(uint32_t)lk[j] < o;
+            const bool f = !(m[j] & 0x10000u) && pres[j] && !deleted_before;
+            found[o] = f ? 1 : 0;
+            meta_eff[o] = f ? mj : (uint16_t)(mj | 0x8000u);  // a reserved bit: an identity in the sort
+            if (!f && op_kind(mj) == LTPMG_OP_READ)
+                for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)o * res_stride + b, 0u);
+        }
     }
 }
 
@@ -86,25 +128,53 @@ __global__ __launch_bounds__(kBlock) void k_row_deletes(DevTables dt, OpView ov,
     }
 }
 
-// The table of op o if it is a valid INSERT that takes effect, else 0xFF.
-__device__ inline uint32_t taken_insert(const DevTables &dt, const OpView &ov, const uint8_t *found, uint32_t o) {
-    if (o >= ov.n_ops) return 0xFFu;
-    const uint16_t m = ov.meta[o];
-    if (!is_insert(m) || !found[o] || !op_ok(dt.tabs[op_tab(m)], m, ov.rows[o])) return 0xFFu;
-    return op_tab(m);
+// Bit t: table t is a live append table with room for a row, so a valid
+// INSERT into it names row 0 (op_check for INSERTs).  Built by the first
+// LTPMG_MAX_TABLES threads; the caller's barrier publishes it.
+__device__ inline void insert_tables_mask(const DevTables &dt, uint32_t *mask) {
+    if (threadIdx.x == 0) *mask = 0;
+    __syncthreads();
+    if (threadIdx.x < LTPMG_MAX_TABLES) {
+        const TabDesc &td = dt.tabs[threadIdx.x];
+        if (td.live && td.present && td.n_rows > 0) atomicOr(mask, 1u << threadIdx.x);
+    }
+}
+constexpr uint32_t kInsPerThread = kInsTile / kBlock;
+// The tables of ops o[0..kInsPerThread) that are valid INSERTs taking effect,
+// else 0xFF: every meta word is loaded first, then found and row only for the
+// INSERT candidates, all in flight together (one dependent TabDesc gather per
+// op had made the insert kernels the slowest of an order-entry batch).
+__device__ inline void taken_inserts(const OpView &ov, const uint8_t *found, uint32_t mask, const uint32_t *o,
+                                     uint32_t *tb) {
+    uint32_t m[kInsPerThread], f[kInsPerThread], r[kInsPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kInsPerThread; ++j) m[j] = o[j] < ov.n_ops ? ov.meta[o[j]] : 0xFFFFu;
+#pragma unroll
+    for (uint32_t j = 0; j < kInsPerThread; ++j) {
+        const uint16_t mj = (uint16_t)m[j];
+        const bool cand = is_insert(mj) && (mj >> 12) == 0 && ((mask >> op_tab(mj)) & 1u);
+        f[j] = cand ? found[o[j]] : 0u;
+        r[j] = cand ? ov.rows[o[j]] : 1u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = f[j] && r[j] == 0 ? op_tab((uint16_t)m[j]) : 0xFFu;
 }
 
 // INSERTs per table in each kInsTile-op tile: cnt[table * n_tiles + tile].
 __global__ __launch_bounds__(kBlock) void k_ins_count(DevTables dt, OpView ov, const uint8_t *found, uint32_t *cnt,
                                                        uint32_t n_tiles) {
-    __shared__ uint32_t c[LTPMG_MAX_TABLES];
+    __shared__ uint32_t c[LTPMG_MAX_TABLES], mask;
     if (threadIdx.x < LTPMG_MAX_TABLES) c[threadIdx.x] = 0;
+    insert_tables_mask(dt, &mask);
     __syncthreads();
     const uint32_t o0 = blockIdx.x * kInsTile;
-    for (uint32_t i = threadIdx.x; i < kInsTile; i += blockDim.x) {
-        const uint32_t tb = taken_insert(dt, ov, found, o0 + i);
-        if (tb != 0xFFu) atomicAdd(&c[tb], 1u);
-    }
+    uint32_t o[kInsPerThread], tb[kInsPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kInsPerThread; ++j) o[j] = o0 + threadIdx.x + j * kBlock;  // coalesced
+    taken_inserts(ov, found, mask, o, tb);
+#pragma unroll
+    for (uint32_t j = 0; j < kInsPerThread; ++j)
+        if (tb[j] != 0xFFu) atomicAdd(&c[tb[j]], 1u);
     __syncthreads();
     if (threadIdx.x < LTPMG_MAX_TABLES) cnt[(uint64_t)threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
 }
@@ -176,8 +246,6 @@ __device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws, uint32_t *total)
     return pre + inc - x;
 }
 
-constexpr uint32_t kInsPerThread = kInsTile / kBlock;
-
 // Every INSERT that takes effect writes its row: row = base + tile offset +
 // rank in the tile (op order; a thread takes kInsPerThread consecutive ops,
 // so thread order then item order is op order).  Per table, the tile's
@@ -190,22 +258,34 @@ __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, u
                                                        const uint32_t *cnt, uint32_t n_tiles, const uint32_t *base,
                                                        uint8_t *results, uint32_t res_stride, uint8_t *found) {
     __shared__ uint32_t ws[kBlock / 64];
-    __shared__ uint32_t list[kInsTile];  // op index of the tile's k-th INSERT into the table
+    __shared__ uint32_t list[kInsTile];            // op index of the tile's k-th INSERT into the table
+    __shared__ unsigned long long seeds[kInsTile];  // its arg (the row's payload seed)
+    __shared__ ColDesc tcols[LTPMG_MAX_COLS];      // the table's column descriptors
+    __shared__ uint32_t ins_mask;
+    insert_tables_mask(dt, &ins_mask);
+    __syncthreads();
     const uint32_t o0 = blockIdx.x * kInsTile + threadIdx.x * kInsPerThread;
-    uint32_t tb[kInsPerThread];
+    uint32_t o[kInsPerThread], tb[kInsPerThread];
 #pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = taken_insert(dt, ov, found, o0 + j);
-    for (uint32_t mask = table_mask; mask; mask &= mask - 1) {
-        const uint32_t t = __builtin_ctz(mask);  // uniform
+    for (uint32_t j = 0; j < kInsPerThread; ++j) o[j] = o0 + j;
+    taken_inserts(ov, found, ins_mask, o, tb);
+    for (uint32_t tm = table_mask; tm; tm &= tm - 1) {
+        const uint32_t t = __builtin_ctz(tm);  // uniform
         uint32_t mine = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kInsPerThread; ++j) mine += tb[j] == t;
         uint32_t n;
-        uint32_t rank = block_excl(mine, ws, &n);  // its barriers also order the previous table's list reads
+        // its barriers also order the previous table's reads of list, seeds
+        // and tcols before the writes below
+        uint32_t rank = block_excl(mine, ws, &n);
         if (n == 0) continue;  // uniform
+        if (threadIdx.x < LTPMG_MAX_COLS) tcols[threadIdx.x] = dt.cols[t * LTPMG_MAX_COLS + threadIdx.x];
 #pragma unroll
         for (uint32_t j = 0; j < kInsPerThread; ++j)
-            if (tb[j] == t) list[rank++] = o0 + j;
+            if (tb[j] == t) {
+                seeds[rank] = ov.args[o0 + j];
+                list[rank++] = o0 + j;
+            }
         __syncthreads();
         const TabDesc td = dt.tabs[t];
         const uint32_t first = base[t] + cnt[(uint64_t)t * n_tiles + blockIdx.x];
@@ -229,10 +309,10 @@ __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, u
             }
         }
         for (uint32_t c = 0; c < td.n_cols; ++c) {
-            const ColDesc cd = dt.cols[t * LTPMG_MAX_COLS + c];
+            const ColDesc cd = tcols[c];
             for (uint32_t k = threadIdx.x; k < fit; k += kBlock) {
                 uint8_t *cell = cd.base + (uint64_t)(first + k) * cd.slot;
-                const uint64_t seed = rowseed(ov.args[list[k]], c);
+                const uint64_t seed = rowseed(seeds[k], c);
                 if (cd.slot >= 16) {
                     for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
                 } else {
@@ -256,8 +336,8 @@ void launch_row_presence_o(hipStream_t s, const DevTables &dt, OpView ov, RowBuf
     hipLaunchKernelGGL(k_next_epoch, dim3(1), dim3(1), 0, s, epoch_ctr);
     if (rb.has_delete)
         hipLaunchKernelGGL(k_row_first_delete, dim3(grid_for(ov.n_ops)), dim3(kBlock), 0, s, dt, ov, epoch_ctr);
-    hipLaunchKernelGGL(k_row_presence, dim3(grid_for(ov.n_ops)), dim3(kBlock), 0, s, dt, ov, epoch_ctr, rb.found,
-                       rb.meta_eff, results, res_stride);
+    hipLaunchKernelGGL(k_row_presence, dim3(grid_for((ov.n_ops + kPresPerThread - 1) / kPresPerThread)),
+                       dim3(kBlock), 0, s, dt, ov, epoch_ctr, rb.found, rb.meta_eff, results, res_stride);
 }
 
 void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb) {

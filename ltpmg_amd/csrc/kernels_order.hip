@@ -50,14 +50,26 @@ __device__ inline Seg seg_op(Seg a, Seg b) {
     return {a.f, c.set, c.d};
 }
 // val = op index | invalid << 29 | kind << 30; an invalid op (only possible
-// with an unchecked submit) is an identity element and emits nothing
+// with an unchecked submit, or an op on an absent row) is an identity
+// element and emits nothing.  An RMW_ADD whose delta fits 29 bits (signed)
+// carries the delta itself instead of its op index, under kind 3 (row ops
+// never reach the sorted list valid): its state then needs no random 8-byte
+// gather of args (three per RMW in the scan and apply walks, most of an
+// order-entry batch's scan time).
 constexpr uint32_t kInvalid = 1u << 29;
 constexpr uint32_t kOpMask = kInvalid - 1u;
+constexpr uint32_t kInlineRmw = 3u << 30;
+__device__ inline uint32_t rmw_val(uint32_t o, uint64_t arg) {
+    const int32_t d = (int32_t)(uint32_t)arg;
+    return d >= -(1 << 28) && d < (1 << 28) ? ((uint32_t)d & kOpMask) | kInlineRmw : o | (LTPMG_OP_RMW_ADD << 30);
+}
+__device__ inline uint32_t inline_delta(uint32_t val) { return (uint32_t)((int32_t)(val << 3) >> 3); }
 __device__ inline St elem_state(uint32_t val, const uint64_t *args) {
     uint32_t kind = val >> 30, o = val & kOpMask;
     if (val & kInvalid) return {kNoSet, 0u};
     if (kind == LTPMG_OP_UPDATE) return {o, 0u};
     if (kind == LTPMG_OP_RMW_ADD) return {kRmwOnly, (uint32_t)args[o]};
+    if (kind == 3u) return {kRmwOnly, inline_delta(val)};
     return {kNoSet, 0u};
 }
 
@@ -519,6 +531,7 @@ struct SortedSrc {
     const uint32_t *k, *v;
     struct Cache {};
     __device__ void prepare(Cache &) const {}
+    __device__ uint32_t key(uint32_t i, const Cache &) const { return k[i]; }
     __device__ void kv(uint32_t i, uint32_t &key, uint32_t &val, const Cache &) const {
         key = k[i];
         val = v[i];
@@ -542,6 +555,7 @@ struct OpSrc {
     const TabDesc *tabs;
     const uint32_t *rows;
     const uint16_t *meta;
+    const uint64_t *args;  // read for RMW_ADDs only (rmw_val)
     // the table descriptors in LDS, so a key costs no dependent global load
     // one 16-byte word per table {cell_base, n_rows (0 if not live), n_cols,
     // cstride}: a key costs one LDS read
@@ -570,8 +584,10 @@ struct OpSrc {
         const uint4 td = c.t[op_tab(m)];
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.y && op_col(m) < td.z;
         key = ok ? td.x + row * td.w + op_col(m) : 0u;
-        val = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
+        val = ok && op_kind(m) == LTPMG_OP_RMW_ADD ? rmw_val(o, args[o])
+                                                   : o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
+    __device__ uint32_t key(uint32_t o, const Cache &c) const { return key_of(meta[o], rows[o], c); }
     __device__ uint32_t key_of(uint16_t m, uint32_t row, const Cache &c) const {
         const uint4 td = c.t[op_tab(m)];
         const bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.y && op_col(m) < td.z;
@@ -621,8 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(Src src, uint32_t n, uint32_
 #pragma unroll
         for (uint32_t it = 0; it < kSortItems; ++it) {
             const uint32_t idx = VEC ? vbase + it : base + it * kBlock + threadIdx.x;
-            uint32_t key = 0u, val = 0u;
-            if (idx < n) src.kv(idx, key, val, cache);
+            const uint32_t key = idx < n ? src.key(idx, cache) : 0u;
             digit[it] = idx < n ? (key >> shift) & (D - 1u) : D;
         }
     }
@@ -683,7 +698,7 @@ __device__ inline void rank_tile(const Src &src, const typename Src::Cache &cach
 #pragma unroll
     for (uint32_t c = 0; c < kSortItems; ++c) {
         const uint32_t idx = wbase + c * 64 + lane;
-        src.cook(idx, k[c], v[c], cache);
+        src.cook(idx < n ? idx : n - 1u, k[c], v[c], cache);  // (cook may read args[op])
         k[c] = idx < n ? k[c] : 0u;
         v[c] = idx < n ? v[c] : 0u;
     }
@@ -1000,7 +1015,8 @@ __global__ __launch_bounds__(kBlock) void k_mo_keys(DevTables dt, OpView ov, uin
         bool ok = (m >> 12) == 0 && op_kind(m) <= LTPMG_OP_RMW_ADD && row < td.n_rows &&
                   op_col(m) < td.n_cols;
         keys[o] = ok ? td.cell_base + row * td.cstride + op_col(m) : 0u;
-        vals[o] = o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
+        vals[o] = ok && op_kind(m) == LTPMG_OP_RMW_ADD ? rmw_val(o, ov.args[o])
+                                                       : o | (op_kind(m) << 30) | (ok ? 0u : kInvalid);
     }
 }
 
@@ -1112,7 +1128,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView o
     if (LCOLS)
         for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kSmallBlock) lcols[i] = dt.clist[i];
     __shared__ uint8_t *s_present[LTPMG_MAX_TABLES];  // each table's presence bytes (append tables)
-    const OpSrc src{dt.tabs, ov.rows, ov.meta};
+    const OpSrc src{dt.tabs, ov.rows, ov.meta, ov.args};
     src.prepare(cache);
     if (ra.on) {
         for (uint32_t i = threadIdx.x; i < kDelSlots; i += kSmallBlock) L.del[i] = kDelEmpty;
@@ -1286,6 +1302,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView o
         if (v[j] & kInvalid) return {kNoSet, 0u};
         if (kind == LTPMG_OP_UPDATE) return {v[j] & kOpMask, 0u};
         if (kind == LTPMG_OP_RMW_ADD) return {kRmwOnly, dd[j]};
+        if (kind == 3u) return {kRmwOnly, inline_delta(v[j])};
         return {kNoSet, 0u};
     };
     Seg agg{0u, kNoSet, 0u};
@@ -1503,14 +1520,14 @@ void launch_rs_scatter(hipStream_t s, SortBufs &b, int cur, uint32_t n, uint32_t
 
 void launch_rs_hist_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
     const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
-    const OpSrc src{dt.tabs, ov.rows, ov.meta};
+    const OpSrc src{dt.tabs, ov.rows, ov.meta, ov.args};
     if (aligned16(ov.rows) && aligned16(ov.meta)) LTPMG_RS_HIST(bits, true, OpSrc, src, n, 0u, n_tiles, b.hist);
     else LTPMG_RS_HIST(bits, false, OpSrc, src, n, 0u, n_tiles, b.hist);
 }
 
 void launch_rs_scatter_ops(hipStream_t s, DevTables dt, OpView ov, SortBufs &b, uint32_t bits) {
     const uint32_t n = ov.n_ops, n_tiles = (n + kSortTile - 1) / kSortTile;
-    LTPMG_RS_BITS(bits, k_rs_scatter, OpSrc, OpSrc{dt.tabs, ov.rows, ov.meta}, b.keys[1], b.vals[1], n,
+    LTPMG_RS_BITS(bits, k_rs_scatter, OpSrc, OpSrc{dt.tabs, ov.rows, ov.meta, ov.args}, b.keys[1], b.vals[1], n,
                   0u, n_tiles, b.hist);
 }
 
