@@ -103,6 +103,7 @@ struct ltpmg_engine {
     ColDesc *d_cols = nullptr;  // the sparse descriptor table, then the column list (kColList)
     uint32_t n_clist = 0;
     KeyDecode kdec{};  // ordered-mode key decode of the live tables
+    uint32_t wide_lanes = 8;  // DevTables::wide_lanes
     TabDesc *d_tabs = nullptr;
     unsigned long long *d_digest = nullptr;
     uint8_t *static_scratch = nullptr;  // this device's copy of `scratch` (ltpmg_execute_static)
@@ -239,19 +240,42 @@ uint32_t max_slot_all(const ltpmg_engine *e) {
 }
 
 DevTables dev_tables(const ltpmg_engine *e) {
-    return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist, e->kdec};
+    return DevTables{e->d_cols, e->d_tabs, e->d_cols + LTPMG_MAX_TABLES * LTPMG_MAX_COLS, e->n_clist, e->kdec,
+                     e->wide_lanes};
 }
 
 // Column stride of the ordered-mode cell keys (TabDesc::cstride): each
 // table's column count rounded up to a power of two when the padded keys of
 // all live tables fit below 2^32 - 1, else the column count itself.
+#ifndef LTPMG_KEY_PAD
+#define LTPMG_KEY_PAD 1
+#endif
 bool pad_key_columns(const ltpmg_engine *e) {
+    if (!LTPMG_KEY_PAD) return false;
     uint64_t padded = 0;
     for (const Table &t : e->tables)
         if (t.live) padded += (uint64_t)t.n_rows << ceil_log2(t.n_cols);
     return padded <= 0xFFFFFFFFull;
 }
 uint32_t key_col_stride(const Table &t, bool pad) { return pad ? 1u << ceil_log2(t.n_cols) : t.n_cols; }
+
+// A cell move takes G lanes, one 16-byte chunk each per step.  Past 128-byte
+// result strides the chunks no longer fit one step of 8 lanes, and a wide
+// lane group mostly idles on the narrow columns that schemas like order entry
+// are made of (98 % of its 92 slots are at most 32 bytes, one is 256): 2
+// lanes per move took its ordered batch from 3.59 to 3.12 ms against 8 (1
+// lane: 3.74 ms, 4 lanes: 3.16 ms).  Schemas of wide columns keep 8, and so
+// does the lowest-TID-wins mode (its W = 64 batch took 1.05 ms with 2 lanes
+// against 1.01 ms with 8).
+#ifndef LTPMG_WIDE_LANES
+#define LTPMG_WIDE_LANES 0  // 0: from the slots; else forced
+#endif
+uint32_t wide_move_lanes(const uint32_t *slots, uint32_t n) {
+    if (LTPMG_WIDE_LANES) return LTPMG_WIDE_LANES;
+    uint32_t narrow = 0;
+    for (uint32_t i = 0; i < n; ++i) narrow += slots[i] <= 32u;
+    return n && narrow * 4 >= n * 3 ? 2u : 8u;
+}
 
 // Writes the device descriptors and assigns ordered-mode global row bases.
 int sync_descriptors(ltpmg_engine *e) {
@@ -285,6 +309,10 @@ int sync_descriptors(ltpmg_engine *e) {
     HIP_TRY(e, hipMemcpy(e->d_cols, cols, sizeof cols, hipMemcpyHostToDevice));
     e->n_clist = n_clist;
     e->kdec = kd;
+    std::vector<uint32_t> slots;
+    for (const Table &tb : e->tables)
+        if (tb.live) slots.insert(slots.end(), tb.slot, tb.slot + tb.n_cols);
+    e->wide_lanes = wide_move_lanes(slots.data(), (uint32_t)slots.size());
     HIP_TRY(e, hipMemcpy(e->d_tabs, tabs, sizeof tabs, hipMemcpyHostToDevice));
     return LTPMG_OK;
 }

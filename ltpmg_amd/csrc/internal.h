@@ -252,7 +252,15 @@ struct DevTables {
     const ColDesc *clist;  // [kColList]
     uint32_t n_clist;      // entries of clist in use
     KeyDecode dec;
+    // lanes per ordered-mode cell move when the result stride exceeds 128
+    // bytes (2 or 8; wide_move_lanes)
+    uint32_t wide_lanes;
 };
+
+// Lanes per cell move for result strides past 128 bytes, from the slots of
+// the live columns: 2 when at least three in four slots are at most 32
+// bytes, else 8.
+uint32_t wide_move_lanes(const uint32_t *slots, uint32_t n);
 
 void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
                         uint32_t n_rows, uint64_t seed);

@@ -1571,11 +1571,11 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint
     hipLaunchKernelGGL(k_mo_keys, dim3(blocks), dim3(kBlock), 0, s, dt, ov, keys, vals);
 }
 
-// G lanes per cell move (a 16-byte chunk each) from the result stride; ONE:
-// one chunk per lane covers a slot.  (Lane groups that always cover the
-// stride, G = 16 for 256-byte slots, measured slower on order-entry batches:
-// 0.356 against 0.250 ms for a 4,863-op batch through the pipeline, half as
-// many moves in flight.)
+// G lanes per cell move (a 16-byte chunk each) from the result stride, and
+// past 128 bytes from dt.wide_lanes; ONE: one chunk per lane covers a slot.
+// (Lane groups that always cover the stride, G = 16 for 256-byte slots,
+// measured slower on order-entry batches: 0.356 against 0.250 ms for a
+// 4,863-op batch through the pipeline, half as many moves in flight.)
 #define LTPMG_MO_LAUNCH_B(KERNEL, G_, LC, GRID, BLOCK, ...)                                         \
     do {                                                                                           \
         if (res_stride == G_ * 16)                                                                 \
@@ -1585,7 +1585,9 @@ void launch_mo_keys(hipStream_t s, DevTables dt, OpView ov, uint32_t *keys, uint
     } while (0)
 #define LTPMG_MO_DISPATCH_B(KERNEL, LC, GRID, BLOCK, ...)                                       \
     do {                                                                                        \
-        if (res_stride >= 128)                                                                  \
+        if (res_stride > 128 && dt.wide_lanes == 2)                                             \
+            LTPMG_MO_LAUNCH_B(KERNEL, 2, LC, GRID, BLOCK, __VA_ARGS__);                         \
+        else if (res_stride >= 128)                                                             \
             LTPMG_MO_LAUNCH_B(KERNEL, 8, LC, GRID, BLOCK, __VA_ARGS__);                         \
         else if (res_stride >= 64)                                                              \
             LTPMG_MO_LAUNCH_B(KERNEL, 4, LC, GRID, BLOCK, __VA_ARGS__);                         \
