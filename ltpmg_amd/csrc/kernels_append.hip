@@ -226,18 +226,22 @@ __global__ __launch_bounds__(kScanThreads) void k_ins_scan(DevTables dt, uint32_
     }
 }
 
-// Exclusive sum over the block's threads; *total gets the sum of all.
-__device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws, uint32_t *total) {
+constexpr uint32_t kInsGroup = 4;  // insert tables per pass of k_ins_write
+
+// Exclusive sum over the block's threads of kInsGroup 16-bit counts packed
+// in 64 bits (no field carries: a tile has kInsTile < 2^16 ops); *total
+// gets the packed totals.
+__device__ inline uint64_t block_excl_packed(uint64_t x, unsigned long long *ws, uint64_t *total) {
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    uint32_t inc = x;
+    uint64_t inc = x;
     for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off);
+        const uint64_t y = __shfl_up((unsigned long long)inc, off);
         if (lane >= off) inc += y;
     }
     __syncthreads();  // earlier readers of ws are done
     if (lane == 63) ws[w] = inc;
     __syncthreads();
-    uint32_t pre = 0, tot = 0;
+    uint64_t pre = 0, tot = 0;
     for (uint32_t i = 0; i < kBlock / 64; ++i) {
         if (i < w) pre += ws[i];
         tot += ws[i];
@@ -248,76 +252,132 @@ __device__ inline uint32_t block_excl(uint32_t x, uint32_t *ws, uint32_t *total)
 
 // Every INSERT that takes effect writes its row: row = base + tile offset +
 // rank in the tile (op order; a thread takes kInsPerThread consecutive ops,
-// so thread order then item order is op order).  Per table, the tile's
-// INSERTs are first listed in LDS in rank order; then the whole workgroup
-// writes the rows column by column, consecutive threads on consecutive rows,
-// so each column's stores are contiguous.  Every column gets the payload of
-// rowseed(arg, column), the presence byte is set, the result slot's first
-// 16 bytes get the row id and zeros.
+// so thread order then item order is op order).  The tile's insert tables
+// are taken kInsGroup at a time: one block scan of packed counts ranks the
+// INSERTs of all of them, which are listed in LDS table by table in rank
+// order, with each table's fields and column descriptors (loaded while the
+// scan runs).  Then the whole workgroup walks the group's INSERTs (presence
+// byte; the result slot's first 16 bytes get the row id and zeros) and its
+// cells, one (row, column) per lane, column by column within a table, so
+// each column's stores are contiguous.  Every column gets the payload of
+// rowseed(arg, column).  (A loop per table, with its own scan and loads,
+// and per column had made this kernel latency-bound: most tiles hold a few
+// rows of each table.)
 __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, uint32_t table_mask,
                                                        const uint32_t *cnt, uint32_t n_tiles, const uint32_t *base,
                                                        uint8_t *results, uint32_t res_stride, uint8_t *found) {
-    __shared__ uint32_t ws[kBlock / 64];
-    __shared__ uint32_t list[kInsTile];            // op index of the tile's k-th INSERT into the table
+    struct GroupTab {
+        uint8_t *present;
+        uint32_t first;   // the tile's first row in the table
+        uint32_t n_rows;
+        uint32_t n_cols;
+    };
+    __shared__ unsigned long long ws[kBlock / 64];
+    __shared__ uint32_t list[kInsTile];            // op index of each INSERT, table by table in rank order
     __shared__ unsigned long long seeds[kInsTile];  // its arg (the row's payload seed)
-    __shared__ ColDesc tcols[LTPMG_MAX_COLS];      // the table's column descriptors
-    __shared__ uint32_t ins_mask;
-    insert_tables_mask(dt, &ins_mask);
+    __shared__ ColDesc tcols[kInsGroup][LTPMG_MAX_COLS];
+    __shared__ GroupTab gt[kInsGroup];
+    __shared__ uint32_t ins_mask, busy;
+    // A tile none of whose INSERTs takes effect writes nothing.  The counts
+    // are offsets by now, so a tile's count per table is the next tile's
+    // offset less its own; the last tile always takes the walk.
+    if (threadIdx.x == 0) busy = blockIdx.x + 1 == n_tiles ? 1u : 0u;
+    insert_tables_mask(dt, &ins_mask);  // its barrier orders the store above before the ones below
+    if (threadIdx.x < LTPMG_MAX_TABLES && ((table_mask >> threadIdx.x) & 1u) && blockIdx.x + 1 < n_tiles) {
+        const uint32_t *c = cnt + (uint64_t)threadIdx.x * n_tiles + blockIdx.x;
+        if (c[1] != c[0]) busy = 1u;
+    }
     __syncthreads();
+    if (!busy) return;  // uniform
     const uint32_t o0 = blockIdx.x * kInsTile + threadIdx.x * kInsPerThread;
     uint32_t o[kInsPerThread], tb[kInsPerThread];
 #pragma unroll
     for (uint32_t j = 0; j < kInsPerThread; ++j) o[j] = o0 + j;
     taken_inserts(ov, found, ins_mask, o, tb);
-    for (uint32_t tm = table_mask; tm; tm &= tm - 1) {
-        const uint32_t t = __builtin_ctz(tm);  // uniform
-        uint32_t mine = 0;
+    for (uint32_t tm = table_mask; tm;) {  // uniform
+        uint32_t gtab[kInsGroup], ng = 0;
+        for (; tm && ng < kInsGroup; tm &= tm - 1) gtab[ng++] = __builtin_ctz(tm);
+        __syncthreads();  // the previous group's readers of gt, tcols, list and seeds are done
+        for (uint32_t x = threadIdx.x; x < ng * LTPMG_MAX_COLS; x += kBlock) {
+            const uint32_t g = x / LTPMG_MAX_COLS, c = x % LTPMG_MAX_COLS;
+            tcols[g][c] = dt.cols[gtab[g] * LTPMG_MAX_COLS + c];
+        }
+        if (threadIdx.x < ng) {
+            const uint32_t t = gtab[threadIdx.x];
+            const TabDesc &td = dt.tabs[t];
+            gt[threadIdx.x] = GroupTab{td.present, base[t] + cnt[(uint64_t)t * n_tiles + blockIdx.x], td.n_rows,
+                                       td.n_cols};
+        }
+        uint32_t grp[kInsPerThread];  // the INSERT's place in the group, or kInsGroup
+        uint64_t mine = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < kInsPerThread; ++j) mine += tb[j] == t;
-        uint32_t n;
-        // its barriers also order the previous table's reads of list, seeds
-        // and tcols before the writes below
-        uint32_t rank = block_excl(mine, ws, &n);
-        if (n == 0) continue;  // uniform
-        if (threadIdx.x < LTPMG_MAX_COLS) tcols[threadIdx.x] = dt.cols[t * LTPMG_MAX_COLS + threadIdx.x];
+        for (uint32_t j = 0; j < kInsPerThread; ++j) {
+            grp[j] = kInsGroup;
+            for (uint32_t g = 0; g < ng; ++g)
+                if (tb[j] == gtab[g]) grp[j] = g;
+            if (grp[j] < kInsGroup) mine += 1ull << (16 * grp[j]);
+        }
+        uint64_t tot;
+        const uint64_t ex = block_excl_packed(mine, ws, &tot);  // its barriers publish gt and tcols
+        if (tot == 0) continue;  // uniform
+        uint32_t at[kInsGroup], n[kInsGroup], fit[kInsGroup], item_at[kInsGroup + 1];
+        uint32_t run = 0, items = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < kInsPerThread; ++j)
-            if (tb[j] == t) {
-                seeds[rank] = ov.args[o0 + j];
-                list[rank++] = o0 + j;
+        for (uint32_t g = 0; g < kInsGroup; ++g) {
+            n[g] = (uint32_t)(tot >> (16 * g)) & 0xFFFFu;
+            at[g] = run;
+            run += n[g];
+            const uint32_t first = g < ng ? gt[g].first : 0u, rows = g < ng ? gt[g].n_rows : 0u;
+            // rows past the capacity never occur with the host's capacity check
+            fit[g] = first >= rows ? 0u : (rows - first < n[g] ? rows - first : n[g]);
+            item_at[g] = items;
+            items += g < ng ? fit[g] * gt[g].n_cols : 0u;
+        }
+        item_at[kInsGroup] = items;
+        {
+            uint64_t rank = ex;
+#pragma unroll
+            for (uint32_t j = 0; j < kInsPerThread; ++j) {
+                if (grp[j] == kInsGroup) continue;
+                const uint32_t g = grp[j];
+                const uint32_t pos = at[g] + ((uint32_t)(rank >> (16 * g)) & 0xFFFFu);
+                rank += 1ull << (16 * g);
+                seeds[pos] = ov.args[o[j]];
+                list[pos] = o[j];
             }
+        }
         __syncthreads();
-        const TabDesc td = dt.tabs[t];
-        const uint32_t first = base[t] + cnt[(uint64_t)t * n_tiles + blockIdx.x];
-        // rows past the capacity never occur with the host's capacity check
-        const uint32_t fit = first >= td.n_rows ? 0u : (td.n_rows - first < n ? td.n_rows - first : n);
-        for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
-            const uint32_t o = list[k];
-            if (k >= fit) {
-                found[o] = 0;
+        for (uint32_t i = threadIdx.x; i < run; i += kBlock) {
+            uint32_t g = 0;
+            while (g + 1 < kInsGroup && i >= at[g + 1]) ++g;
+            const uint32_t k = i - at[g], op = list[i];
+            if (k >= fit[g]) {
+                found[op] = 0;
                 continue;
             }
-            td.present[first + k] = 1;
+            const uint32_t row = gt[g].first + k;
+            gt[g].present[row] = 1;
             // the row id and zeros to 16 bytes (the host zeroes the rest of
             // the result slot when it fetches the results)
-            uint8_t *res = results + (uint64_t)o * res_stride;
+            uint8_t *res = results + (uint64_t)op * res_stride;
             if ((res_stride & 15u) == 0) {
-                st16(res, u32x4{first + k, 0u, 0u, 0u});
+                st16(res, u32x4{row, 0u, 0u, 0u});
             } else {
-                st4(res, first + k);
+                st4(res, row);
                 for (uint32_t b = 4; b < res_stride && b < 16; b += 4) st4(res + b, 0u);
             }
         }
-        for (uint32_t c = 0; c < td.n_cols; ++c) {
-            const ColDesc cd = tcols[c];
-            for (uint32_t k = threadIdx.x; k < fit; k += kBlock) {
-                uint8_t *cell = cd.base + (uint64_t)(first + k) * cd.slot;
-                const uint64_t seed = rowseed(seeds[k], c);
-                if (cd.slot >= 16) {
-                    for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
-                } else {
-                    for (uint32_t b = 0; b < cd.slot; b += 4) st4(cell + b, b < cd.width ? stream_word(seed, b >> 2) : 0u);
-                }
+        for (uint32_t i = threadIdx.x; i < items; i += kBlock) {
+            uint32_t g = 0;
+            while (g + 1 < kInsGroup && i >= item_at[g + 1]) ++g;
+            const uint32_t local = i - item_at[g], c = local / fit[g], k = local - c * fit[g];
+            const ColDesc cd = tcols[g][c];
+            uint8_t *cell = cd.base + (uint64_t)(gt[g].first + k) * cd.slot;
+            const uint64_t seed = rowseed(seeds[at[g] + k], c);
+            if (cd.slot >= 16) {
+                for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
+            } else {
+                for (uint32_t b = 0; b < cd.slot; b += 4) st4(cell + b, b < cd.width ? stream_word(seed, b >> 2) : 0u);
             }
         }
     }

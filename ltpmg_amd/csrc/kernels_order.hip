@@ -1442,16 +1442,17 @@ __global__ __launch_bounds__(kSmallBlock) void k_mo_small(DevTables dt, OpView o
             uint8_t *res = results + (uint64_t)i * res_stride;
             for (uint32_t b = 0; b < res_stride && b < 16; b += 4) st4(res + b, b == 0 ? base + q : 0u);
         }
-        for (uint32_t col = 0; col < td.n_cols; ++col) {  // uniform
+        // one (row, column) cell per lane, column by column
+        const uint32_t items = fit * td.n_cols;
+        for (uint32_t i = threadIdx.x; i < items; i += kSmallBlock) {
+            const uint32_t col = i / fit, q = i - col * fit;
             const ColDesc cd = tcols[col];
-            for (uint32_t q = threadIdx.x; q < fit; q += kSmallBlock) {
-                uint8_t *cell = cd.base + (uint64_t)(base + q) * cd.slot;
-                const uint64_t seed = rowseed(ins_arg[q], col);
-                if (cd.slot >= 16) {
-                    for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
-                } else {
-                    for (uint32_t b = 0; b < cd.slot; b += 4) st4(cell + b, b < cd.width ? stream_word(seed, b >> 2) : 0u);
-                }
+            uint8_t *cell = cd.base + (uint64_t)(base + q) * cd.slot;
+            const uint64_t seed = rowseed(ins_arg[q], col);
+            if (cd.slot >= 16) {
+                for (uint32_t b = 0; b < cd.slot; b += 16) st16(cell + b, payload16(seed, b, cd.width));
+            } else {
+                for (uint32_t b = 0; b < cd.slot; b += 4) st4(cell + b, b < cd.width ? stream_word(seed, b >> 2) : 0u);
             }
         }
         __syncthreads();  // the list and tcols are read before the next table rewrites them
