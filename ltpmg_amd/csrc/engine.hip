@@ -1529,8 +1529,12 @@ int ltpmg_fetch_status(ltpmg_engine *e, uint32_t slot, uint8_t *status) {
     const uint32_t *off = stage_ptrs(*s, s->n_txns, s->n_ops).txn_off;
     std::vector<uint8_t> commit(s->n_txns);
     HIP_TRY(e, hipMemcpy(commit.data(), s->d_commit, s->n_txns, hipMemcpyDeviceToHost));
-    if (s->rb.has_append) HIP_TRY(e, hipMemcpy(status, s->d_found, s->n_ops, hipMemcpyDeviceToHost));
-    else std::memset(status, 1, s->n_ops);
+    if (s->rb.has_append) {
+        HIP_TRY(e, hipMemcpy(status, s->d_found, s->n_ops, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < s->n_ops; ++i) status[i] = status[i] != 0;  // an INSERT's byte is 2 + its table
+    } else {
+        std::memset(status, 1, s->n_ops);
+    }
     for (uint32_t t = 0; t < s->n_txns; ++t)
         if (!commit[t]) std::memset(status + off[t], 0, off[t + 1] - off[t]);
     return LTPMG_OK;

@@ -305,8 +305,9 @@ hipError_t launch_cc_fused(hipStream_t s, DevTables dt, OpView ov, uint32_t tile
 struct RowBufs {
     bool has_append = false, has_delete = false, has_insert = false;
     uint32_t insert_tables = 0;  // bit t: the batch INSERTs into table t
-    uint8_t *found = nullptr;    // [n_ops]: 1 = the op takes effect: its txn commits and its row
-                                 // is present (an INSERT: it gets a row)
+    uint8_t *found = nullptr;    // [n_ops]: nonzero = the op takes effect: its txn commits and its
+                                 // row is present (an INSERT: it gets a row; found holds 2 + its
+                                 // table until the insert kernels have run)
     uint16_t *meta_eff = nullptr;  // [n_ops]: ordered mode's meta, ops on absent rows marked invalid
     uint32_t *ins_cnt = nullptr;   // [LTPMG_MAX_TABLES * ins_tiles(n_ops)] per table and tile, then offsets
     uint32_t *ins_base = nullptr;  // [LTPMG_MAX_TABLES] cursor of each table at the batch start

@@ -107,7 +107,8 @@ __global__ __launch_bounds__(kBlock) void k_row_presence(DevTables dt, OpView ov
             const bool deleted_before =
                 (m[j] & 0x20000u) && (uint32_t)(lk[j] >> 32) == epoch && (uint32_t)~(uint32_t)lk[j] < o;
             const bool f = !(m[j] & 0x10000u) && pres[j] && !deleted_before;
-            found[o] = f ? 1 : 0;
+            // 2 + table for an INSERT that takes effect (taken_inserts)
+            found[o] = f ? (is_insert(mj) ? (uint8_t)(2u + op_tab(mj)) : (uint8_t)1) : (uint8_t)0;
             meta_eff[o] = f ? mj : (uint16_t)(mj | 0x8000u);  // a reserved bit: an identity in the sort
             if (!f && op_kind(mj) == LTPMG_OP_READ)
                 for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)o * res_stride + b, 0u);
@@ -128,50 +129,36 @@ __global__ __launch_bounds__(kBlock) void k_row_deletes(DevTables dt, OpView ov,
     }
 }
 
-// Bit t: table t is a live append table with room for a row, so a valid
-// INSERT into it names row 0 (op_check for INSERTs).  Built by the first
-// LTPMG_MAX_TABLES threads; the caller's barrier publishes it.
-__device__ inline void insert_tables_mask(const DevTables &dt, uint32_t *mask) {
-    if (threadIdx.x == 0) *mask = 0;
-    __syncthreads();
-    if (threadIdx.x < LTPMG_MAX_TABLES) {
-        const TabDesc &td = dt.tabs[threadIdx.x];
-        if (td.live && td.present && td.n_rows > 0) atomicOr(mask, 1u << threadIdx.x);
-    }
-}
 constexpr uint32_t kInsPerThread = kInsTile / kBlock;
-// The tables of ops o[0..kInsPerThread) that are valid INSERTs taking effect,
-// else 0xFF: every meta word is loaded first, then found and row only for the
-// INSERT candidates, all in flight together (one dependent TabDesc gather per
-// op had made the insert kernels the slowest of an order-entry batch).
-__device__ inline void taken_inserts(const OpView &ov, const uint8_t *found, uint32_t mask, const uint32_t *o,
-                                     uint32_t *tb) {
-    uint32_t m[kInsPerThread], f[kInsPerThread], r[kInsPerThread];
+static_assert(kInsPerThread == 8, "a thread's found bytes are one 8-byte load");
+// The tables of ops o0 .. o0 + 7 that are INSERTs taking effect, else 0xFF.
+// found[] holds 2 + table for exactly those (k_row_presence, and
+// tile_validate_read in the lowest-TID-wins mode: a valid INSERT of a
+// committed txn), so one 8-byte load of found answers for all eight (the
+// meta, row and found loads per op this replaced had made the insert
+// kernels latency-bound).
+__device__ inline void taken_inserts(uint32_t n_ops, const uint8_t *found, uint32_t o0, uint32_t *tb) {
+    uint32_t f[kInsPerThread];
+    if (o0 + kInsPerThread <= n_ops && (reinterpret_cast<uintptr_t>(found + o0) & 7u) == 0) {
+        const uint2 w = *reinterpret_cast<const uint2 *>(found + o0);
 #pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) m[j] = o[j] < ov.n_ops ? ov.meta[o[j]] : 0xFFFFu;
+        for (uint32_t j = 0; j < kInsPerThread; ++j) f[j] = ((j < 4 ? w.x : w.y) >> (8 * (j & 3u))) & 0xFFu;
+    } else {
 #pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) {
-        const uint16_t mj = (uint16_t)m[j];
-        const bool cand = is_insert(mj) && (mj >> 12) == 0 && ((mask >> op_tab(mj)) & 1u);
-        f[j] = cand ? found[o[j]] : 0u;
-        r[j] = cand ? ov.rows[o[j]] : 1u;
+        for (uint32_t j = 0; j < kInsPerThread; ++j) f[j] = o0 + j < n_ops ? found[o0 + j] : 0u;
     }
 #pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = f[j] && r[j] == 0 ? op_tab((uint16_t)m[j]) : 0xFFu;
+    for (uint32_t j = 0; j < kInsPerThread; ++j) tb[j] = f[j] >= 2u ? f[j] - 2u : 0xFFu;
 }
 
 // INSERTs per table in each kInsTile-op tile: cnt[table * n_tiles + tile].
-__global__ __launch_bounds__(kBlock) void k_ins_count(DevTables dt, OpView ov, const uint8_t *found, uint32_t *cnt,
+__global__ __launch_bounds__(kBlock) void k_ins_count(OpView ov, const uint8_t *found, uint32_t *cnt,
                                                        uint32_t n_tiles) {
-    __shared__ uint32_t c[LTPMG_MAX_TABLES], mask;
+    __shared__ uint32_t c[LTPMG_MAX_TABLES];
     if (threadIdx.x < LTPMG_MAX_TABLES) c[threadIdx.x] = 0;
-    insert_tables_mask(dt, &mask);
     __syncthreads();
-    const uint32_t o0 = blockIdx.x * kInsTile;
-    uint32_t o[kInsPerThread], tb[kInsPerThread];
-#pragma unroll
-    for (uint32_t j = 0; j < kInsPerThread; ++j) o[j] = o0 + threadIdx.x + j * kBlock;  // coalesced
-    taken_inserts(ov, found, mask, o, tb);
+    uint32_t tb[kInsPerThread];
+    taken_inserts(ov.n_ops, found, blockIdx.x * kInsTile + threadIdx.x * kInsPerThread, tb);
 #pragma unroll
     for (uint32_t j = 0; j < kInsPerThread; ++j)
         if (tb[j] != 0xFFu) atomicAdd(&c[tb[j]], 1u);
@@ -277,12 +264,12 @@ __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, u
     __shared__ unsigned long long seeds[kInsTile];  // its arg (the row's payload seed)
     __shared__ ColDesc tcols[kInsGroup][LTPMG_MAX_COLS];
     __shared__ GroupTab gt[kInsGroup];
-    __shared__ uint32_t ins_mask, busy;
+    __shared__ uint32_t busy;
     // A tile none of whose INSERTs takes effect writes nothing.  The counts
     // are offsets by now, so a tile's count per table is the next tile's
     // offset less its own; the last tile always takes the walk.
     if (threadIdx.x == 0) busy = blockIdx.x + 1 == n_tiles ? 1u : 0u;
-    insert_tables_mask(dt, &ins_mask);  // its barrier orders the store above before the ones below
+    __syncthreads();
     if (threadIdx.x < LTPMG_MAX_TABLES && ((table_mask >> threadIdx.x) & 1u) && blockIdx.x + 1 < n_tiles) {
         const uint32_t *c = cnt + (uint64_t)threadIdx.x * n_tiles + blockIdx.x;
         if (c[1] != c[0]) busy = 1u;
@@ -293,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_ins_write(DevTables dt, OpView ov, u
     uint32_t o[kInsPerThread], tb[kInsPerThread];
 #pragma unroll
     for (uint32_t j = 0; j < kInsPerThread; ++j) o[j] = o0 + j;
-    taken_inserts(ov, found, ins_mask, o, tb);
+    taken_inserts(ov.n_ops, found, o0, tb);
     for (uint32_t tm = table_mask; tm;) {  // uniform
         uint32_t gtab[kInsGroup], ng = 0;
         for (; tm && ng < kInsGroup; tm &= tm - 1) gtab[ng++] = __builtin_ctz(tm);
@@ -409,7 +396,7 @@ void launch_inserts(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb, 
                     uint32_t res_stride) {
     if (ov.n_ops == 0 || !rb.has_insert) return;
     const uint32_t tiles = ins_tiles(ov.n_ops);
-    hipLaunchKernelGGL(k_ins_count, dim3(tiles), dim3(kBlock), 0, s, dt, ov, rb.found, rb.ins_cnt, tiles);
+    hipLaunchKernelGGL(k_ins_count, dim3(tiles), dim3(kBlock), 0, s, ov, rb.found, rb.ins_cnt, tiles);
     launch_zero_u32(s, rb.inserted, 1);
     hipLaunchKernelGGL(k_ins_scan, dim3(LTPMG_MAX_TABLES), dim3(kScanThreads), 0, s, dt, rb.ins_cnt, tiles,
                        rb.insert_tables, rb.cursor, rb.ins_base, rb.inserted);

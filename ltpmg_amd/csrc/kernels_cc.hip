@@ -295,7 +295,11 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
                 if (i >= t.nops) continue;
                 const uint16_t m = (uint16_t)mm[j];
                 const uint8_t *pres = s_pres[op_tab(m)];
-                found[t.op0 + i] = (valid[j] && pres && !is_insert(m)) ? pres[rr[j]] : (uint8_t)1;
+                // 2 + table for a valid INSERT (the insert kernels read it
+                // back), a presence byte for other ops on append tables
+                found[t.op0 + i] = valid[j] && is_insert(m)   ? (uint8_t)(2u + op_tab(m))
+                                   : (valid[j] && pres)         ? pres[rr[j]]
+                                                                : (uint8_t)1;
             }
         }
 #pragma unroll
