@@ -295,11 +295,7 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
                 if (i >= t.nops) continue;
                 const uint16_t m = (uint16_t)mm[j];
                 const uint8_t *pres = s_pres[op_tab(m)];
-                // 2 + table for a valid INSERT (the insert kernels read it
-                // back), a presence byte for other ops on append tables
-                found[t.op0 + i] = valid[j] && is_insert(m)   ? (uint8_t)(2u + op_tab(m))
-                                   : (valid[j] && pres)         ? pres[rr[j]]
-                                                                : (uint8_t)1;
+                found[t.op0 + i] = (valid[j] && pres && !is_insert(m)) ? pres[rr[j]] : (uint8_t)1;
             }
         }
 #pragma unroll
@@ -326,14 +322,18 @@ __device__ inline uint32_t tile_validate_read(const DevTables &dt, const OpView 
     // a READ of an absent row gets zeros.  found[] of op i was written by this
     // same thread above (both loops give op i to thread i % blockDim.x); an
     // op of an aborted txn takes no effect, so found[] becomes presence AND
-    // commit, all the row-op kernels after this one need
+    // commit, all the row-op kernels after this one need.  An INSERT of a
+    // committed txn is valid (an invalid op aborts its txn): its byte becomes
+    // 2 + its table, which the insert kernels read back.
     for (uint32_t base = 0; base < t.nops; base += blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool take = false;
         if (i < t.nops) {
             const bool c = ok[owner[i]] != 0;
-            take = op_kind(ov.meta[t.op0 + i]) == LTPMG_OP_READ && c;
+            const uint16_t m = ov.meta[t.op0 + i];
+            take = op_kind(m) == LTPMG_OP_READ && c;
             if (found && !c) found[t.op0 + i] = 0;
+            else if (found && is_insert(m)) found[t.op0 + i] = (uint8_t)(2u + op_tab(m));
         }
         if (take && found && !found[t.op0 + i]) {
             for (uint32_t b = 0; b < res_stride; b += 4) st4(results + (uint64_t)(t.op0 + i) * res_stride + b, 0u);
