@@ -264,17 +264,14 @@ uint32_t key_col_stride(const Table &t, bool pad) { return pad ? 1u << ceil_log2
 // lane group mostly idles on the narrow columns that schemas like order entry
 // are made of (98 % of its 92 slots are at most 32 bytes, one is 256): 2
 // lanes per move took its ordered batch from 3.59 to 3.12 ms against 8 (1
-// lane: 3.74 ms, 4 lanes: 3.16 ms).  Schemas of wide columns keep 8, and so
-// does the lowest-TID-wins mode (its W = 64 batch took 1.05 ms with 2 lanes
+// lane: 3.74 ms, 4 lanes: 3.16 ms).  Tables of wide columns keep 8 (the
+// apply kernel chooses per tile, from the tables its keys span), and so does
+// the lowest-TID-wins mode (its W = 64 batch took 1.05 ms with 2 lanes
 // against 1.01 ms with 8).
-#ifndef LTPMG_WIDE_LANES
-#define LTPMG_WIDE_LANES 0  // 0: from the slots; else forced
-#endif
-uint32_t wide_move_lanes(const uint32_t *slots, uint32_t n) {
-    if (LTPMG_WIDE_LANES) return LTPMG_WIDE_LANES;
+bool narrow_columns(const uint32_t *slots, uint32_t n) {
     uint32_t narrow = 0;
     for (uint32_t i = 0; i < n; ++i) narrow += slots[i] <= 32u;
-    return n && narrow * 4 >= n * 3 ? 2u : 8u;
+    return n && narrow * 4 >= n * 3;
 }
 
 // Writes the device descriptors and assigns ordered-mode global row bases.
@@ -298,6 +295,7 @@ int sync_descriptors(ltpmg_engine *e) {
         kd.stride[kd.n] = stride;
         kd.shift[kd.n] = (stride & (stride - 1u)) == 0u ? (uint32_t)__builtin_ctz(stride) : 32u;
         kd.cpos[kd.n] = kd.n_cols_all;
+        kd.narrow |= (uint32_t)narrow_columns(tb.slot, tb.n_cols) << kd.n;
         kd.n_cols_all += tb.n_cols;
         kd.end_all = kd.end[kd.n];
         ++kd.n;
@@ -312,7 +310,7 @@ int sync_descriptors(ltpmg_engine *e) {
     std::vector<uint32_t> slots;
     for (const Table &tb : e->tables)
         if (tb.live) slots.insert(slots.end(), tb.slot, tb.slot + tb.n_cols);
-    e->wide_lanes = wide_move_lanes(slots.data(), (uint32_t)slots.size());
+    e->wide_lanes = narrow_columns(slots.data(), (uint32_t)slots.size()) ? 2u : 8u;
     HIP_TRY(e, hipMemcpy(e->d_tabs, tabs, sizeof tabs, hipMemcpyHostToDevice));
     return LTPMG_OK;
 }

@@ -216,6 +216,19 @@ struct KeyDecode {
     uint32_t shift[LTPMG_MAX_TABLES];   // log2 of its column stride, 32 if no power of two
     uint32_t stride[LTPMG_MAX_TABLES];  // its column stride
     uint32_t cpos[LTPMG_MAX_TABLES];    // position of its first column in the column list
+    uint32_t narrow;                    // bit i: live table i's columns are narrow (narrow_columns)
+    // the live table (index in key order) whose range holds the key
+    __device__ uint32_t index(uint32_t key) const {
+        uint32_t idx = 0;
+        for (uint32_t i = 1; i < n; ++i) idx = key >= end[i - 1] ? i : idx;
+        return idx;
+    }
+    // whether every live table whose range meets [lo, hi] is narrow
+    __device__ bool narrow_range(uint32_t lo, uint32_t hi) const {
+        const uint32_t i0 = index(lo), i1 = index(hi);
+        const uint32_t m = (i1 >= 31u ? ~0u : (2u << i1) - 1u) & ~((1u << i0) - 1u);
+        return (narrow & m) == m;
+    }
     // cell key -> (table, row, col) and the column's position in the column
     // list (DevTables::clist order); false past the last key
     __device__ bool operator()(uint32_t key, uint32_t &t, uint32_t &row, uint32_t &col,
@@ -253,14 +266,14 @@ struct DevTables {
     uint32_t n_clist;      // entries of clist in use
     KeyDecode dec;
     // lanes per ordered-mode cell move when the result stride exceeds 128
-    // bytes (2 or 8; wide_move_lanes)
+    // bytes, in the kernels that take one lane count for the whole batch:
+    // 2 when the live columns together are narrow (narrow_columns), else 8
     uint32_t wide_lanes;
 };
 
-// Lanes per cell move for result strides past 128 bytes, from the slots of
-// the live columns: 2 when at least three in four slots are at most 32
-// bytes, else 8.
-uint32_t wide_move_lanes(const uint32_t *slots, uint32_t n);
+// Whether columns with these slots are narrow: at least three in four
+// slots of at most 32 bytes.
+bool narrow_columns(const uint32_t *slots, uint32_t n);
 
 void launch_fill_column(hipStream_t s, ColDesc cd, uint32_t table, uint32_t col,
                         uint32_t n_rows, uint64_t seed);

@@ -404,7 +404,10 @@ __device__ inline void move_cells(uint32_t n, Fetch fetch, const DevTables &dt, 
 // ends here with a write, its entry {key, set, d} goes to the batch's
 // final-write list (at most one per tile, one global atomic each) for
 // k_mo_final_writes, which runs after every tile's reads.
-template <int G, bool VEC16, bool ONE, bool LCOLS>
+//
+// TILEG (for G = 8 past 128-byte strides): a tile whose keys lie in narrow
+// tables only (KeyDecode::narrow) moves its cells with 2 lanes each instead.
+template <int G, bool VEC16, bool ONE, bool LCOLS, bool TILEG = false>
 __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const uint32_t *tflag,
                                                            const uint32_t *tset,
                                                            const uint32_t *tdelta, DevTables dt,
@@ -478,19 +481,30 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
         if (wr && key[j] == cont_key) wlist[atomicAdd(wtotal, 1u)] = make_uint4(key[j], run.set, run.d, 0u);
     });
     __syncthreads();
-    move_cells<G, VEC16, ONE, LCOLS>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
-        uint4 e = list[k];
-        key_out = e.w;
-        st = St{e.y, e.z};
-        op = e.x;
-    }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / G, kBlock / G, lcols);
-    __syncthreads();  // every read of this tile's cells is done
-    move_cells<G, VEC16, ONE, LCOLS>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
-        uint4 e = list[kSegTile - 1 - k];
-        key_out = e.x;
-        st = St{e.y, e.z};
-        op = kCellOp;  // the cell itself
-    }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / G, kBlock / G, lcols);
+    auto moves = [&](auto lanes) {
+        constexpr int L = decltype(lanes)::value;
+        move_cells<L, VEC16, ONE && L == G, LCOLS>(n_rd, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
+            uint4 e = list[k];
+            key_out = e.w;
+            st = St{e.y, e.z};
+            op = e.x;
+        }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / L, kBlock / L, lcols);
+        __syncthreads();  // every read of this tile's cells is done
+        move_cells<L, VEC16, ONE && L == G, LCOLS>(n_wr, [&](uint32_t k, uint32_t &key_out, St &st, uint32_t &op) {
+            uint4 e = list[kSegTile - 1 - k];
+            key_out = e.x;
+            st = St{e.y, e.z};
+            op = kCellOp;  // the cell itself
+        }, dt, dec, src.args, results, res_stride, res_fill, threadIdx.x / L, kBlock / L, lcols);
+    };
+    if constexpr (TILEG) {
+        const uint32_t last = (src.n - t0 < kSegTile ? src.n : t0 + kSegTile) - 1u;
+        if (dec.narrow_range(src.keys[t0], src.keys[last])) {  // uniform
+            moves(std::integral_constant<int, 2>{});
+            return;
+        }
+    }
+    moves(std::integral_constant<int, G>{});
 }
 
 // The final writes of segments that span tiles, after every tile's reads:
@@ -1643,8 +1657,18 @@ void launch_mo_apply_reads(hipStream_t s, const DevTables &dt, OpView ov, OrderB
     if (n == 0) return;
     const uint32_t n_tiles = (n + kSegTile - 1) / kSegTile;
     OrderSrc src{ob.sort.keys[sorted], ob.sort.vals[sorted], ov.args, n};
-    // the column list in LDS when it holds every live column
-    if (dt.n_clist >= dt.dec.n_cols_all)
+    // the column list in LDS when it holds every live column; past 128-byte
+    // strides with narrow tables, lanes per move chosen per tile
+    if (res_stride > 128 && dt.dec.narrow != 0) {
+        if (dt.n_clist >= dt.dec.n_cols_all)
+            hipLaunchKernelGGL((k_mo_apply_reads<8, true, false, true, true>), dim3(n_tiles), dim3(kBlock), 0, s, src,
+                               ob.tile_flag, ob.tile_set, ob.tile_delta, dt, results, res_stride, res_fill, ob.wlist,
+                               ob.wtotal);
+        else
+            hipLaunchKernelGGL((k_mo_apply_reads<8, true, false, false, true>), dim3(n_tiles), dim3(kBlock), 0, s,
+                               src, ob.tile_flag, ob.tile_set, ob.tile_delta, dt, results, res_stride, res_fill,
+                               ob.wlist, ob.wtotal);
+    } else if (dt.n_clist >= dt.dec.n_cols_all)
         LTPMG_MO_DISPATCH(k_mo_apply_reads, true, dim3(n_tiles), src, ob.tile_flag, ob.tile_set, ob.tile_delta,
                           dt, results, res_stride, res_fill, ob.wlist, ob.wtotal);
     else
