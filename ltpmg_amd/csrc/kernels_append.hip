@@ -72,9 +72,11 @@ constexpr uint32_t kPresPerThread = 8;  // ops per thread per round, their loads
 // thread takes kPresPerThread ops a round (consecutive threads on
 // consecutive ops), issues every meta and row load, then every lock-word and
 // presence load, before it uses any.
+// Without a DELETE in the batch (has_delete false) no lock word can carry
+// this batch's epoch, so none is read.
 __global__ __launch_bounds__(kBlock) void k_row_presence(DevTables dt, OpView ov, const uint32_t *epoch_ctr,
                                                           uint8_t *found, uint16_t *meta_eff, uint8_t *results,
-                                                          uint32_t res_stride) {
+                                                          uint32_t res_stride, bool has_delete) {
     __shared__ RowTab rt[LTPMG_MAX_TABLES];
     load_row_tabs(dt, rt);
     __syncthreads();
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_row_presence(DevTables dt, OpView ov
             const bool ok = op_check(mj, row[j], t.n_rows, t.n_cols, t.present != nullptr);
             const bool look = ok && t.present && !is_insert(mj);
             pres[j] = look ? t.present[row[j]] : 1u;
-            lk[j] = look ? t.lock[row[j]] : 0ull;
+            lk[j] = look && has_delete ? t.lock[row[j]] : 0ull;
             m[j] |= (ok ? 0u : 0x10000u) | (look ? 0x20000u : 0u);  // bit 16: op_check failed; 17: looked up
         }
 #pragma unroll
@@ -384,7 +386,8 @@ void launch_row_presence_o(hipStream_t s, const DevTables &dt, OpView ov, RowBuf
     if (rb.has_delete)
         hipLaunchKernelGGL(k_row_first_delete, dim3(grid_for(ov.n_ops)), dim3(kBlock), 0, s, dt, ov, epoch_ctr);
     hipLaunchKernelGGL(k_row_presence, dim3(grid_for((ov.n_ops + kPresPerThread - 1) / kPresPerThread)),
-                       dim3(kBlock), 0, s, dt, ov, epoch_ctr, rb.found, rb.meta_eff, results, res_stride);
+                       dim3(kBlock), 0, s, dt, ov, epoch_ctr, rb.found, rb.meta_eff, results, res_stride,
+                       rb.has_delete);
 }
 
 void launch_row_deletes(hipStream_t s, const DevTables &dt, OpView ov, RowBufs &rb) {
