@@ -98,6 +98,32 @@ def test_row_ops_over_many_tiles(mode):
         _check_tables(e, im)
 
 
+@pytest.mark.parametrize("mode", BOTH)
+def test_inserts_into_six_append_tables_in_one_batch(mode):
+    # the insert writer ranks four tables per pass: six tables take two
+    # passes per 2048-op tile, over about 40k ops
+    rng = np.random.default_rng(13)
+    schema = {t: (6000 + 500 * t, [4, 8 + 4 * t, 32][: 1 + t % 3]) for t in range(6)}
+    parts = [random_row_batch(rng, 1500, 2000 + 200 * t, len(schema[t][1]), max_len=6, table=t, insert_pct=40,
+                              delete_pct=5) for t in range(6)]
+    txns = []
+    for p in parts:
+        for t in range(p.n_txns):
+            a, z = int(p.txn_off[t]), int(p.txn_off[t + 1])
+            txns.append([(int(p.meta[i] & 3), int((p.meta[i] >> 8) & 15), int((p.meta[i] >> 2) & 63),
+                          int(p.rows[i]), int(p.args[i])) for i in range(a, z)])
+    order = rng.permutation(len(txns))
+    b = Batch.from_txns([txns[i] for i in order])
+    assert b.n_ops > 4 * 2048
+    with Engine() as e:
+        im = _setup(e, schema, {t: 2000 + 200 * t for t in range(6)})
+        _, found = _run_compare(e, im, b, mode, pipeline=True)
+        is_ins = (b.meta & 0xFF) == OP_ROW
+        tabs = (b.meta >> 8) & 15
+        assert all(found[is_ins & (tabs == t)].sum() > 0 for t in range(6))
+        _check_tables(e, im)
+
+
 @pytest.mark.parametrize("mode,pipeline", PATHS, ids=PATH_IDS)
 def test_row_ops_with_ordinary_and_two_append_tables(mode, pipeline):
     rng = np.random.default_rng(11)
