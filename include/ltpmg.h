@@ -229,10 +229,12 @@ int ltpmg_submit_staged(ltpmg_engine *e, uint32_t slot, uint32_t flags);
  * caller writes only staging.txn_off; the batch's ops are the runs, each
  * n_ops consecutive ops of pool (rows / meta / args; pool->txn_off is not
  * read) copied to ops [dst_op, dst_op + n_ops) of the batch.  The runs must
- * tile the batch's ops in order.  Runs of 4096 ops or more are copied
- * straight from the pool by DMA, so a pool in pinned memory
- * (ltpmg_pinned_alloc) is uploaded without passing through the CPU; shorter
- * runs are gathered into the staging buffer first.  The ops are trusted as
+ * tile the batch's ops in order.  A batch of at most 8 runs out of a pool
+ * in pinned memory (ltpmg_pinned_alloc) is read by one kernel straight from
+ * host memory, with its offsets; otherwise runs of 4096 ops or more are
+ * copied straight from the pool by DMA and shorter runs are gathered into
+ * the staging buffer first.  Either way the ops do not pass through the
+ * CPU when the pool is pinned.  The ops are trusted as
  * with LTPMG_SUBMIT_TRUSTED (check the pool once with ltpmg_check_batch);
  * the offsets are checked.  Replaces the reference's per-batch pinned
  * allocation and upload (cuda/src/gpuquery.cu:2098-2146) for the serving

@@ -399,6 +399,20 @@ constexpr uint32_t kSmallMaxOps = LTPMG_SMALL_MAX_OPS;
 void launch_mo_small(hipStream_t s, const DevTables &dt, OpView ov, uint32_t key_bits, uint8_t *results,
                      uint32_t res_stride, uint8_t *commit, const RowBufs *rb);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t v);
+// A batch's ops and offsets read by a kernel straight out of pinned host
+// memory (device-visible pointers): up to kGatherRuns runs of pool ops,
+// each to its place in the batch, and the staged offsets.
+constexpr uint32_t kGatherRuns = 8;
+struct GatherRuns {
+    const uint32_t *rows;
+    const uint16_t *meta;
+    const uint64_t *args;
+    const uint32_t *txn_off;
+    uint32_t src[kGatherRuns], dst[kGatherRuns];
+    uint32_t n_runs, n_ops, n_txns;
+};
+void launch_gather_runs(hipStream_t s, const GatherRuns &g, uint32_t *txn_off, uint32_t *rows, uint16_t *meta,
+                        uint64_t *args);
 void launch_selftest_segscan(hipStream_t s, const uint32_t *vals, const uint8_t *head,
                              uint32_t *out, uint32_t n, uint32_t *tile_scratch);
 
