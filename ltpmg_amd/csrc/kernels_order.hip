@@ -116,7 +116,7 @@ __device__ inline Seg tile_carry(const uint32_t *tflag, const uint32_t *tset, co
     for (uint32_t top = tile; top > 0;) {  // lane l reads tile top - 1 - l
         const bool in = lane < top;
         const uint32_t j = in ? top - 1 - lane : 0u;
-        Seg x = in ? Seg{tflag[j], tset[j], tdelta[j]} : Seg{0u, kNoSet, 0u};
+        Seg x = in ? Seg{tflag[j] & 1u, tset[j], tdelta[j]} : Seg{0u, kNoSet, 0u};
         const bool found = __ballot(in && x.f != 0u) != 0ull;
         // fold lanes 63 .. 0 (oldest first): lane l ends with its own tile
         // and the older ones above it
@@ -232,10 +232,22 @@ __global__ __launch_bounds__(kBlock) void k_seg_reduce(OrderSrc src, uint32_t *t
     const uint32_t first = blockIdx.x * kSegTile + threadIdx.x * kSegItems;
     uint32_t key[kSegItems], val[kSegItems], prev;
     load8(src, first, key, val, prev);
+    // bit 1 of the tile's flag: the apply kernel has work here -- a valid
+    // element, or the end of a segment (whose final write may fall to this
+    // tile).  A tile of invalid elements inside one key's run (row ops and
+    // ops on absent rows sort to key 0 together) has none.
+    bool act = false;
+#pragma unroll
+    for (uint32_t j = 0; j < kSegItems; ++j) act |= !(val[j] & kInvalid);
+    if (threadIdx.x == kBlock - 1) {
+        const uint32_t end = blockIdx.x * kSegTile + kSegTile;  // one past the tile
+        act |= end >= src.n || key[kSegItems - 1] != src.keys[end];
+    }
+    act = __syncthreads_or(act);
     Seg tot;
     block_seg_exclusive(agg8(src, first, key, val, prev), &tot);
     if (threadIdx.x == 0) {
-        tflag[blockIdx.x] = tot.f;
+        tflag[blockIdx.x] = tot.f | (act ? 2u : 0u);
         tset[blockIdx.x] = tot.set;
         tdelta[blockIdx.x] = tot.d;
     }
@@ -422,6 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_apply_reads(OrderSrc src, const u
     // bytes before any entry is written.
     __shared__ uint4 list[kSegTile];
     __shared__ ColDesc lcols[kColList];  // the column list: no global load per cell for its descriptor
+    if (!(tflag[blockIdx.x] & 2u)) return;  // uniform: no work in this tile (k_seg_reduce)
     if (LCOLS)
         for (uint32_t i = threadIdx.x; i < dt.n_clist; i += kBlock) lcols[i] = dt.clist[i];
     const KeyDecode &dec = dt.dec;
